@@ -1,0 +1,199 @@
+/*
+ * sdem.h -- C ABI of libsdem_hip.so, the MI355X (gfx950) implementation of
+ * sDEM's per-timestep DEM path.
+ *
+ * The library replaces the body of DEMS_iterate (src/DEM/Prtcl_DEMSystem.f90:201-317)
+ * on one GPU: near-wall list upkeep, contact search (NBS-Munjiza grid or the
+ * hierarchical multi-level variant), the persistent contact list with its
+ * tangential history, linear (LSD) and Hertz-Mindlin (nLin) particle-particle,
+ * particle-ghost, particle-fixed and particle-wall forces, accelerations,
+ * FE/AB2/AB3 integration, and the periodic wrap / open-boundary deletion.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative SDEM_E_* code on failure; the message
+ *     is then available from sdem_last_error();
+ *   - host arrays are owned by the caller and copied before return; no
+ *     pointer is retained;
+ *   - a context is used by one host thread at a time;
+ *   - array layouts follow the Fortran arrays of the reference so a Fortran
+ *     host can pass them through ISO_C_BINDING unchanged:
+ *       posr      : n x 4 doubles   (GPrtcl_PosR, real4 x,y,z,w=radius)
+ *       lin_vel   : n x tsize x 3   (GPrtcl_linVel(tsize,n), real3)
+ *       lin_acc   : n x tsize x 3
+ *       theta     : n x 3
+ *       rot_vel   : n x rsize x 3,  rot_acc : n x rsize x 3
+ *     where tsize/rsize = 1,2,3 for FE/AB2/AB3 (Prtcl_Variables.f90:100-113).
+ */
+#ifndef SDEM_H
+#define SDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes -------------------------------------------------------- */
+#define SDEM_OK           0
+#define SDEM_E_ARG       -1  /* invalid argument or configuration            */
+#define SDEM_E_HIP       -2  /* a HIP runtime call failed                      */
+#define SDEM_E_CAPACITY  -3  /* particle / contact-history capacity exceeded   */
+#define SDEM_E_STATE     -4  /* call order violated (e.g. step before state)   */
+#define SDEM_E_NODEV     -5  /* no usable gfx950 device                        */
+
+/* ---- enumerations, same values as Prtcl_Parameters.f90:13-24 ------------- */
+#define SDEM_CSM_NBS_MUNJIZA        1
+#define SDEM_CSM_NBS_MUNJIZA_HRCHL  2
+#define SDEM_CFT_LSD   1
+#define SDEM_CFT_NLIN  2
+#define SDEM_CTM_CONSTANT_TORQUE  1
+#define SDEM_CTM_VARIABLE_TORQUE  2
+#define SDEM_PIM_FE   1
+#define SDEM_PIM_AB2  2
+#define SDEM_PIM_AB3  3
+
+/* contact kinds, the four counters of ContactList%numCntcts
+ * (Prtcl_CL_and_CF.f90:156,187,214,242) */
+#define SDEM_KIND_PP     1
+#define SDEM_KIND_PPG    2
+#define SDEM_KIND_PPFIX  3
+#define SDEM_KIND_PW     4
+
+/* profiling phases reported by sdem_get_profile */
+#define SDEM_PHASE_WALL_LIST  0   /* UpdateNearPrtclsPW                         */
+#define SDEM_PHASE_BIN        1   /* cell keys + counting sort                  */
+#define SDEM_PHASE_STEP       2   /* fused contact/force/accel/integrate kernel */
+#define SDEM_NUM_PHASES       3
+
+/*
+ * The DEMS_Options fields the step reads (Prtcl_Parameters.f90:45-95,
+ * namelist &DEMOptions read at :120-124).  Doubles first, then 32-bit
+ * integers, so the layout is the same as a BIND(C) Fortran derived type.
+ */
+typedef struct sdem_options {
+  double  dt;                    /* dtDEM                                      */
+  double  gravity[3];            /* gravity                                    */
+  double  domain_min[3];         /* minpoint  (SimDomain_min)                  */
+  double  domain_max[3];         /* maxpoint  (SimDomain_max)                  */
+  double  prtcl_cs_ratio;        /* Prtcl_cs_ratio (cell = ratio * d_max)      */
+  double  wall_neighbor_ratio;   /* Wall_neighbor_ratio                        */
+  int32_t is_periodic[3];        /* isPeriodic (0/1)                           */
+  int32_t cs_method;             /* CS_Method   (SDEM_CSM_*)                   */
+  int32_t cf_type;               /* CF_Type     (SDEM_CFT_*)                   */
+  int32_t ct_model;              /* CT_Model    (SDEM_CTM_*)                   */
+  int32_t pi_method;             /* PI_Method   (SDEM_PIM_*)                   */
+  int32_t pri_method;            /* PRI_Method  (SDEM_PIM_*)                   */
+  int32_t cs_numlvls;            /* CS_numlvls  (0 = levels from d_max/d_min)  */
+  int32_t cntct_list_size;       /* CntctList_Size: history slots / particle   */
+  int32_t base_wall_id;          /* Base_wall_id                               */
+  int32_t wall_max_update_iter;  /* Wall_max_update_iter                       */
+  int32_t num_prtcl;             /* numPrtcl: particle capacity                */
+  int32_t ifirst;                /* ifirstDEM: iterNumber starts at ifirst-1   */
+} sdem_options;
+
+typedef struct sdem_ctx sdem_ctx;
+
+/* Fill *o with the defaults of DEMS_Options (Prtcl_Parameters.f90:45-95). */
+int sdem_options_default(sdem_options* o);
+
+/* Create a context on the current HIP device. Replaces the DEM_opt setup of
+ * DO_ReadDEMOption (Prtcl_Parameters.f90:104-205); everything the MPI
+ * decomposition computed is replaced by the whole domain on one GPU. */
+int sdem_create(const sdem_options* o, sdem_ctx** out);
+
+/* InitPrtclProperty (Prtcl_Property.f90:62-143): per-type pure properties
+ * and the particle-particle binary table, built with the reference's
+ * row-type rule (pair (i,j) takes both partners' properties from type i,
+ * Prtcl_Property.f90:137-138). Arrays have ntype entries (type ids 1..ntype). */
+int sdem_set_particle_property(sdem_ctx* c, int32_t ntype,
+                               const double* diameter, const double* density,
+                               const double* youngs_modulus, const double* poisson_ratio,
+                               double fric_s, double fric_k, double fric_roll,
+                               double rest_n, double rest_t);
+
+/* InitWallProperty (Prtcl_Property.f90:148-195): wall pure properties and the
+ * particle-wall binary table. Call after sdem_set_particle_property. */
+int sdem_set_wall_property(sdem_ctx* c, int32_t nwtype,
+                           const double* youngs_modulus, const double* poisson_ratio,
+                           double fric_s, double fric_k, double fric_roll,
+                           double rest_n, double rest_t);
+
+/* G_add_PlaneWall (Prtcl_Geometry.f90:457-537): adds one plane quad; normal,
+ * offset and bounding box are derived as the reference derives them.
+ * wall_id = Base_wall_id + running count. Walls outside the (expanded)
+ * domain are skipped as in the reference (returns 0, *wall_id = 0). */
+int sdem_add_plane_wall(sdem_ctx* c, const double p1[3], const double p2[3],
+                        const double p3[3], const double p4[3],
+                        int32_t user_id, int32_t wall_type,
+                        int32_t both_side, int32_t is_infinite,
+                        const double trans_vel[3], int32_t* wall_id);
+
+/* Fixed particles (PIO_ReadFixedCoord, Prtcl_IOAndVisu.f90:164-291): ids
+ * must not collide with moving ids (the reference uses numPrtcl+k). */
+int sdem_set_fixed_particles(sdem_ctx* c, int32_t nfix, const int32_t* id,
+                             const int32_t* ptype, const double* posr);
+
+/* Particle state (what Read_Restart / MakingAllPrtcl produce,
+ * Prtcl_IOAndVisu.f90:731-844).  usr_mark may be NULL. */
+int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* ptype,
+                   const int32_t* usr_mark, const double* posr,
+                   const double* lin_vel, const double* lin_acc, const double* theta,
+                   const double* rot_vel, const double* rot_acc);
+
+/* Contact-history records (what RestartCL / Add_RestartCntctlink install,
+ * Prtcl_CL_and_CF.f90:577-622).  A record belongs to owner_id's bucket:
+ * the lower-id particle for PP, the local particle for PPG/PPFix/PW.
+ * partner_id is a particle id, fixed-particle id or wall id. */
+int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind,
+                      const int32_t* owner_id, const int32_t* partner_id,
+                      const double* tan_delta, const double* vrel_init);
+
+/* nsteps x DEMS_iterate (Prtcl_DEMSystem.f90:201-248) without I/O or logs.
+ * Asynchronous w.r.t. the host only in the sense that it returns once the
+ * work is queued and checked; it synchronises before returning. */
+int sdem_step(sdem_ctx* c, int32_t nsteps);
+
+/* Number of particles in the domain (np_InDomain, Prtcl_DEMSystem.f90:252). */
+int sdem_get_num_particles(sdem_ctx* c, int32_t* n);
+
+/* State in ascending id order; cap = capacity of the arrays in particles.
+ * Any output pointer may be NULL. */
+int sdem_get_state(sdem_ctx* c, int32_t cap, int32_t* n,
+                   int32_t* id, int32_t* ptype, int32_t* usr_mark, double* posr,
+                   double* lin_vel, double* lin_acc, double* theta,
+                   double* rot_vel, double* rot_acc);
+
+/* GPrtcl_cntctForce / GPrtcl_torque of the last step, ascending id order. */
+int sdem_get_forces(sdem_ctx* c, int32_t cap, int32_t* id,
+                    double* cntct_force, double* torque);
+
+/* Contact records after the last step, sorted by (owner_id, partner_id). */
+int sdem_get_num_contacts(sdem_ctx* c, int64_t* m);
+int sdem_get_contacts(sdem_ctx* c, int64_t cap, int64_t* m, int32_t* kind,
+                      int32_t* owner_id, int32_t* partner_id,
+                      double* tan_delta, double* vrel_init);
+
+/* numCntcts(1:4) of the last step: P-P | P-GP | P-FP | P-W
+ * (Prtcl_DEMSystem.f90:313). */
+int sdem_get_counts(sdem_ctx* c, int32_t num_cntcts[4]);
+
+/* iterNumber and the near-wall list's Next_iter_update
+ * (Prtcl_ContactSearchPW.f90:29,229). */
+int sdem_get_iter(sdem_ctx* c, int32_t* iter_number, int32_t* next_wall_update);
+
+/* Per-phase kernel timing with HIP events on the library's stream.
+ * enable=1 starts collecting (resets totals); totals are in milliseconds. */
+int sdem_set_profiling(sdem_ctx* c, int32_t enable);
+int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
+                     int64_t launches[SDEM_NUM_PHASES]);
+
+/* Memory the context holds on the device, in bytes. */
+int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);
+
+const char* sdem_last_error(const sdem_ctx* c);
+void sdem_destroy(sdem_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDEM_H */

@@ -1,0 +1,109 @@
+// sdem_internal.h -- device-side data layout shared by the kernels and the
+// C-ABI implementation of libsdem_hip.so.
+//
+// Particle state lives in two SoA buffers (A = state at the start of a step,
+// B = state at its end).  The fused step kernel reads A through the cell-sort
+// permutation and writes B in cell order, so every step leaves the particles
+// stored in cell order for the next one.  Contact history is held per owner
+// particle in CAP slots next to the particle (owner = the particle whose
+// bucket holds the record in the reference: the lower id for PP, the local
+// particle for PPG / PPFix / PW; Prtcl_CL_and_CF.f90:142-249).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdem {
+
+// partner key: id in the low 29 bits, contact kind (1..4) above it
+constexpr int KEY_BITS = 29;
+constexpr int KEY_MASK = (1 << KEY_BITS) - 1;
+constexpr int MAX_WALLS = 64;     // walls seen by one context (near-wall bit mask)
+constexpr int MAX_TYPES = 64;
+
+// binary (pair) coefficients, Prtcl_Property.f90:28-41
+struct Bnry {
+  double rad_eff, mass_eff, kn, kt, dn, fric_s, fric_k, fric_roll;
+};
+
+// plane wall, Prtcl_Geometry.f90:17-37 (+ derived quantities)
+struct Wall {
+  double p1[3], p2[3], p3[3], p4[3];
+  double n[3], d;
+  double min_point[3], max_point[3];
+  double trans_vel[3];
+  int both_side, is_infinite, wall_type, wall_id;
+};
+
+// uniform cell grid of one search level (x fastest).  A periodic dimension
+// tiles its period with n cells of width L/n; a closed one starts at the
+// domain minimum.  Cell indices are clamped to [0,n) for stored particles.
+struct Grid {
+  double origin[3];
+  double inv_w[3];
+  double w[3];
+  int n[3];
+  int64_t ncell;
+  int64_t key_base;   // first key of this level in the combined key space
+  double rmax;        // largest radius among this level's particles
+};
+constexpr int MAX_LEVELS = 6;
+
+// one SoA particle buffer
+struct Buf {
+  double *x, *y, *z, *r;
+  double *lv;     // (tsize*3) x cap  : lv[(l*3+c)*cap + i]
+  double *la;     // (ta*3) x cap     : stored acceleration levels
+  double *th;     // 3 x cap
+  double *rv;     // (rsize*3) x cap
+  double *ra;     // (tr*3) x cap
+  int *id, *type, *mark;
+  uint64_t *wmask;  // near-wall list as a bit mask over wall indices
+  int *hcnt;        // history records owned by the particle
+  int *hkey;        // cap*hcap partner keys
+  double4 *hval;    // cap*hcap (TanDelta.x,.y,.z, VelRel_Init)
+  double *force;    // 6 x cap (F, T) when forces are kept
+};
+
+// everything the kernels read that does not change inside a step
+struct Params {
+  double dt;
+  double g[3];
+  double dmin[3], dmax[3], simlen[3];
+  int periodic[3];
+  int cf_type, ct_model, pi_method, pri_method;
+  int tsize, rsize, ta, tr;   // velocity / stored acceleration levels
+  int cap;                    // particle capacity (stride of SoA arrays)
+  int hcap;                   // history slots per particle
+  int ntype, nwtype, nwall;
+  int search_fixed;           // 0 when CS_Method=2 (hierarchical search has no fixed branch)
+  int keep_forces;
+  int nlvl;                   // number of search levels (1 = single grid)
+  Grid lvl[MAX_LEVELS];
+  const int* type_level;      // ntype: search level of each particle type
+  Grid fgrid;                 // grid of the fixed particles
+  int nfix;
+  double dxmax1, dxmax2, max_wall_vel;
+  int wall_max_update_iter;
+  const Bnry* pp;             // ntype*ntype, row = type of the record's lower-id particle
+  const Bnry* pw;             // ntype*nwtype
+  const double* mass;         // ntype (1-based types -> index t-1)
+  const double* inertia;
+  const Wall* walls;          // nwall
+};
+
+// device-side scalars of one context
+struct Scalars {
+  int n_stored;            // entries of buffer A at the start of the step
+  int n_alive;             // live particles after this step's sort
+  int n_deleted;           // particles removed at the end of the last step
+  int iter;                // iterNumber
+  int next_wall_update;    // Next_iter_update
+  int overflow;            // history slots exhausted (first failing step+1)
+  int num_cntcts[4];
+  unsigned long long max_v_bits, max_a_bits;
+  int use_host_max_a;      // first step after set_state: max|a| from host copy
+  double host_max_a;
+};
+
+}  // namespace sdem

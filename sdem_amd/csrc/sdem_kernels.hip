@@ -1,0 +1,767 @@
+// sdem_kernels.hip -- gfx950 kernels of one DEM time step.
+//
+// A step is a short pipeline on one stream:
+//   k_begin        1 thread : counters for this step
+//   k_wall_list    N        : near-wall bit masks + max|v|, max|a| (only on
+//                             update iterations, UpdateNearPrtclsPW)
+//   k_wall_cadence 1 thread : Next_iter_update
+//   k_bin          N        : (level, cell) key per particle, per-cell counts
+//   scan                    : exclusive scan of the counts (rocPRIM)
+//   k_scatter / k_rank      : deterministic counting sort (in-cell order =
+//                             storage order) + packed (x,y,z,r) in cell order
+//   k_step         N        : fused contact search, history match, contact
+//                             forces, accelerations, integration, wrap/delete
+//
+// All floating-point kernels are built with -ffp-contract=off so that every
+// product and sum is rounded on its own, in the order the reference's
+// operator functions impose (Prtcl_TypeDef.f90:105-288).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "sdem_internal.h"
+
+namespace sdem {
+
+// ---------------------------------------------------------------------------
+// real3 arithmetic in the reference's evaluation order (Prtcl_TypeDef.f90)
+// ---------------------------------------------------------------------------
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 D3(double x, double y, double z) {
+  d3 r;
+  r.x = x; r.y = y; r.z = z;
+  return r;
+}
+__device__ __forceinline__ d3 add3(d3 a, d3 b) { return D3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ d3 sub3(d3 a, d3 b) { return D3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ d3 mul3(d3 a, double v) { return D3(a.x * v, a.y * v, a.z * v); }
+__device__ __forceinline__ d3 div3(d3 a, double v) { return D3(a.x / v, a.y / v, a.z / v); }
+__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 cross3(d3 u, d3 v) {
+  return D3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+__device__ __forceinline__ double norm3(d3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+// .nv. : unit(p1-p2), zero when |p1-p2| <= 1e-10 (Prtcl_TypeDef.f90:224-281)
+__device__ __forceinline__ d3 nv3(d3 p1, d3 p2) {
+  d3 s = sub3(p1, p2);
+  double n = norm3(s);
+  if (n > 1.0e-10) return div3(s, n);
+  return D3(0.0, 0.0, 0.0);
+}
+__device__ __forceinline__ d3 ld3(const double* a, int cap, int i) {
+  return D3(a[i], a[cap + i], a[2 * cap + i]);
+}
+__device__ __forceinline__ void st3(double* a, int cap, int i, d3 v) {
+  a[i] = v.x;
+  a[cap + i] = v.y;
+  a[2 * cap + i] = v.z;
+}
+
+// ---------------------------------------------------------------------------
+// Prtcl_ContactForce_inc.f90:1-68 for one contact record
+// ---------------------------------------------------------------------------
+struct CF {
+  d3 fsum, moment, mrij, tandel;
+  double vri;
+};
+
+__device__ __forceinline__ CF contact_force(const Params& P, const Bnry& B, d3 norm_v, double ri,
+                                            double rj, d3 veli, d3 velj, d3 rvei, d3 rvej,
+                                            double ovrlp, d3 tandel, double vri) {
+  const double dt = P.dt;
+  d3 vel_w = cross3(add3(mul3(rvei, ri), mul3(rvej, rj)), norm_v);
+  d3 vrij = add3(sub3(veli, velj), vel_w);
+  double vrn = dot3(vrij, norm_v);
+  if (vri < -1.0) vri = fabs(vrn);
+  d3 vij_n = mul3(norm_v, vrn);
+  d3 vij_t = sub3(vrij, vij_n);
+
+  d3 ot = tandel;
+  double n1 = norm3(ot);
+  ot = sub3(ot, mul3(norm_v, dot3(ot, norm_v)));
+  double n2 = norm3(ot);
+  if (n2 > 1.0e-10) ot = mul3(ot, n1 / n2);
+  else ot = D3(0.0, 0.0, 0.0);
+  ot = add3(mul3(vij_t, dt), ot);
+
+  double fn, k_t;
+  const double d_t = 0.0;
+  d3 ftij;
+  if (P.cf_type == 1) {  // CFT_LSD
+    double vel_in = vri > 0.01 ? vri : 0.01;
+    double k_n = -(B.kn * pow(vel_in, 0.4));
+    double d_n = -(B.dn * pow(vel_in, 0.2));
+    fn = k_n * ovrlp + d_n * vrn;
+    k_t = -(B.kt * pow(vel_in, 0.4));
+  } else {               // CFT_nLin
+    double k_n = -B.kn;
+    double d_n = -B.dn;
+    fn = k_n * pow(ovrlp, 1.5) + d_n * pow(ovrlp, 0.25) * vrn;
+    k_t = -(B.kt * sqrt(ovrlp));
+  }
+  ftij = add3(mul3(ot, k_t), mul3(vij_t, d_t));
+  d3 fnij = mul3(norm_v, fn);
+
+  double ft = norm3(ftij);
+  double ft_fric = B.fric_s * fabs(fn);
+  if (ft > ft_fric) {
+    ft_fric = B.fric_k * fabs(fn);
+    ftij = mul3(ftij, ft_fric / ft);
+    ot = div3(ftij, k_t);
+  }
+
+  d3 w_hat = sub3(rvei, rvej);
+  double wm = norm3(w_hat);
+  if (wm > 1.0e-10) w_hat = div3(w_hat, wm);
+  else w_hat = D3(0.0, 0.0, 0.0);
+  d3 mrij;
+  if (P.ct_model == 1)
+    mrij = mul3(w_hat, -(B.fric_roll * fabs(fn) * B.rad_eff));
+  else
+    mrij = mul3(w_hat, -(B.fric_roll * fabs(fn) * B.rad_eff * norm3(vel_w)));
+
+  CF o;
+  o.tandel = ot;
+  o.vri = vri;
+  o.moment = cross3(norm_v, ftij);
+  o.fsum = add3(fnij, ftij);
+  o.mrij = mrij;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// PW_isInContact and helpers (Prtcl_Geometry.f90:55-109, 185-261)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ d3 wp(const double* p) { return D3(p[0], p[1], p[2]); }
+
+__device__ bool pw_in_plane(d3 p1, d3 p2, d3 p3, d3 p4, d3 cp) {
+  d3 p1p = sub3(p1, cp), p2p = sub3(p2, cp), p3p = sub3(p3, cp), p4p = sub3(p4, cp);
+  double p1p3 = dot3(p1p, p3p), p2p4 = dot3(p2p, p4p), p1p4 = dot3(p1p, p4p), p2p3 = dot3(p2p, p3p);
+  if (p1p3 * p2p4 - p1p4 * p2p3 < 0.0) return false;
+  double p1p2 = dot3(p1p, p2p), p3p4 = dot3(p3p, p4p);
+  if (p1p3 * p2p4 - p1p2 * p3p4 < 0.0) return false;
+  double p2p2 = dot3(p2p, p2p);
+  if (p1p2 * p2p3 - p1p3 * p2p2 < 0.0) return false;
+  return true;
+}
+
+__device__ bool line_point_check(d3 lp1, d3 lp2, d3 pos, double r, d3* nv, double* ovrlp) {
+  d3 w = sub3(pos, lp1);
+  d3 v = sub3(lp2, lp1);
+  double length = norm3(sub3(lp1, lp2));
+  double t = dot3(w, v) / dot3(v, v);
+  d3 cp;
+  if (t >= 0.0 && t <= 1.0) {
+    cp = add3(mul3(v, t), lp1);
+  } else if (t >= (-r / length) && t < 0.0) {
+    cp = lp1;
+  } else if (t > 1.0 && t >= (1.0 + r / length)) {  // as written in the reference
+    cp = lp2;
+  } else {
+    return false;
+  }
+  double ov = r - norm3(sub3(cp, pos));
+  if (ov >= 0.0) {
+    *ovrlp = ov;
+    *nv = nv3(pos, cp);
+    return true;
+  }
+  return false;
+}
+
+__device__ bool pw_in_contact(const Wall& W, d3 p, double radius, double* ovrlp, d3* nv) {
+  d3 n = wp(W.n);
+  double dist = dot3(n, p) + W.d;
+  double ov;
+  if (W.both_side) {
+    ov = radius - fabs(dist);
+  } else {
+    if (dist < 0.0) return false;
+    ov = radius - dist;
+  }
+  if (ov < 0.0) return false;
+  if (W.is_infinite) {
+    *ovrlp = ov;
+    *nv = mul3(n, copysign(1.0, dist));
+    return true;
+  }
+  double t = -dist;
+  d3 cp = add3(mul3(n, t), p);
+  d3 p1 = wp(W.p1), p2 = wp(W.p2), p3 = wp(W.p3), p4 = wp(W.p4);
+  if (pw_in_plane(p1, p2, p3, p4, cp)) {
+    *ovrlp = ov;
+    *nv = mul3(n, copysign(1.0, dist));
+    return true;
+  }
+  if (line_point_check(p1, p2, p, radius, nv, ovrlp)) return true;
+  if (line_point_check(p2, p3, p, radius, nv, ovrlp)) return true;
+  if (line_point_check(p3, p4, p, radius, nv, ovrlp)) return true;
+  if (line_point_check(p4, p1, p, radius, nv, ovrlp)) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// grid helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cell_raw(const Grid& G, int d, double x) {
+  return (int)floor((x - G.origin[d]) * G.inv_w[d]);
+}
+__device__ __forceinline__ int cell_clamped(const Grid& G, int d, double x) {
+  int c = cell_raw(G, d, x);
+  return c < 0 ? 0 : (c >= G.n[d] ? G.n[d] - 1 : c);
+}
+__device__ __forceinline__ int64_t cell_key(const Grid& G, double x, double y, double z) {
+  int cx = cell_clamped(G, 0, x), cy = cell_clamped(G, 1, y), cz = cell_clamped(G, 2, z);
+  return G.key_base + ((int64_t)cz * G.n[1] + cy) * G.n[0] + cx;
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_begin(Scalars* S) {
+  S->n_stored = S->n_alive;
+  for (int k = 0; k < 4; k++) S->num_cntcts[k] = 0;
+  S->n_deleted = 0;
+}
+
+// UpdateNearPrtclsPW (Prtcl_ContactSearchPW.f90:159-235), particle part
+__global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) {
+  if (S->iter < S->next_wall_update) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double mv = 0.0, ma = 0.0;
+  if (s < S->n_stored) {
+    double x = A.x[s], y = A.y[s], z = A.z[s];
+    uint64_t mask = 0;
+    for (int w = 0; w < P.nwall; w++) {
+      const Wall& W = P.walls[w];
+      double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2, mnz = W.min_point[2] - P.dxmax2;
+      double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2, mxz = W.max_point[2] + P.dxmax2;
+      if (x >= mnx && x <= mxx && y >= mny && y <= mxy && z >= mnz && z <= mxz) mask |= (1ull << w);
+    }
+    A.wmask[s] = mask;
+    mv = norm3(ld3(A.lv, P.cap, s));
+    ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
+  }
+  // wave then block max (non-negative doubles order like their bit patterns)
+  for (int off = 32; off > 0; off >>= 1) {
+    mv = fmax(mv, __shfl_down(mv, off, 64));
+    ma = fmax(ma, __shfl_down(ma, off, 64));
+  }
+  __shared__ double smv[4], sma[4];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { smv[wid] = mv; sma[wid] = ma; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) { mv = fmax(mv, smv[k]); ma = fmax(ma, sma[k]); }
+    atomicMax(&S->max_v_bits, (unsigned long long)__double_as_longlong(mv));
+    atomicMax(&S->max_a_bits, (unsigned long long)__double_as_longlong(ma));
+  }
+}
+
+// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances
+__global__ void k_wall_cadence(Params P, Scalars* S) {
+  if (S->iter >= S->next_wall_update) {
+    double max_v = __longlong_as_double((long long)S->max_v_bits) + P.max_wall_vel;
+    max_v = max_v > 0.3 ? max_v : 0.3;
+    max_v = max_v * 1.5;
+    double max_a = S->use_host_max_a ? S->host_max_a : __longlong_as_double((long long)S->max_a_bits);
+    double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
+    max_a = max_a > gn ? max_a : gn;
+    max_a = max_a * 1.5;
+    int num_iter;
+    if (max_a > 0.0) {
+      double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
+      double q = t / P.dt;
+      num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
+    } else {
+      num_iter = P.wall_max_update_iter;
+    }
+    if (num_iter < 1) num_iter = 1;
+    if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
+    S->next_wall_update = S->iter + num_iter;
+    S->max_v_bits = 0ull;
+    S->max_a_bits = 0ull;
+  }
+  S->use_host_max_a = 0;
+}
+
+// (level, cell) key and per-key counts; dead particles go to the last key
+__global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_t dead_key,
+                                             int* __restrict__ keys, int* __restrict__ rank,
+                                             int* __restrict__ counts) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S->n_stored) return;
+  int t = A.type[s];
+  int64_t key;
+  if (t < 1) {
+    key = dead_key;
+  } else {
+    const Grid& G = P.lvl[P.type_level[t - 1]];
+    key = cell_key(G, A.x[s], A.y[s], A.z[s]);
+  }
+  keys[s] = (int)key;
+  rank[s] = atomicAdd(&counts[key], 1);
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __restrict__ keys,
+                                                 const int* __restrict__ rank,
+                                                 const int* __restrict__ start,
+                                                 int* __restrict__ perm_tmp) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S->n_stored) return;
+  perm_tmp[start[keys[s]] + rank[s]] = s;
+}
+
+// Make the in-cell order the storage order (deterministic) and pack the
+// positions in cell order.  The number of live particles is start[dead_key].
+__global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_key,
+                                              const int* __restrict__ keys,
+                                              const int* __restrict__ start,
+                                              const int* __restrict__ perm_tmp,
+                                              int* __restrict__ perm, double4* __restrict__ pk) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) S->n_alive = start[dead_key];
+  if (k >= S->n_stored) return;
+  int s = perm_tmp[k];
+  int key = keys[s];
+  int a = start[key], b = start[key + 1];
+  int r = 0;
+  for (int j = a; j < b; j++) r += (perm_tmp[j] < s);
+  perm[a + r] = s;
+  pk[a + r] = make_double4(A.x[s], A.y[s], A.z[s], A.r[s]);
+}
+
+// ---------------------------------------------------------------------------
+// the fused step kernel
+// ---------------------------------------------------------------------------
+struct Hist {
+  d3 td;
+  double vri;
+};
+
+__device__ __forceinline__ Hist find_hist(const Buf& A, int hcap, int owner_s, int partner) {
+  Hist h;
+  h.td = D3(0.0, 0.0, 0.0);
+  h.vri = -2.0;
+  int c = A.hcnt[owner_s];
+  int base = owner_s * hcap;
+  for (int k = 0; k < c; k++) {
+    if ((A.hkey[base + k] & KEY_MASK) == partner) {
+      double4 v = A.hval[base + k];
+      h.td = D3(v.x, v.y, v.z);
+      h.vri = v.w;
+      break;
+    }
+  }
+  return h;
+}
+
+struct Acc {
+  d3 F, T;
+  int hc;
+  int nkind[4];
+  int overflow;
+};
+
+__device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, Acc& acc, int kind,
+                                          int partner, const CF& cf) {
+  if (acc.hc < P.hcap) {
+    int slot = k * P.hcap + acc.hc;
+    B.hkey[slot] = partner | (kind << KEY_BITS);
+    B.hval[slot] = make_double4(cf.tandel.x, cf.tandel.y, cf.tandel.z, cf.vri);
+    acc.hc++;
+  } else {
+    acc.overflow = 1;
+  }
+  acc.nkind[kind - 1]++;
+}
+
+__global__ __launch_bounds__(256) void k_step(Params P, Buf A, Buf B, Scalars* S,
+                                              const int* __restrict__ perm,
+                                              const double4* __restrict__ pk,
+                                              const int* __restrict__ cstart,
+                                              const double4* __restrict__ fpk,
+                                              const int* __restrict__ fid,
+                                              const int* __restrict__ ftype,
+                                              const int* __restrict__ fstart) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = S->n_alive;
+  Acc acc;
+  acc.F = D3(0.0, 0.0, 0.0);
+  acc.T = D3(0.0, 0.0, 0.0);
+  acc.hc = 0;
+  acc.nkind[0] = acc.nkind[1] = acc.nkind[2] = acc.nkind[3] = 0;
+  acc.overflow = 0;
+  if (k < n) {
+    const int cap = P.cap;
+    const int s = perm[k];
+    const double4 pi = pk[k];
+    const d3 xi = D3(pi.x, pi.y, pi.z);
+    const double ri = pi.w;
+    const int idi = A.id[s];
+    const int ti = A.type[s];
+    const d3 vi = ld3(A.lv, cap, s);
+    const d3 wi = ld3(A.rv, cap, s);
+    const double xv[3] = {pi.x, pi.y, pi.z};
+
+    // ---------------- moving particles of every search level -------------
+    for (int L = 0; L < P.nlvl; L++) {
+      const Grid& G = P.lvl[L];
+      const double reach = ri + G.rmax;
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; d++) {
+        double pad = 1.0e-6 * G.w[d];
+        lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+        hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+        if (!P.periodic[d]) {
+          lo[d] = lo[d] < 0 ? 0 : lo[d];
+          hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+        }
+      }
+      for (int cz = lo[2]; cz <= hi[2]; cz++) {
+        int wz = cz;
+        double shz = 0.0;
+        if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
+        else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
+        for (int cy = lo[1]; cy <= hi[1]; cy++) {
+          int wy = cy;
+          double shy = 0.0;
+          if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
+          else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
+          const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
+          // split the x range into up to three contiguous key segments
+          for (int seg = 0; seg < 3; seg++) {
+            int xa, xb;
+            double shx;
+            if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; shx = -P.simlen[0]; }
+            else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; shx = 0.0; }
+            else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; shx = P.simlen[0]; }
+            if (xa > xb) continue;
+            int wa = xa, wb = xb;
+            if (seg == 0) { wa += G.n[0]; wb += G.n[0]; }
+            if (seg == 2) { wa -= G.n[0]; wb -= G.n[0]; }
+            const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
+            const int ja = cstart[rowkey + wa], jb = cstart[rowkey + wb + 1];
+            for (int kj = ja; kj < jb; kj++) {
+              if (kj == k && !shifted) continue;
+              const double4 pj = pk[kj];
+              double xj = pj.x, yj = pj.y, zj = pj.z;
+              if (shifted) { xj = xj + shx; yj = yj + shy; zj = zj + shz; }
+              // narrow phase, NBSM_FineSearch2/3 (Prtcl_NBS_Munjiza.f90:503-557)
+              const double dr = ri + pj.w;
+              const double dr2 = dr * dr;
+              const double ddx = xi.x - xj;
+              double d2 = ddx * ddx;
+              if (d2 > dr2) continue;
+              const double ddy = xi.y - yj;
+              d2 = ddy * ddy + d2;
+              if (d2 > dr2) continue;
+              const double ddz = xi.z - zj;
+              d2 = ddz * ddz + d2;
+              if (d2 > dr2) continue;
+              const double ovrlp = dr - sqrt(d2);
+              const int sj = perm[kj];
+              const int idj = A.id[sj];
+              const int tj = A.type[sj];
+              const d3 vj = ld3(A.lv, cap, sj);
+              const d3 wj = ld3(A.rv, cap, sj);
+              const d3 xjv = D3(xj, yj, zj);
+              if (!shifted) {
+                // PP: record in the lower id's bucket, force from its side
+                // (FineSearch2 :525-529, ContactForce_PP :732-756)
+                const bool ilo = idi < idj;
+                const Hist h = ilo ? find_hist(A, P.hcap, s, idj) : find_hist(A, P.hcap, sj, idi);
+                const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
+                const Bnry& Bp = P.pp[(tlo - 1) * P.ntype + (thi - 1)];
+                CF cf;
+                if (ilo) {
+                  cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
+                  acc.F = add3(acc.F, cf.fsum);
+                  acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+                  push_hist(P, B, k, acc, 1, idj, cf);
+                } else {
+                  cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
+                  acc.F = sub3(acc.F, cf.fsum);
+                  acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+                }
+              } else {
+                // PPG: each side keeps its own record against the other's
+                // image (ContactForce_PPG :761-798); row type = local particle
+                const Hist h = find_hist(A, P.hcap, s, idj);
+                const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (tj - 1)];
+                CF cf;
+                if (idi < idj) {
+                  cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
+                  acc.F = add3(acc.F, cf.fsum);
+                  acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+                } else {
+                  cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
+                  acc.F = sub3(acc.F, cf.fsum);
+                  acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+                }
+                push_hist(P, B, k, acc, 2, idj, cf);
+              }
+            }
+          }
+        }
+      }
+    }
+
+    // ---------------- fixed particles (NBS only) --------------------------
+    if (P.search_fixed && P.nfix > 0) {
+      const Grid& G = P.fgrid;
+      const double reach = ri + G.rmax;
+      int lo[3], hi[3];
+      for (int d = 0; d < 3; d++) {
+        double pad = 1.0e-6 * G.w[d];
+        lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+        hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+        if (!P.periodic[d]) {
+          lo[d] = lo[d] < 0 ? 0 : lo[d];
+          hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+        }
+      }
+      for (int cz = lo[2]; cz <= hi[2]; cz++) {
+        int wz = cz;
+        double shz = 0.0;
+        if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
+        else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
+        for (int cy = lo[1]; cy <= hi[1]; cy++) {
+          int wy = cy;
+          double shy = 0.0;
+          if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
+          else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
+          for (int cx = lo[0]; cx <= hi[0]; cx++) {
+            int wx = cx;
+            double shx = 0.0;
+            if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
+            else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
+            const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
+            for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
+              const double4 pj = fpk[kj];
+              const double xj = pj.x + shx, yj = pj.y + shy, zj = pj.z + shz;
+              // NBSM_FineSearch1 (Prtcl_NBS_Munjiza.f90:478-498)
+              const double dr = ri + pj.w;
+              const double dr2 = dr * dr;
+              const double ddx = xi.x - xj;
+              double d2 = ddx * ddx;
+              if (d2 > dr2) continue;
+              const double ddy = xi.y - yj;
+              d2 = ddy * ddy + d2;
+              if (d2 > dr2) continue;
+              const double ddz = xi.z - zj;
+              d2 = ddz * ddz + d2;
+              if (d2 > dr2) continue;
+              const double ovrlp = dr - sqrt(d2);
+              const int idj = fid[kj];
+              const Hist h = find_hist(A, P.hcap, s, idj);
+              const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (ftype[kj] - 1)];
+              const d3 z3 = D3(0.0, 0.0, 0.0);
+              CF cf = contact_force(P, Bp, nv3(D3(xj, yj, zj), xi), ri, pj.w, vi, z3, wi, z3,
+                                    ovrlp, h.td, h.vri);
+              acc.F = add3(acc.F, cf.fsum);
+              acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+              push_hist(P, B, k, acc, 3, idj, cf);
+            }
+          }
+        }
+      }
+    }
+
+    // ---------------- walls of the near-wall list -------------------------
+    // (FindContactsPW :298-319 walks the list newest first = highest index)
+    uint64_t mask = A.wmask[s];
+    while (mask) {
+      const int w = 63 - __clzll((long long)mask);
+      mask &= ~(1ull << w);
+      const Wall& W = P.walls[w];
+      double ovrlp;
+      d3 nv;
+      if (pw_in_contact(W, xi, ri, &ovrlp, &nv)) {
+        // ContactForce_PW (Prtcl_CL_and_CF.f90:832-856)
+        const Hist h = find_hist(A, P.hcap, s, W.wall_id);
+        const Bnry& Bp = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
+        const d3 norm_v = mul3(nv, -1.0);
+        CF cf = contact_force(P, Bp, norm_v, ri, 1.00e20 * ri, vi, wp(W.trans_vel), wi,
+                              D3(0.0, 0.0, 0.0), ovrlp, h.td, h.vri);
+        acc.F = add3(acc.F, cf.fsum);
+        acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+        push_hist(P, B, k, acc, 4, W.wall_id, cf);
+      }
+    }
+
+    // ---------------- acceleration (DEMS_clc_Acceleration :360-375) -------
+    const double mass = P.mass[ti - 1], inertia = P.inertia[ti - 1];
+    const d3 a1 = add3(div3(acc.F, mass), D3(P.g[0], P.g[1], P.g[2]));
+    const d3 al1 = div3(acc.T, inertia);
+
+    // ---------------- integration (Prtcl_Integrate :15-88) ----------------
+    const double dt = P.dt;
+    d3 pos = xi;
+    d3 lin1;  // linVel(1) after the translational update
+    if (P.pi_method == 1) {
+      d3 sft = mul3(vi, dt);
+      pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
+      lin1 = add3(vi, mul3(a1, dt));
+      st3(B.lv, cap, k, lin1);
+      st3(B.la, cap, k, a1);
+    } else if (P.pi_method == 2) {
+      const double c1 = 3.0 / 2.0, c2 = -1.0 / 2.0;
+      d3 v2 = ld3(A.lv + 3 * cap, cap, s);
+      d3 a2 = ld3(A.la, cap, s);
+      d3 sft = mul3(add3(mul3(vi, c1), mul3(v2, c2)), dt);
+      pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
+      lin1 = add3(vi, mul3(add3(mul3(a1, c1), mul3(a2, c2)), dt));
+      st3(B.lv, cap, k, lin1);
+      st3(B.lv + 3 * cap, cap, k, vi);
+      st3(B.la, cap, k, a1);  // linAcc(2) <- linAcc(1)
+    } else {
+      const double c1 = 23.0 / 12.0, c2 = -16.0 / 12.0, c3 = 5.0 / 12.0;
+      d3 v2 = ld3(A.lv + 3 * cap, cap, s);
+      d3 v3 = ld3(A.lv + 6 * cap, cap, s);
+      d3 a2 = ld3(A.la, cap, s);
+      d3 a3 = ld3(A.la + 3 * cap, cap, s);
+      d3 sft = mul3(add3(add3(mul3(vi, c1), mul3(v2, c2)), mul3(v3, c3)), dt);
+      pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
+      lin1 = add3(vi, mul3(add3(add3(mul3(a1, c1), mul3(a2, c2)), mul3(a3, c3)), dt));
+      st3(B.lv, cap, k, lin1);
+      st3(B.lv + 3 * cap, cap, k, vi);
+      st3(B.lv + 6 * cap, cap, k, v2);
+      st3(B.la, cap, k, a1);
+      st3(B.la + 3 * cap, cap, k, a2);
+    }
+    d3 th = ld3(A.th, cap, s);
+    if (P.pri_method == 1) {
+      th = add3(th, mul3(wi, dt));
+      st3(B.rv, cap, k, add3(wi, mul3(al1, dt)));
+      st3(B.ra, cap, k, al1);
+    } else if (P.pri_method == 2) {
+      // Prtcl_Integration.f90:64: rotVel1 = GPrtcl_linVel(1,i)
+      const double c1 = 3.0 / 2.0, c2 = -1.0 / 2.0;
+      d3 w1 = lin1;
+      d3 w2 = ld3(A.rv + 3 * cap, cap, s);
+      d3 al2 = ld3(A.ra, cap, s);
+      th = add3(th, mul3(add3(mul3(w1, c1), mul3(w2, c2)), dt));
+      st3(B.rv, cap, k, add3(w1, mul3(add3(mul3(al1, c1), mul3(al2, c2)), dt)));
+      st3(B.rv + 3 * cap, cap, k, w1);
+      st3(B.ra, cap, k, al1);
+    } else {
+      const double c1 = 23.0 / 12.0, c2 = -16.0 / 12.0, c3 = 5.0 / 12.0;
+      d3 w2 = ld3(A.rv + 3 * cap, cap, s);
+      d3 w3 = ld3(A.rv + 6 * cap, cap, s);
+      d3 al2 = ld3(A.ra, cap, s);
+      d3 al3 = ld3(A.ra + 3 * cap, cap, s);
+      th = add3(th, mul3(add3(add3(mul3(wi, c1), mul3(w2, c2)), mul3(w3, c3)), dt));
+      st3(B.rv, cap, k, add3(wi, mul3(add3(add3(mul3(al1, c1), mul3(al2, c2)), mul3(al3, c3)), dt)));
+      st3(B.rv + 3 * cap, cap, k, wi);
+      st3(B.rv + 6 * cap, cap, k, w2);
+      st3(B.ra, cap, k, al1);
+      st3(B.ra + 3 * cap, cap, k, al2);
+    }
+    st3(B.th, cap, k, th);
+
+    // ---------------- periodic wrap / deletion (Comm_For_Exchange) --------
+    double pv[3] = {pos.x, pos.y, pos.z};
+    bool dead = false;
+    for (int d = 0; d < 3; d++) {
+      if (P.periodic[d]) {
+        if (pv[d] >= P.dmax[d]) pv[d] = pv[d] - P.simlen[d];
+        else if (pv[d] < P.dmin[d]) pv[d] = pv[d] + P.simlen[d];
+      } else if (pv[d] >= P.dmax[d] || pv[d] < P.dmin[d]) {
+        dead = true;
+      }
+    }
+    B.x[k] = pv[0];
+    B.y[k] = pv[1];
+    B.z[k] = pv[2];
+    B.r[k] = ri;
+    B.id[k] = idi;
+    B.type[k] = dead ? -ti : ti;
+    B.mark[k] = A.mark[s];
+    B.wmask[k] = A.wmask[s];
+    B.hcnt[k] = dead ? 0 : acc.hc;
+    if (P.keep_forces) {
+      st3(B.force, cap, k, acc.F);
+      st3(B.force + 3 * cap, cap, k, acc.T);
+    }
+    if (dead) atomicAdd(&S->n_deleted, 1);
+  }
+  // counters: wave reduction then one atomic per wave
+  for (int q = 0; q < 4; q++) {
+    int v = acc.nkind[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&S->num_cntcts[q], v);
+  }
+  if (acc.overflow) atomicMax(&S->overflow, 1);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called from sdem_api.cpp)
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_begin(Scalars* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, int n_upper, hipStream_t st) {
+  if (P.nwall > 0 && n_upper > 0)
+    hipLaunchKernelGGL(k_wall_list, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S);
+  hipLaunchKernelGGL(k_wall_cadence, dim3(1), dim3(1), 0, st, P, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
+                      int* keys, int* rank, int* counts, int* start, int* perm_tmp, int* perm,
+                      double4* pk, void* scan_tmp, size_t scan_bytes, hipStream_t st) {
+  hipError_t e = launch_fill_i32(counts, dead_key + 2, 0, st);
+  if (e != hipSuccess) return e;
+  if (n_upper > 0)
+    hipLaunchKernelGGL(k_bin, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, dead_key, keys,
+                       rank, counts);
+  size_t bytes = scan_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, counts, start, (int)(dead_key + 2), st);
+  if (e != hipSuccess) return e;
+  if (n_upper > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n_upper, 256)), dim3(256), 0, st, S, keys, rank, start,
+                       perm_tmp);
+    hipLaunchKernelGGL(k_rank, dim3(nblk(n_upper, 256)), dim3(256), 0, st, A, S, dead_key, keys,
+                       start, perm_tmp, perm, pk);
+  }
+  return hipGetLastError();
+}
+
+size_t scan_temp_bytes(int64_t nitems) {
+  size_t bytes = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int*)nullptr, (int*)nullptr, (int)nitems) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+hipError_t launch_step(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
+                       const int* perm, const double4* pk, const int* cstart, const double4* fpk,
+                       const int* fid, const int* ftype, const int* fstart, hipStream_t st) {
+  if (n_upper > 0)
+    hipLaunchKernelGGL(k_step, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, pk,
+                       cstart, fpk, fid, ftype, fstart);
+  return hipGetLastError();
+}
+
+}  // namespace sdem

@@ -1,0 +1,77 @@
+"""Comparison helpers for HIP-vs-oracle parity (used by tests and smoke).
+
+Bar: particle ids, contact keys (kind, owner, partner) and positions after
+one step from an identical state are bit-exact; FP64 forces, torques,
+velocities, accelerations and contact history agree within 1e-12 relative,
+with the relative error of a per-particle vector taken against its own size
+plus the largest size of that quantity in the system (summation order of
+contact forces differs between the two implementations, and a resting
+particle's net force is a small difference of large contact forces).
+"""
+import numpy as np
+
+from sdem_amd import native
+
+RTOL = 1e-12
+
+
+def close(a, b, rtol=RTOL, axis_scale=None):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    if a.shape != b.shape:
+        return False, f"shape {a.shape} vs {b.shape}"
+    if a.size == 0:
+        return True, ""
+    scale = np.abs(b).max() if axis_scale is None else axis_scale
+    tol = rtol * (np.abs(b) + scale) + 1e-300
+    bad = np.abs(a - b) > tol
+    if bad.any():
+        i = np.argwhere(bad)[0]
+        return False, f"max rel err {np.max(np.abs(a - b) / (np.abs(b) + scale + 1e-300)):.3e} at {tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}"
+    return True, ""
+
+
+def compare(gpu: native.Engine, ref: native.Engine, exact_positions=True, rtol=RTOL):
+    """Return a list of failure strings (empty = parity)."""
+    errs = []
+    g, r = gpu.get_state(), ref.get_state()
+    if not np.array_equal(g["id"], r["id"]):
+        return [f"particle ids differ: {len(g['id'])} vs {len(r['id'])}"]
+    for k in ("ptype", "usr_mark"):
+        if not np.array_equal(g[k], r[k]):
+            errs.append(f"{k} differs")
+    if exact_positions:
+        if not np.array_equal(g["posr"], r["posr"]):
+            d = np.abs(g["posr"] - r["posr"]).max()
+            errs.append(f"positions not bit-exact (max diff {d:.3e})")
+    else:
+        ok, m = close(g["posr"][:, :3], r["posr"][:, :3], rtol)
+        if not ok:
+            errs.append("positions: " + m)
+    for k in ("lin_vel", "lin_acc", "theta", "rot_vel", "rot_acc"):
+        ok, m = close(g[k], r[k], rtol)
+        if not ok:
+            errs.append(f"{k}: {m}")
+    gc, rc = gpu.get_contacts(), ref.get_contacts()
+    for k in ("kind", "owner", "partner"):
+        if not np.array_equal(gc[k], rc[k]):
+            errs.append(f"contact {k} differs ({len(gc[k])} vs {len(rc[k])} records)")
+            return errs
+    ok, m = close(gc["tan_delta"], rc["tan_delta"], rtol)
+    if not ok:
+        errs.append("tan_delta: " + m)
+    ok, m = close(gc["vrel_init"], rc["vrel_init"], rtol)
+    if not ok:
+        errs.append("vrel_init: " + m)
+    if not np.array_equal(gpu.get_counts(), ref.get_counts()):
+        errs.append(f"counts {gpu.get_counts()} vs {ref.get_counts()}")
+    gi, gf, gt = gpu.get_forces()
+    ri, rf, rt = ref.get_forces()
+    ok, m = close(gf, rf, rtol)
+    if not ok:
+        errs.append("force: " + m)
+    ok, m = close(gt, rt, rtol)
+    if not ok:
+        errs.append("torque: " + m)
+    if gpu.get_iter() != ref.get_iter():
+        errs.append(f"iter/next wall update {gpu.get_iter()} vs {ref.get_iter()}")
+    return errs

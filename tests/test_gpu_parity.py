@@ -1,0 +1,53 @@
+"""HIP path vs the CPU oracle on identical seeded states (needs the GPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from parity import compare
+from sdem_amd import native, scenarios
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "box_nlin_ab2": dict(),
+    "box_lsd_ab2": dict(cf_type=native.CFT_LSD),
+    "periodic_x_walls_yz": dict(periodic=(1, 0, 0), walls="yYzZ"),
+    "periodic_xy_bottom_infinite": dict(periodic=(1, 1, 0), walls="z", infinite_walls=True),
+    "periodic_xyz": dict(periodic=(1, 1, 1), walls=""),
+    "ab3_ab3": dict(pi=native.PIM_AB3, pri=native.PIM_AB3),
+    "fe_fe_variable_torque": dict(pi=native.PIM_FE, pri=native.PIM_FE,
+                                  ct_model=native.CTM_VARIABLE_TORQUE),
+    "fixed_particles": dict(nfixed=25),
+    "hierarchical": dict(diameters=(0.6e-3, 1.2e-3, 3.0e-3), cs_method=native.CSM_NBS_MUNJIZA_HRCHL,
+                         nfixed=10),
+}
+
+
+def engines(sc, hip_lib, oracle_lib):
+    gpu = native.build_engine(hip_lib, sc)
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    return gpu, ref
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_one_step_parity(name, hip_lib, oracle_lib):
+    print(f"case {name}: one step", flush=True)
+    sc = scenarios.random_cloud(n=400, seed=11, **CASES[name])
+    gpu, ref = engines(sc, hip_lib, oracle_lib)
+    gpu.iterate(1)
+    ref.iterate(1)
+    errs = compare(gpu, ref)
+    assert not errs, errs
+    assert gpu.get_counts().sum() > 50
+
+
+@pytest.mark.parametrize("name", ["box_nlin_ab2", "periodic_xy_bottom_infinite", "hierarchical"])
+def test_many_steps_parity(name, hip_lib, oracle_lib):
+    print(f"case {name}: 40 steps", flush=True)
+    sc = scenarios.random_cloud(n=300, seed=5, **CASES[name])
+    gpu, ref = engines(sc, hip_lib, oracle_lib)
+    for k in range(4):
+        gpu.iterate(10)
+        ref.iterate(10)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, (k, errs)
