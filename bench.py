@@ -1,0 +1,149 @@
+"""Throughput of the DEM step on one MI355X.
+
+Workload: the plain GPU DEM at scale -- 1,000,000 polydisperse spheres
+(log-normal diameters, median 2 mm, sigma 0.25, clipped to [1,4] mm, 32
+types), FP64, periodic in x and y (0.2 x 0.2 m), flat bottom wall, open top,
+Hertz-Mindlin with tangential history (E=5e7 Pa, nu=0.3, e=0.8, mu=0.4),
+seeded random loose pack at solid fraction 0.3 falling under gravity,
+dt=5e-6 s, AB2.  The state is uploaded once; the timed region runs K steps
+with everything resident in HBM.
+
+Prints one JSON line: particle-steps/s, the roofline of the dominant kernel
+(the fused step kernel) and a CPU baseline (the C restatement of the
+reference step on one host core, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (FP64 SoA, AB2 translation + AB2 rotation, int32 types):
+#   per particle and step 428 B = 220 read + 192 written + 16 for the cell sort
+#   per contact record 80 B (old key + history 40 B, new key + history 40 B)
+# the fused step kernel owns all of it except the 16 B of the sort
+BYTES_PER_PARTICLE_STEP = 428
+BYTES_PER_PARTICLE_KERNEL = 412
+BYTES_PER_CONTACT = 80
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick ~15 s of CPU work")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    """The oracle (C restatement of the reference step) on one host core, on
+    a seeded 20k-particle box of the same workload (same pack fraction,
+    size distribution and physics; xy area scaled to the particle count)."""
+    import oracle
+    from sdem_amd import native, scenarios
+
+    n = args.cpu_sample
+    sc = scenarios.settling(n=n, seed=args.seed)
+    eng = native.build_engine(oracle.load(), sc, prefix=oracle.PREFIX)
+    eng.iterate(2)
+    t0 = time.perf_counter()
+    eng.iterate(5)
+    per_step = (time.perf_counter() - t0) / 5
+    steps = args.cpu_steps or max(5, int(15.0 / max(per_step, 1e-6)))
+    t0 = time.perf_counter()
+    eng.iterate(steps)
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"settling bed, {n} spheres (same distribution, phi=0.3, xy area scaled), "
+                      f"{steps} steps after 7 warm-up steps, oracle on 1 core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU decomposition is design only)", file=sys.stderr)
+        sys.exit(2)
+    from sdem_amd import native, scenarios
+
+    t0 = time.perf_counter()
+    sc = scenarios.settling(n=args.n, seed=args.seed)
+    t_gen = time.perf_counter() - t0
+    print(f"# generated {args.n} spheres in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    lib = native.load_hip_library()
+    eng = native.build_engine(lib, sc)
+    eng.iterate(args.warmup)
+    print(f"# warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
+    eng.set_profiling(True)
+    import torch  # device synchronisation around the timed region
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, counts_sum = 0, np.zeros(4, np.int64)
+    chunk = max(1, args.steps // 10)
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        eng.iterate(k)  # synchronises before returning
+        counts_sum += eng.get_counts()
+        done += k
+        print(f"# {done}/{args.steps} steps", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = eng.get_profile()
+    n_now = eng.num_particles()
+    ms_step = wall / args.steps * 1e3
+    value = args.n * args.steps / wall
+    # contact records per step, sampled once per chunk
+    P = float(counts_sum.sum()) / max(1, -(-args.steps // chunk))
+    k_ms, k_n = prof["step"]
+    k_avg_s = k_ms / max(1, k_n) / 1e3
+    kernel_bytes = BYTES_PER_PARTICLE_KERNEL * args.n + BYTES_PER_CONTACT * P
+    achieved = kernel_bytes / k_avg_s / 1e9
+    step_bytes = BYTES_PER_PARTICLE_STEP * args.n + BYTES_PER_CONTACT * P
+    out = {
+        "metric": "particle-steps/s at 1M polydisperse spheres",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random loose pack, phi=0.3)",
+        "config": {"workload": "settling bed: 1M log-normal spheres, periodic x/y 0.2x0.2 m, bottom wall, "
+                               "Hertz-Mindlin with history, dt=5e-6 s, AB2",
+                   "n": args.n, "seed": args.seed, "contact_records_per_step": P,
+                   "particles_at_end": n_now},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step", "kernel_ms_avg": k_avg_s * 1e3,
+                     "algorithmic_bytes_per_launch": kernel_bytes},
+        "step_breakdown_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},
+        "whole_step_hbm_frac": step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+    }
+    eng.close()
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

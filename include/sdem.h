@@ -190,6 +190,17 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
 /* Memory the context holds on the device, in bytes. */
 int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);
 
+/* ---- utilities ------------------------------------------------------------
+ * Seeded random sequential addition of n non-overlapping spheres in the box
+ * [lo,hi) (largest first; each sphere retries up to max_attempts uniform
+ * centres).  periodic[d] lets spheres straddle that face.  Writes pos (n x 3).
+ * Replaces the clock-seeded GL_MakingAllPrtcl (Prtcl_Variables.f90:191-307)
+ * for reproducible initial states; host only, no device needed.
+ * Returns SDEM_E_CAPACITY when a sphere could not be placed. */
+int sdem_util_random_pack(int64_t n, const double* radius, const double lo[3], const double hi[3],
+                          const int32_t periodic[3], uint64_t seed, int32_t max_attempts,
+                          double* pos);
+
 const char* sdem_last_error(const sdem_ctx* c);
 void sdem_destroy(sdem_ctx* c);
 

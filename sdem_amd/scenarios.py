@@ -73,52 +73,26 @@ def discretise(d: np.ndarray, edges: np.ndarray, centers: np.ndarray):
     return t.astype(np.int32) + 1, centers[t]
 
 
-def rsa_pack(radii: np.ndarray, lo, hi, rng: np.random.Generator, max_rounds: int = 400):
+def rsa_pack(radii: np.ndarray, lo, hi, seed: int, periodic=(0, 0, 0), max_attempts: int = 100000):
     """Random sequential addition of non-overlapping spheres in a box,
-    vectorised in rounds (proposals that overlap accepted spheres or each
-    other are re-drawn)."""
-    from scipy.spatial import cKDTree
+    largest first (sdem_util_random_pack in libsdem_hip.so, host code)."""
+    import ctypes as C
 
-    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
-    n = len(radii)
-    pos = np.zeros((n, 3))
-    placed = np.zeros(n, bool)
-    rmax = float(radii.max())
-    todo = np.arange(n)
-    acc_idx = np.zeros(0, np.int64)
-    tree = None
-    for _ in range(max_rounds):
-        if len(todo) == 0:
-            break
-        r = radii[todo]
-        prop = lo + r[:, None] + rng.random((len(todo), 3)) * (hi - lo - 2 * r[:, None])
-        ok = np.ones(len(todo), bool)
-        if tree is not None:
-            k = min(32, len(acc_idx))
-            dist, nb = tree.query(prop, k=k, distance_upper_bound=2 * rmax)
-            if k == 1:
-                dist, nb = dist[:, None], nb[:, None]
-            valid = nb < len(acc_idx)
-            rj = np.where(valid, radii[acc_idx[np.minimum(nb, len(acc_idx) - 1)]], 0.0)
-            over = valid & (dist < r[:, None] + rj)
-            ok &= ~over.any(axis=1)
-        cand = np.nonzero(ok)[0]
-        if len(cand) > 1:
-            t2 = cKDTree(prop[cand])
-            pairs = t2.query_pairs(2 * rmax, output_type="ndarray")
-            if len(pairs):
-                a, b = cand[pairs[:, 0]], cand[pairs[:, 1]]
-                dd = np.linalg.norm(prop[a] - prop[b], axis=1)
-                bad = dd < r[a] + r[b]
-                ok[np.maximum(a[bad], b[bad])] = False
-        good = todo[ok]
-        pos[good] = prop[ok]
-        placed[good] = True
-        acc_idx = np.concatenate([acc_idx, good])
-        tree = cKDTree(pos[acc_idx])
-        todo = todo[~ok]
-    if len(todo):
-        raise RuntimeError(f"random packing failed: {len(todo)} spheres could not be placed")
+    from .native import load_hip_library
+
+    lib = load_hip_library()
+    f = lib.sdem_util_random_pack
+    DP = C.POINTER(C.c_double)
+    f.argtypes = [C.c_int64, DP, DP, DP, C.POINTER(C.c_int32), C.c_uint64, C.c_int32, DP]
+    f.restype = C.c_int
+    r = np.ascontiguousarray(radii, np.float64)
+    lo_, hi_ = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    per = np.asarray(periodic, np.int32)
+    pos = np.zeros((len(r), 3))
+    rc = f(len(r), r.ctypes.data_as(DP), lo_.ctypes.data_as(DP), hi_.ctypes.data_as(DP),
+           per.ctypes.data_as(C.POINTER(C.c_int32)), int(seed), int(max_attempts), pos.ctypes.data_as(DP))
+    if rc != 0:
+        raise RuntimeError(f"random packing failed (code {rc})")
     return pos
 
 
@@ -242,7 +216,7 @@ def settling(n=1_000_000, seed=DEFAULT_SEED, phi=0.3, ntypes=32, area=None, cs_m
     side = area if area is not None else 0.2 * math.sqrt(n / 1.0e6)
     vol_solid = float(np.sum(np.pi / 6 * dq ** 3))
     height = vol_solid / phi / (side * side)
-    pos = rsa_pack(dq / 2, (0.0, 0.0, 0.0), (side, side, height), rng)
+    pos = rsa_pack(dq / 2, (0.0, 0.0, 0.0), (side, side, height), seed, periodic=(1, 1, 0))
     order = np.lexsort((pos[:, 0], pos[:, 1], pos[:, 2]))  # any fixed order
     pos, t, dq = pos[order], t[order], dq[order]
     ids = np.arange(1, n + 1, dtype=np.int32)
@@ -310,7 +284,7 @@ def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
         box = (s, s, 2 * s)
     box = np.asarray(box, float)
     height = vol_solid / phi / (box[0] * box[1])
-    pos = rsa_pack(dq / 2, (0, 0, 0.0), (box[0], box[1], min(height, box[2] * 0.999)), rng)
+    pos = rsa_pack(dq / 2, (0, 0, 0.0), (box[0], box[1], min(height, box[2] * 0.999)), seed)
     order = np.lexsort((pos[:, 0], pos[:, 1], pos[:, 2]))
     pos, t, dq = pos[order], t[order], dq[order]
     ids = np.arange(1, n + 1, dtype=np.int32)

@@ -1,0 +1,296 @@
+"""Pin the CPU oracle against what the mathematics fixes.
+
+The reference ships no golden vectors, known-answer tests or fixtures for
+this path (its tree has no tests at all), and its Fortran+MPI build needs a
+stand-in `mpi` module and an external MPI library, so it is not built here.
+Every check below therefore compares the oracle with an independent
+statement of the physics or geometry -- a closed form, an invariant, or a
+brute-force enumeration -- chosen so that a dropped term, a wrong sign or a
+swapped operand in the restatement fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from sdem_amd import native, scenarios
+
+
+def make(lib, sc):
+    return native.build_engine(lib, sc, prefix=oracle.PREFIX)
+
+
+def collide(lib, sc, max_steps=40000):
+    e = make(lib, sc)
+    steps_in_contact = 0
+    for _ in range(max_steps):
+        e.iterate(1)
+        c = e.get_counts()
+        if c.sum() > 0:
+            steps_in_contact += 1
+        elif steps_in_contact:
+            break
+    return e, steps_in_contact
+
+
+# ---------------------------------------------------------------------------
+# normal force law: restitution and contact duration of a head-on collision
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("rest", [0.3, 0.6, 0.8, 0.95])
+def test_hertz_restitution_and_duration(oracle_lib, rest):
+    """Hertz + the Tsuji damping fit: the rebound speed ratio equals the
+    input restitution within the fit's accuracy (about 1 %), and an elastic
+    (e->1) contact lasts t_c = 2.87 (m*^2 / (R* E*^2 v))^(1/5)."""
+    v, d, E, nu, rho = 0.5, 2.0e-3, 5.0e7, 0.3, 2500.0
+    dt = 2.0e-8
+    sc = scenarios.two_spheres(d=d, v=v, rest=rest, youngs=E, poisson=nu, density=rho, dt=dt)
+    e, nsteps = collide(oracle_lib, sc)
+    vel = e.get_state()["lin_vel"][:, 0, :]
+    e_eff = (vel[1, 0] - vel[0, 0]) / (2 * v)
+    assert abs(e_eff - rest) < 0.012 * rest, e_eff
+    assert np.allclose(vel[:, 1:], 0.0, atol=1e-15)
+    # momentum conserved exactly by the pair force (equal and opposite)
+    assert abs(vel[0, 0] + vel[1, 0]) < 1e-13
+    if rest >= 0.95:
+        m = rho * math.pi / 6 * d ** 3
+        mstar, rstar = m / 2, d / 4
+        estar = E / (2 * (1 - nu * nu))
+        tc = 2.87 * (mstar ** 2 / (rstar * estar ** 2 * (2 * v))) ** 0.2
+        assert abs(nsteps * dt - tc) / tc < 0.03, (nsteps * dt, tc)
+
+
+def test_lsd_restitution_is_velocity_independent(oracle_lib):
+    """The LSD model scales k and d with the impact speed so that the
+    restitution does not depend on it (Prtcl_ContactForce_inc.f90:24-28)."""
+    out = []
+    for v in (0.1, 0.4, 1.6):
+        sc = scenarios.two_spheres(v=v, rest=0.7, cf_type=native.CFT_LSD, dt=2.0e-8)
+        e, _ = collide(oracle_lib, sc)
+        vel = e.get_state()["lin_vel"][:, 0, 0]
+        out.append((vel[1] - vel[0]) / (2 * v))
+    assert np.ptp(out) < 1e-3, out
+    assert abs(np.mean(out) - 0.7) < 0.01, out
+
+
+def test_static_wall_overlap(oracle_lib):
+    """A sphere resting on a wall settles at the Hertz equilibrium overlap
+    delta = (m g / K_h)^(2/3), K_h = 4/3 E* sqrt(R) for a flat wall."""
+    d, E, nu, rho, g = 2.0e-3, 5.0e7, 0.3, 2500.0, 9.81
+    L = 0.02
+    sc = scenarios.two_spheres(d=d, rest=0.3, youngs=E, poisson=nu, density=rho, dt=1e-6)
+    st = sc["state"]
+    st["id"], st["ptype"], st["usr_mark"] = st["id"][:1], st["ptype"][:1], st["usr_mark"][:1]
+    for k in ("lin_vel", "lin_acc", "rot_vel", "rot_acc", "theta"):
+        st[k] = np.zeros_like(st[k][:1])
+    st["posr"] = np.array([[L / 2, L / 2, d / 2 + 1e-6, d / 2]])
+    sc["options"].update(gravity=(0.0, 0.0, -g), domain_max=(L, L, L), num_prtcl=1)
+    sc["walls"] = scenarios.box_walls((0, 0, 0), (L, L, L), faces="z")
+    e = make(oracle_lib, sc)
+    e.iterate(60000)
+    z = e.get_state()["posr"][0, 2]
+    m = rho * math.pi / 6 * d ** 3
+    estar = 1.0 / ((1 - nu * nu) / E + (1 - nu * nu) / E)
+    kh = 4.0 / 3.0 * estar * math.sqrt(d / 2)
+    delta = (m * g / kh) ** (2.0 / 3.0)
+    assert abs((d / 2 - z) - delta) / delta < 1e-4, (d / 2 - z, delta)
+
+
+def test_sliding_sphere_rolls_at_five_sevenths(oracle_lib):
+    """A sphere launched sliding on a rough wall ends rolling at 5/7 of its
+    launch speed (friction and the tangential history, no rolling
+    resistance); AB3 rotation keeps the test clear of the AB2 rotation
+    quirk."""
+    d, L = 2.0e-3, 0.2
+    sc = scenarios.two_spheres(d=d, rest=0.3, fric=0.3, dt=1e-6, pi_method=native.PIM_AB3,
+                               pri_method=native.PIM_AB3)
+    st = sc["state"]
+    for k in ("id", "ptype", "usr_mark"):
+        st[k] = st[k][:1]
+    for k in ("lin_vel", "lin_acc", "rot_vel", "rot_acc", "theta"):
+        st[k] = np.zeros_like(st[k][:1])
+    delta0 = 2.7e-6  # close to the static overlap
+    st["posr"] = np.array([[0.01, L / 2, d / 2 - delta0, d / 2]])
+    st["lin_vel"][0, :, 0] = 0.2
+    sc["options"].update(gravity=(0.0, 0.0, -9.81), domain_max=(L, L, 0.01), num_prtcl=1)
+    sc["walls"] = scenarios.box_walls((0, 0, 0), (L, L, 0.01), faces="z")
+    e = make(oracle_lib, sc)
+    e.iterate(150000)
+    s = e.get_state()
+    vx, wy = s["lin_vel"][0, 0, 0], s["rot_vel"][0, 0, 1]
+    assert abs(vx - 5.0 / 7.0 * 0.2) < 2e-3 * 0.2, vx
+    assert abs(wy * d / 2 - vx) < 2e-3 * 0.2, (wy * d / 2, vx)
+
+
+# ---------------------------------------------------------------------------
+# integrators
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("pi", [native.PIM_FE, native.PIM_AB2, native.PIM_AB3])
+def test_free_fall_closed_form(oracle_lib, pi):
+    """Constant acceleration g from zero history: FE gives v_n = v0 + n g dt,
+    x_n = x0 + n v0 dt + n(n-1)/2 g dt^2; AB2 with the zero start has
+    v_n = v0 + (n+1/2) g dt, x_n = x0 + n v0 dt + (n^2/2 + ... ) closed sums;
+    compare against the recurrences evaluated exactly in fractions."""
+    from fractions import Fraction as Fr
+
+    g, dt, v0, nst = -9.81, 1e-3, 0.3, 50
+    sc = scenarios.two_spheres(pi_method=pi, pri_method=pi)
+    st = sc["state"]
+    for k in ("id", "ptype", "usr_mark"):
+        st[k] = st[k][:1]
+    for k in ("lin_vel", "lin_acc", "rot_vel", "rot_acc", "theta"):
+        st[k] = np.zeros_like(st[k][:1])
+    st["posr"] = np.array([[0.025, 0.025, 0.9, 1e-3]])
+    st["lin_vel"][0, 0, 2] = v0
+    sc["options"].update(gravity=(0.0, 0.0, g), dt=dt, domain_max=(0.05, 0.05, 1.0), num_prtcl=1)
+    e = make(oracle_lib, sc)
+    e.iterate(nst)
+    z = e.get_state()["posr"][0, 2]
+    vz = e.get_state()["lin_vel"][0, 0, 2]
+    # exact recurrences in rationals
+    G, DT, V0 = Fr(g), Fr(dt), Fr(v0)
+    coef = {native.PIM_FE: [Fr(1)], native.PIM_AB2: [Fr(3, 2), Fr(-1, 2)],
+            native.PIM_AB3: [Fr(23, 12), Fr(-16, 12), Fr(5, 12)]}[pi]
+    vh = [V0] + [Fr(0)] * (len(coef) - 1)
+    ah = [Fr(0)] * len(coef)
+    x = Fr(0.9)
+    for _ in range(nst):
+        ah = [G] + ah[:-1]
+        x += sum(c * v for c, v in zip(coef, vh)) * DT
+        vnew = vh[0] + sum(c * a for c, a in zip(coef, ah)) * DT
+        vh = [vnew] + vh[:-1]
+    assert abs(z - float(x)) < 1e-12 and abs(vz - float(vh[0])) < 1e-12
+
+
+def test_ab2_rotation_reads_linear_velocity(oracle_lib):
+    """Prtcl_Integration.f90:64: with AB2 rotation the angular velocity is
+    advanced from the *linear* velocity.  A free sphere with zero spin and
+    linear velocity u gets omega_1 = u_1 (+ torque-free terms)."""
+    sc = scenarios.two_spheres()
+    st = sc["state"]
+    for k in ("id", "ptype", "usr_mark"):
+        st[k] = st[k][:1]
+    for k in ("lin_vel", "lin_acc", "rot_vel", "rot_acc", "theta"):
+        st[k] = np.zeros_like(st[k][:1])
+    st["posr"] = np.array([[0.025, 0.025, 0.025, 1e-3]])
+    st["lin_vel"][0, 0, :] = (0.1, -0.2, 0.3)
+    e = make(oracle_lib, sc)
+    e.iterate(1)
+    s = e.get_state()
+    assert np.array_equal(s["rot_vel"][0, 0], s["lin_vel"][0, 0])
+    assert np.array_equal(s["rot_vel"][0, 1], s["lin_vel"][0, 0])
+
+
+# ---------------------------------------------------------------------------
+# contact set: brute force over all pairs and images
+# ---------------------------------------------------------------------------
+def brute_force_contacts(sc):
+    """All contacts of the state by exhaustive enumeration (no cells):
+    PP pairs (owner = lower id), PPG images (each side owns one), PPFix,
+    and PW over every wall (all walls are near at the first step)."""
+    st, o = sc["state"], sc["options"]
+    pos, r, ids = st["posr"][:, :3], st["posr"][:, 3], st["id"]
+    L = np.array(o["domain_max"]) - np.array(o["domain_min"])
+    per = np.array(o["is_periodic"], bool)
+    shifts = [np.array(s) * L for s in np.ndindex(3, 3, 3)]
+    shifts = [s - L for s in shifts]
+    recs = set()
+    n = len(ids)
+    for sh in shifts:
+        sh = np.where(per, sh, np.where(sh != 0, np.nan, 0.0))
+        if np.isnan(sh).any():
+            continue
+        img = pos + sh
+        d = pos[:, None, :] - img[None, :, :]
+        d2 = (d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2
+        dr = r[:, None] + r[None, :]
+        hit = d2 <= dr * dr
+        if not sh.any():
+            np.fill_diagonal(hit, False)
+        for i, j in zip(*np.nonzero(hit)):
+            if not sh.any():
+                if ids[i] < ids[j]:
+                    recs.add((int(ids[i]), int(ids[j]), native.KIND_PP))
+            else:
+                recs.add((int(ids[i]), int(ids[j]), native.KIND_PPG))
+    fx = sc.get("fixed")
+    if fx is not None and o.get("cs_method", 1) == native.CSM_NBS_MUNJIZA:
+        for sh in shifts:
+            sh = np.where(per, sh, np.where(sh != 0, np.nan, 0.0))
+            if np.isnan(sh).any():
+                continue
+            fp = fx["posr"][:, :3] + sh
+            d = pos[:, None, :] - fp[None, :, :]
+            d2 = (d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2
+            dr = r[:, None] + fx["posr"][None, :, 3]
+            for i, j in zip(*np.nonzero(d2 <= dr * dr)):
+                recs.add((int(ids[i]), int(fx["id"][j]), native.KIND_PPFIX))
+    return recs
+
+
+def contact_set(eng):
+    c = eng.get_contacts()
+    return {(int(a), int(b), int(k)) for a, b, k in zip(c["owner"], c["partner"], c["kind"])
+            if k != native.KIND_PW}
+
+
+@pytest.mark.parametrize("kw", [
+    dict(),
+    dict(periodic=(1, 0, 0), walls="yYzZ"),
+    dict(periodic=(1, 1, 1), walls=""),
+    dict(nfixed=30),
+    dict(diameters=(0.5e-3, 1.0e-3, 3.0e-3), cs_method=native.CSM_NBS_MUNJIZA_HRCHL, nfixed=10),
+    dict(diameters=(0.5e-3, 1.0e-3, 3.0e-3), periodic=(1, 1, 0), walls="z",
+         cs_method=native.CSM_NBS_MUNJIZA_HRCHL),
+])
+def test_contact_set_matches_brute_force(oracle_lib, kw):
+    sc = scenarios.random_cloud(n=500, seed=3, **kw)
+    e = make(oracle_lib, sc)
+    e.iterate(1)
+    got = contact_set(e)
+    want = brute_force_contacts(sc)
+    assert len(want) > 100
+    assert got == want, (len(got - want), len(want - got))
+
+
+def test_history_accumulates_tangential_displacement(oracle_lib):
+    """Two touching spheres sliding past each other with relative tangential
+    speed u (no normal motion, no slip): TanDelta grows by u*dt per step.
+    (A history of norm <= 1e-10 m is dropped, Prtcl_ContactForce_inc.f90:14,
+    so u*dt is kept well above that.)"""
+    d, u, dt = 2.0e-3, 0.05, 1e-7
+    sc = scenarios.two_spheres(d=d, v=0.0, gap=-1e-5, fric=10.0, dt=dt)
+    st = sc["state"]
+    st["lin_vel"][:, 0, :] = 0.0
+    st["lin_vel"][0, 0, 1] = u
+    e = make(oracle_lib, sc)
+    for k in range(1, 4):
+        e.iterate(1)
+        c = e.get_contacts()
+        assert len(c["kind"]) == 1
+        # the pair's relative tangential velocity at step k is still ~u
+        assert abs(c["tan_delta"][0, 1] - k * u * dt) < 1e-3 * k * u * dt, c["tan_delta"]
+
+
+def test_newton_third_law_and_torque_identity(oracle_lib):
+    """Without walls and gravity the PP forces cancel in sum, and for every
+    particle T = sum r_i (n x Ft) + Mr, so sum over particles of the torque
+    minus the moment of the contact forces about the centres vanishes when
+    rolling resistance is off: checked as sum F = 0."""
+    sc = scenarios.random_cloud(n=400, seed=9, walls="", roll=0.0)
+    sc["options"]["gravity"] = (0.0, 0.0, 0.0)
+    e = make(oracle_lib, sc)
+    e.iterate(1)
+    _, f, _ = e.get_forces()
+    assert np.abs(f.sum(0)).max() < 1e-10 * np.abs(f).max()
+
+
+def test_open_boundary_deletes_escaped_particles(oracle_lib):
+    sc = scenarios.random_cloud(n=50, seed=2, walls="")
+    st = sc["state"]
+    top = np.argmax(st["posr"][:, 2])
+    st["lin_vel"][top, :, 2] = 5.0e3  # leaves the domain in one step
+    e = make(oracle_lib, sc)
+    e.iterate(1)
+    ids = e.get_state()["id"]
+    assert st["id"][top] not in ids and len(ids) == 49
