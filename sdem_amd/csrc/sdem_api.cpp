@@ -981,7 +981,7 @@ static int collect(sdem_ctx* c, std::vector<Rec>& recs, std::vector<double4>& hv
     int k = pr.second;
     for (int q = 0; q < hc[k]; q++) {
       size_t slot = (size_t)k * c->hcap + q;
-      recs.push_back({pr.first, hk[slot] & KEY_MASK, hk[slot] >> KEY_BITS, slot});
+      recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), slot});
     }
   }
   std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {

@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) 
   }
 }
 
-// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances
+// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances (only
+// the two wall kernels read it)
 __global__ void k_wall_cadence(Params P, Scalars* S) {
   if (S->iter >= S->next_wall_update) {
     double max_v = __longlong_as_double((long long)S->max_v_bits) + P.max_wall_vel;
@@ -291,6 +292,7 @@ __global__ void k_wall_cadence(Params P, Scalars* S) {
     S->max_a_bits = 0ull;
   }
   S->use_host_max_a = 0;
+  S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
 }
 
 // (level, cell) key and per-key counts; dead particles go to the last key

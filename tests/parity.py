@@ -54,7 +54,10 @@ def compare(gpu: native.Engine, ref: native.Engine, exact_positions=True, rtol=R
     gc, rc = gpu.get_contacts(), ref.get_contacts()
     for k in ("kind", "owner", "partner"):
         if not np.array_equal(gc[k], rc[k]):
-            errs.append(f"contact {k} differs ({len(gc[k])} vs {len(rc[k])} records)")
+            gs = set(zip(gc["owner"].tolist(), gc["partner"].tolist(), gc["kind"].tolist()))
+            rs = set(zip(rc["owner"].tolist(), rc["partner"].tolist(), rc["kind"].tolist()))
+            errs.append(f"contact {k} differs ({len(gc[k])} vs {len(rc[k])} records); "
+                        f"only hip: {sorted(gs - rs)[:5]}, only oracle: {sorted(rs - gs)[:5]}")
             return errs
     ok, m = close(gc["tan_delta"], rc["tan_delta"], rtol)
     if not ok:
