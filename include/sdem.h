@@ -62,8 +62,9 @@ extern "C" {
 /* profiling phases reported by sdem_get_profile */
 #define SDEM_PHASE_WALL_LIST  0   /* UpdateNearPrtclsPW                         */
 #define SDEM_PHASE_BIN        1   /* cell keys + counting sort                  */
-#define SDEM_PHASE_STEP       2   /* fused contact/force/accel/integrate kernel */
-#define SDEM_NUM_PHASES       3
+#define SDEM_PHASE_SEARCH     2   /* contact search kernel                      */
+#define SDEM_PHASE_FORCE      3   /* history/force/accel/integrate kernel       */
+#define SDEM_NUM_PHASES       4
 
 /*
  * The DEMS_Options fields the step reads (Prtcl_Parameters.f90:45-95,
@@ -186,6 +187,16 @@ int sdem_get_iter(sdem_ctx* c, int32_t* iter_number, int32_t* next_wall_update);
 int sdem_set_profiling(sdem_ctx* c, int32_t enable);
 int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
                      int64_t launches[SDEM_NUM_PHASES]);
+
+/* Search and output tuning (no counterpart in the reference; results do not
+ * depend on it).  cell_factor[d] scales the search-cell edge (the reference's
+ * Prtcl_cs_ratio * d_max) in dimension d, 0 keeps the current value; the
+ * contact set is the same for any positive factors.  keep_forces = 0 stops
+ * writing the per-particle contact force / torque arrays that
+ * sdem_get_forces reads (the forces are still computed and integrated);
+ * -1 keeps the current setting.  Call before sdem_set_state for the cell
+ * factors to apply. */
+int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], int32_t keep_forces);
 
 /* Memory the context holds on the device, in bytes. */
 int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);

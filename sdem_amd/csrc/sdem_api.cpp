@@ -27,8 +27,9 @@ hipError_t launch_bin(const Params& P, const Buf& A, Scalars* S, int n_upper, in
                       double4* pk, void* scan_tmp, size_t scan_bytes, hipStream_t st);
 size_t scan_temp_bytes(int64_t nitems);
 hipError_t launch_step(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                       const int* perm, const double4* pk, const int* cstart, const double4* fpk,
-                       const int* fid, const int* ftype, const int* fstart, hipStream_t st);
+                       const int* perm, const double4* pk, const int* cstart, unsigned* clist,
+                       int* ccnt, const double4* fpk, const int* fid, const int* ftype,
+                       const int* fstart, hipStream_t st, hipEvent_t mid);
 }  // namespace sdem
 
 using namespace sdem;
@@ -201,6 +202,10 @@ struct sdem_ctx {
   int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr;
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
+  unsigned* clist = nullptr;
+  int* ccnt = nullptr;
+  double cell_factor[3] = {1.0, 1.0, 1.0};
+  int keep_forces = 1;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
   Bnry *d_pp = nullptr, *d_pw = nullptr;
@@ -218,8 +223,8 @@ struct sdem_ctx {
   std::vector<int> type_level;
 
   bool prof = false;
-  double prof_ms[SDEM_NUM_PHASES] = {0, 0, 0};
-  int64_t prof_n[SDEM_NUM_PHASES] = {0, 0, 0};
+  double prof_ms[SDEM_NUM_PHASES] = {};
+  int64_t prof_n[SDEM_NUM_PHASES] = {};
   std::vector<void*> allocs_state, allocs_grid, allocs_tab;
   int64_t dev_bytes = 0;
 
@@ -311,17 +316,21 @@ std::vector<double> plan_levels(sdem_ctx* c, const std::vector<int>& count_per_t
   return cells;
 }
 
-int make_grid(sdem_ctx* c, double cell_len, double rmax, int64_t key_base, Grid* G) {
+int make_grid(sdem_ctx* c, double cell_len0, double rmax, int64_t key_base, Grid* G) {
   G->key_base = key_base;
   G->rmax = rmax;
   G->ncell = 1;
   for (int d = 0; d < 3; d++) {
     double L = c->o.domain_max[d] - c->o.domain_min[d];
+    double cell_len = cell_len0 * c->cell_factor[d];
     G->origin[d] = c->o.domain_min[d];
     if (c->o.is_periodic[d]) {
       int n = (int)std::floor(L / cell_len);
-      if (n < 3)
-        return c->fail(SDEM_E_ARG, "a periodic dimension must hold at least 3 search cells");
+      // the search range of one particle must not wrap onto itself
+      double reach = 2.0 * max_type_radius(c) + rmax;
+      int span = (int)std::floor(2.0 * reach * (1.0 + 1e-6) / (L / n)) + 2;
+      if (n < 3 || span > n)
+        return c->fail(SDEM_E_ARG, "periodic dimension too short for the search range");
       G->n[d] = n;
       G->w[d] = L / n;
       G->inv_w[d] = (double)n / L;
@@ -559,6 +568,8 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm_tmp, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->pk, c->cap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
     if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(sort)");
   }
   const int cap = c->cap, ts = c->tsize, rs = c->rsize;
@@ -715,7 +726,7 @@ static int prepare(sdem_ctx* c) {
   P.nwtype = c->nwtype;
   P.nwall = (int)c->walls.size();
   P.search_fixed = c->o.cs_method == SDEM_CSM_NBS_MUNJIZA ? 1 : 0;
-  P.keep_forces = 1;
+  P.keep_forces = c->keep_forces;
   double rmax = max_type_radius(c);
   P.dxmax1 = rmax * c->o.wall_neighbor_ratio;
   P.dxmax2 = rmax * (c->o.wall_neighbor_ratio + 1.0);
@@ -793,16 +804,17 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
   if (!c->have_state) return c->fail(SDEM_E_STATE, "state must be set before stepping");
   if (nsteps < 0) return c->fail(SDEM_E_ARG, "nsteps < 0");
   if (int rc = prepare(c)) return rc;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  const int NE = SDEM_NUM_PHASES + 1;
+  hipEvent_t ev[SDEM_NUM_PHASES + 1] = {};
   std::vector<hipEvent_t> evs;
-  if (c->prof) evs.resize((size_t)nsteps * 4);
+  if (c->prof) evs.resize((size_t)nsteps * NE);
   for (auto& x : evs)
     if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
   hipError_t e = hipSuccess;
   for (int it = 0; it < nsteps && e == hipSuccess; it++) {
     Buf& A = c->buf[c->cur];
     Buf& B = c->buf[1 - c->cur];
-    if (c->prof) for (int q = 0; q < 4; q++) ev[q] = evs[(size_t)it * 4 + q];
+    if (c->prof) for (int q = 0; q < NE; q++) ev[q] = evs[(size_t)it * NE + q];
     if (c->prof) (void)hipEventRecord(ev[0], c->stream);
     e = launch_begin(c->S, c->stream);
     if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->n_upper, c->stream);
@@ -812,9 +824,10 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
                      c->perm_tmp, c->perm, c->pk, c->scan_tmp, c->scan_bytes, c->stream);
     if (c->prof) (void)hipEventRecord(ev[2], c->stream);
     if (e == hipSuccess)
-      e = launch_step(c->P, A, B, c->S, c->n_upper, c->perm, c->pk, c->start, c->d_fpk, c->d_fid,
-                      c->d_ftype, c->d_fstart, c->stream);
-    if (c->prof) (void)hipEventRecord(ev[3], c->stream);
+      e = launch_step(c->P, A, B, c->S, c->n_upper, c->perm, c->pk, c->start, c->clist, c->ccnt,
+                      c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->stream,
+                      c->prof ? ev[3] : nullptr);
+    if (c->prof) (void)hipEventRecord(ev[4], c->stream);
     c->cur = 1 - c->cur;
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -823,10 +836,10 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     return hip_fail(c, e, "sdem_step");
   }
   if (nsteps > 0) c->stepped = true;
-  for (int it = 0; it < (int)evs.size() / 4; it++) {
-    for (int q = 0; q < 3; q++) {
+  for (int it = 0; it < (int)evs.size() / NE; it++) {
+    for (int q = 0; q < SDEM_NUM_PHASES; q++) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, evs[(size_t)it * 4 + q], evs[(size_t)it * 4 + q + 1]) == hipSuccess) {
+      if (hipEventElapsedTime(&ms, evs[(size_t)it * NE + q], evs[(size_t)it * NE + q + 1]) == hipSuccess) {
         c->prof_ms[q] += ms;
         c->prof_n[q]++;
       }
@@ -1060,6 +1073,21 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES], int64_t launches[S
   for (int q = 0; q < SDEM_NUM_PHASES; q++) {
     if (ms) ms[q] = c->prof_ms[q];
     if (launches) launches[q] = c->prof_n[q];
+  }
+  return SDEM_OK;
+}
+
+int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], int32_t keep_forces) {
+  if (!c) return SDEM_E_ARG;
+  if (cell_factor) {
+    for (int d = 0; d < 3; d++)
+      if (cell_factor[d] < 0.0 || cell_factor[d] > 64.0) return c->fail(SDEM_E_ARG, "cell factor out of range");
+    for (int d = 0; d < 3; d++)
+      if (cell_factor[d] > 0.0) c->cell_factor[d] = cell_factor[d];
+  }
+  if (keep_forces >= 0) {
+    c->keep_forces = keep_forces ? 1 : 0;
+    c->P.keep_forces = c->keep_forces;
   }
   return SDEM_OK;
 }

@@ -9,8 +9,9 @@
 //   scan                    : exclusive scan of the counts (rocPRIM)
 //   k_scatter / k_rank      : deterministic counting sort (in-cell order =
 //                             storage order) + packed (x,y,z,r) in cell order
-//   k_step         N        : fused contact search, history match, contact
-//                             forces, accelerations, integration, wrap/delete
+//   k_search       N        : contact search into a per-particle partner list
+//   k_force        N        : history match, contact forces, accelerations,
+//                             integration, wrap/delete
 //
 // All floating-point kernels are built with -ffp-contract=off so that every
 // product and sum is rounded on its own, in the order the reference's
@@ -295,22 +296,37 @@ __global__ void k_wall_cadence(Params P, Scalars* S) {
   S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
 }
 
-// (level, cell) key and per-key counts; dead particles go to the last key
+// (level, cell) key and per-key counts; dead particles go to the last key.
+// Lanes of a wave holding the same key (neighbours in storage order usually
+// share a cell) take their slots with one atomic per run.
 __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_t dead_key,
                                              int* __restrict__ keys, int* __restrict__ rank,
                                              int* __restrict__ counts) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S->n_stored) return;
-  int t = A.type[s];
-  int64_t key;
-  if (t < 1) {
-    key = dead_key;
-  } else {
-    const Grid& G = P.lvl[P.type_level[t - 1]];
-    key = cell_key(G, A.x[s], A.y[s], A.z[s]);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = s < S->n_stored;
+  int key = -1 - lane;  // unique per lane when inactive
+  if (valid) {
+    int t = A.type[s];
+    if (t < 1) {
+      key = (int)dead_key;
+    } else {
+      const Grid& G = P.lvl[P.type_level[t - 1]];
+      key = (int)cell_key(G, A.x[s], A.y[s], A.z[s]);
+    }
+    keys[s] = key;
   }
-  keys[s] = (int)key;
-  rank[s] = atomicAdd(&counts[key], 1);
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = (lane == 0) || (prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int my_head = 63 - __clzll((long long)(heads & le));
+  const unsigned long long after = heads & ~le;
+  const int next_head = after ? (__ffsll((long long)after) - 1) : 64;
+  int base = 0;
+  if (head && valid) base = atomicAdd(&counts[key], next_head - lane);
+  base = __shfl(base, my_head, 64);
+  if (valid) rank[s] = base + (lane - my_head);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __restrict__ keys,
@@ -386,14 +402,104 @@ __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, 
   acc.nkind[kind - 1]++;
 }
 
-__global__ __launch_bounds__(256) void k_step(Params P, Buf A, Buf B, Scalars* S,
-                                              const int* __restrict__ perm,
-                                              const double4* __restrict__ pk,
-                                              const int* __restrict__ cstart,
-                                              const double4* __restrict__ fpk,
-                                              const int* __restrict__ fid,
-                                              const int* __restrict__ ftype,
-                                              const int* __restrict__ fstart) {
+// Contact search: every particle gathers all its contacts (both roles) over
+// the cells of every level that overlap [x - reach, x + reach], reach =
+// r_i + largest radius of the level.  The narrow-phase test is the
+// reference's (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557):
+// contact iff ((dx*dx + dy*dy) + dz*dz) <= (ri+rj)^2.  Found partners are
+// stored as (sorted index | periodic shift code) in traversal order.
+constexpr int SHIFT_BITS = 26;
+constexpr unsigned IDX_MASK = (1u << SHIFT_BITS) - 1u;
+
+__global__ __launch_bounds__(256) void k_search(Params P, Scalars* S, const double4* __restrict__ pk,
+                                                const int* __restrict__ cstart,
+                                                unsigned* __restrict__ clist, int* __restrict__ ccnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S->n_alive) return;
+  const double4 pi = pk[k];
+  const double xv[3] = {pi.x, pi.y, pi.z};
+  const double ri = pi.w;
+  const int cap = P.hcap;
+  unsigned* out = clist + (size_t)k * cap;
+  int c = 0;
+  for (int L = 0; L < P.nlvl; L++) {
+    const Grid& G = P.lvl[L];
+    const double reach = ri + G.rmax;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      const double pad = 1.0e-6 * G.w[d];
+      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+      if (!P.periodic[d]) {
+        lo[d] = lo[d] < 0 ? 0 : lo[d];
+        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      }
+    }
+    for (int cz = lo[2]; cz <= hi[2]; cz++) {
+      int wz = cz;
+      unsigned sz = 0;
+      double shz = 0.0;
+      if (wz < 0) { wz += G.n[2]; sz = 1; shz = -P.simlen[2]; }
+      else if (wz >= G.n[2]) { wz -= G.n[2]; sz = 2; shz = P.simlen[2]; }
+      for (int cy = lo[1]; cy <= hi[1]; cy++) {
+        int wy = cy;
+        unsigned sy = 0;
+        double shy = 0.0;
+        if (wy < 0) { wy += G.n[1]; sy = 1; shy = -P.simlen[1]; }
+        else if (wy >= G.n[1]) { wy -= G.n[1]; sy = 2; shy = P.simlen[1]; }
+        const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
+        for (int seg = 0; seg < 3; seg++) {
+          int xa, xb;
+          unsigned sx;
+          double shx;
+          if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; sx = 1; shx = -P.simlen[0]; }
+          else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; sx = 0; shx = 0.0; }
+          else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; sx = 2; shx = P.simlen[0]; }
+          if (xa > xb) continue;
+          if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
+          if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
+          const unsigned code = (sx | (sy << 2) | (sz << 4)) << SHIFT_BITS;
+          const bool shifted = code != 0u;
+          const int ja = cstart[rowkey + xa], jb = cstart[rowkey + xb + 1];
+          for (int kj = ja; kj < jb; kj++) {
+            const double4 pj = pk[kj];
+            // image position = stored position + shift, as the reference's
+            // ghost copies hold fl(x +- L) (Prtcl_Comm.f90:242,252)
+            const double xj = sx ? pj.x + shx : pj.x;
+            const double yj = sy ? pj.y + shy : pj.y;
+            const double zj = sz ? pj.z + shz : pj.z;
+            const double dr = ri + pj.w;
+            const double dr2 = dr * dr;
+            const double ddx = pi.x - xj;
+            double d2 = ddx * ddx;
+            if (d2 > dr2) continue;
+            const double ddy = pi.y - yj;
+            d2 = ddy * ddy + d2;
+            if (d2 > dr2) continue;
+            const double ddz = pi.z - zj;
+            d2 = ddz * ddz + d2;
+            if (d2 > dr2) continue;
+            if (kj == k && !shifted) continue;
+            if (c < cap) out[c] = (unsigned)kj | code;
+            c++;
+          }
+        }
+      }
+    }
+  }
+  ccnt[k] = c;
+  if (c > cap) atomicMax(&S->overflow, 1);
+}
+
+__global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* S,
+                                               const int* __restrict__ perm,
+                                               const double4* __restrict__ pk,
+                                               const unsigned* __restrict__ clist,
+                                               const int* __restrict__ ccnt,
+                                               const double4* __restrict__ fpk,
+                                               const int* __restrict__ fid,
+                                               const int* __restrict__ ftype,
+                                               const int* __restrict__ fstart) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = S->n_alive;
   Acc acc;
@@ -414,106 +520,68 @@ __global__ __launch_bounds__(256) void k_step(Params P, Buf A, Buf B, Scalars* S
     const d3 wi = ld3(A.rv, cap, s);
     const double xv[3] = {pi.x, pi.y, pi.z};
 
-    // ---------------- moving particles of every search level -------------
-    for (int L = 0; L < P.nlvl; L++) {
-      const Grid& G = P.lvl[L];
-      const double reach = ri + G.rmax;
-      int lo[3], hi[3];
-      for (int d = 0; d < 3; d++) {
-        double pad = 1.0e-6 * G.w[d];
-        lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-        hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-        if (!P.periodic[d]) {
-          lo[d] = lo[d] < 0 ? 0 : lo[d];
-          hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-        }
+    // ---------------- moving particles found by k_search -----------------
+    const int nc = min(ccnt[k], P.hcap);
+    const unsigned* lst = clist + (size_t)k * P.hcap;
+    for (int q = 0; q < nc; q++) {
+      const unsigned e = lst[q];
+      const int kj = (int)(e & IDX_MASK);
+      const unsigned code = e >> SHIFT_BITS;
+      const bool shifted = code != 0u;
+      const double4 pj = pk[kj];
+      double xj = pj.x, yj = pj.y, zj = pj.z;
+      if (shifted) {
+        const unsigned sx = code & 3u, sy = (code >> 2) & 3u, sz = (code >> 4) & 3u;
+        if (sx) xj = pj.x + (sx == 1u ? -P.simlen[0] : P.simlen[0]);
+        if (sy) yj = pj.y + (sy == 1u ? -P.simlen[1] : P.simlen[1]);
+        if (sz) zj = pj.z + (sz == 1u ? -P.simlen[2] : P.simlen[2]);
       }
-      for (int cz = lo[2]; cz <= hi[2]; cz++) {
-        int wz = cz;
-        double shz = 0.0;
-        if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
-        else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
-        for (int cy = lo[1]; cy <= hi[1]; cy++) {
-          int wy = cy;
-          double shy = 0.0;
-          if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
-          else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
-          const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
-          // split the x range into up to three contiguous key segments
-          for (int seg = 0; seg < 3; seg++) {
-            int xa, xb;
-            double shx;
-            if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; shx = -P.simlen[0]; }
-            else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; shx = 0.0; }
-            else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; shx = P.simlen[0]; }
-            if (xa > xb) continue;
-            int wa = xa, wb = xb;
-            if (seg == 0) { wa += G.n[0]; wb += G.n[0]; }
-            if (seg == 2) { wa -= G.n[0]; wb -= G.n[0]; }
-            const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
-            const int ja = cstart[rowkey + wa], jb = cstart[rowkey + wb + 1];
-            for (int kj = ja; kj < jb; kj++) {
-              if (kj == k && !shifted) continue;
-              const double4 pj = pk[kj];
-              double xj = pj.x, yj = pj.y, zj = pj.z;
-              if (shifted) { xj = xj + shx; yj = yj + shy; zj = zj + shz; }
-              // narrow phase, NBSM_FineSearch2/3 (Prtcl_NBS_Munjiza.f90:503-557)
-              const double dr = ri + pj.w;
-              const double dr2 = dr * dr;
-              const double ddx = xi.x - xj;
-              double d2 = ddx * ddx;
-              if (d2 > dr2) continue;
-              const double ddy = xi.y - yj;
-              d2 = ddy * ddy + d2;
-              if (d2 > dr2) continue;
-              const double ddz = xi.z - zj;
-              d2 = ddz * ddz + d2;
-              if (d2 > dr2) continue;
-              const double ovrlp = dr - sqrt(d2);
-              const int sj = perm[kj];
-              const int idj = A.id[sj];
-              const int tj = A.type[sj];
-              const d3 vj = ld3(A.lv, cap, sj);
-              const d3 wj = ld3(A.rv, cap, sj);
-              const d3 xjv = D3(xj, yj, zj);
-              if (!shifted) {
-                // PP: record in the lower id's bucket, force from its side
-                // (FineSearch2 :525-529, ContactForce_PP :732-756)
-                const bool ilo = idi < idj;
-                const Hist h = ilo ? find_hist(A, P.hcap, s, idj) : find_hist(A, P.hcap, sj, idi);
-                const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
-                const Bnry& Bp = P.pp[(tlo - 1) * P.ntype + (thi - 1)];
-                CF cf;
-                if (ilo) {
-                  cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
-                  acc.F = add3(acc.F, cf.fsum);
-                  acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-                  push_hist(P, B, k, acc, 1, idj, cf);
-                } else {
-                  cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
-                  acc.F = sub3(acc.F, cf.fsum);
-                  acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-                }
-              } else {
-                // PPG: each side keeps its own record against the other's
-                // image (ContactForce_PPG :761-798); row type = local particle
-                const Hist h = find_hist(A, P.hcap, s, idj);
-                const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (tj - 1)];
-                CF cf;
-                if (idi < idj) {
-                  cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
-                  acc.F = add3(acc.F, cf.fsum);
-                  acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-                } else {
-                  cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
-                  acc.F = sub3(acc.F, cf.fsum);
-                  acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-                }
-                push_hist(P, B, k, acc, 2, idj, cf);
-              }
-            }
-          }
+      const double dr = ri + pj.w;
+      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
+      double d2 = ddx * ddx;
+      d2 = ddy * ddy + d2;
+      d2 = ddz * ddz + d2;
+      const double ovrlp = dr - sqrt(d2);
+      const int sj = perm[kj];
+      const int idj = A.id[sj];
+      const int tj = A.type[sj];
+      const d3 vj = ld3(A.lv, cap, sj);
+      const d3 wj = ld3(A.rv, cap, sj);
+      const d3 xjv = D3(xj, yj, zj);
+      if (!shifted) {
+        // PP: record in the lower id's bucket, force from its side
+        // (FineSearch2 :525-529, ContactForce_PP :732-756)
+        const bool ilo = idi < idj;
+        const Hist h = ilo ? find_hist(A, P.hcap, s, idj) : find_hist(A, P.hcap, sj, idi);
+        const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
+        const Bnry& Bp = P.pp[(tlo - 1) * P.ntype + (thi - 1)];
+        CF cf;
+        if (ilo) {
+          cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
+          acc.F = add3(acc.F, cf.fsum);
+          acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+          push_hist(P, B, k, acc, 1, idj, cf);
+        } else {
+          cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
+          acc.F = sub3(acc.F, cf.fsum);
+          acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
         }
+      } else {
+        // PPG: each side keeps its own record against the other's image
+        // (ContactForce_PPG :761-798); row type = the local particle
+        const Hist h = find_hist(A, P.hcap, s, idj);
+        const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (tj - 1)];
+        CF cf;
+        if (idi < idj) {
+          cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
+          acc.F = add3(acc.F, cf.fsum);
+          acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+        } else {
+          cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
+          acc.F = sub3(acc.F, cf.fsum);
+          acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+        }
+        push_hist(P, B, k, acc, 2, idj, cf);
       }
     }
 
@@ -758,11 +826,16 @@ size_t scan_temp_bytes(int64_t nitems) {
 }
 
 hipError_t launch_step(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                       const int* perm, const double4* pk, const int* cstart, const double4* fpk,
-                       const int* fid, const int* ftype, const int* fstart, hipStream_t st) {
-  if (n_upper > 0)
-    hipLaunchKernelGGL(k_step, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, pk,
-                       cstart, fpk, fid, ftype, fstart);
+                       const int* perm, const double4* pk, const int* cstart, unsigned* clist,
+                       int* ccnt, const double4* fpk, const int* fid, const int* ftype,
+                       const int* fstart, hipStream_t st, hipEvent_t mid) {
+  if (n_upper > 0) {
+    hipLaunchKernelGGL(k_search, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, S, pk, cstart, clist,
+                       ccnt);
+    if (mid) (void)hipEventRecord(mid, st);
+    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, pk,
+                       clist, ccnt, fpk, fid, ftype, fstart);
+  }
   return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ CFT_LSD, CFT_NLIN = 1, 2
 CTM_CONSTANT_TORQUE, CTM_VARIABLE_TORQUE = 1, 2
 PIM_FE, PIM_AB2, PIM_AB3 = 1, 2, 3
 KIND_PP, KIND_PPG, KIND_PPFIX, KIND_PW = 1, 2, 3, 4
-PHASE_NAMES = ("wall_list", "bin_sort", "step")
+PHASE_NAMES = ("wall_list", "bin_sort", "search", "force")
 
 
 class SdemError(RuntimeError):
@@ -110,6 +110,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "set_profiling": ([V, C.c_int32], C.c_int),
         "get_profile": ([V, _DP, _LP], C.c_int),
         "get_device_bytes": ([V, _LP], C.c_int),
+        "set_tuning": ([V, _DP, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = f(name)
@@ -281,9 +282,13 @@ class Engine:
     def set_profiling(self, enable: bool) -> None:
         self._check(self._fn("set_profiling")(self.h, int(bool(enable))), "set_profiling")
 
+    def set_tuning(self, cell_factor=None, keep_forces=-1) -> None:
+        cf = None if cell_factor is None else _f64(cell_factor)
+        self._check(self._fn("set_tuning")(self.h, _dptr(cf), int(keep_forces)), "set_tuning")
+
     def get_profile(self):
-        ms = np.zeros(3)
-        n = np.zeros(3, np.int64)
+        ms = np.zeros(len(PHASE_NAMES))
+        n = np.zeros(len(PHASE_NAMES), np.int64)
         self._check(self._fn("get_profile")(self.h, _dptr(ms), n.ctypes.data_as(_LP)), "get_profile")
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(PHASE_NAMES)}
 
@@ -317,11 +322,13 @@ def make_options(lib: C.CDLL, prefix: str = "sdem_", **kw) -> Options:
     return o
 
 
-def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_") -> Engine:
+def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None) -> Engine:
     """Create an engine and push a scenario (see sdem_amd.scenarios) into it,
     in the reference's initialisation order (DEMS_Initialize :58-195)."""
     o = make_options(lib, prefix, **scenario["options"])
     eng = Engine(lib, o, prefix)
+    if tuning and hasattr(lib, prefix + "set_tuning"):
+        eng.set_tuning(**tuning)
     pt = scenario["ptypes"]
     eng.set_particle_property(pt["diameter"], pt["density"], pt["youngs"], pt["poisson"], *scenario["pp"])
     wt = scenario["wtypes"]
