@@ -190,13 +190,25 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
 
 /* Search and output tuning (no counterpart in the reference; results do not
  * depend on it).  cell_factor[d] scales the search-cell edge (the reference's
- * Prtcl_cs_ratio * d_max) in dimension d, 0 keeps the current value; the
- * contact set is the same for any positive factors.  keep_forces = 0 stops
- * writing the per-particle contact force / torque arrays that
- * sdem_get_forces reads (the forces are still computed and integrated);
- * -1 keeps the current setting.  Call before sdem_set_state for the cell
- * factors to apply. */
-int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], int32_t keep_forces);
+ * Prtcl_cs_ratio * d_max) in dimension d, 0 keeps the current value.
+ * skin_factor (fraction of d_max, below 0.45; negative keeps the current
+ * value, default 0.1) is the margin of the candidate lists, which are rebuilt
+ * with the cell sort only after some particle has moved half the skin; the
+ * contact set is decided every step with the exact predicate, so it is the
+ * same for any factors.  keep_forces = 0 stops writing the per-particle
+ * contact force / torque arrays that sdem_get_forces reads (the forces are
+ * still computed and integrated); -1 keeps the current setting.  Call before
+ * sdem_set_state for the cell factors to apply. */
+int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor,
+                    int32_t keep_forces);
+
+/* Initial capacities (no counterpart in the reference, whose contact list
+ * holds CntctList_Size * mlocal records in one pool): hist_slots history
+ * records per particle (0 = a kissing-number bound from the type radii) and
+ * list_entries candidate-list entries in total (0 = 32 per particle).  Both
+ * grow on demand: a step that runs out is replayed from its intact input
+ * after the storage has been enlarged.  Call before sdem_set_state. */
+int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int64_t list_entries);
 
 /* Memory the context holds on the device, in bytes. */
 int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);

@@ -19,17 +19,19 @@
 #include "sdem_internal.h"
 
 namespace sdem {
-hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
-hipError_t launch_begin(Scalars* S, hipStream_t st);
+hipError_t launch_begin(const Params& P, Scalars* S, hipStream_t st);
 hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, int n_upper, hipStream_t st);
-hipError_t launch_bin(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
-                      int* keys, int* rank, int* counts, int* start, int* perm_tmp, int* perm,
-                      double4* pk, void* scan_tmp, size_t scan_bytes, hipStream_t st);
-size_t scan_temp_bytes(int64_t nitems);
-hipError_t launch_step(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                       const int* perm, const double4* pk, const int* cstart, unsigned* clist,
-                       int* ccnt, const double4* fpk, const int* fid, const int* ftype,
-                       const int* fstart, hipStream_t st, hipEvent_t mid);
+hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
+                       int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
+                       int* perm, double4* pk, hipStream_t st);
+hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
+                        const int* cstart, int* vcnt, int* voff, int* vlist, int* tile_sum,
+                        hipStream_t st);
+hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
+                        const int* perm, const int* voff, const int* vlist, double4* ref,
+                        const double4* fpk, const int* fid, const int* ftype, const int* fstart,
+                        hipStream_t st);
+int64_t scan_tiles(int64_t n);
 }  // namespace sdem
 
 using namespace sdem;
@@ -202,12 +204,17 @@ struct sdem_ctx {
   int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr;
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
-  unsigned* clist = nullptr;
-  int* ccnt = nullptr;
+  int *vcnt = nullptr, *voff = nullptr, *vlist = nullptr;
+  double4* ref = nullptr;
+  int* vlist_own = nullptr;
+  int64_t vcap = 0;
   double cell_factor[3] = {1.0, 1.0, 1.0};
+  double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
   int keep_forces = 1;
-  void* scan_tmp = nullptr;
-  size_t scan_bytes = 0;
+  bool first_step_pending = false;
+  int hist_slots_hint = 0;
+  int64_t list_entries_hint = 0;
+  int* tile_sum = nullptr;
   Bnry *d_pp = nullptr, *d_pw = nullptr;
   double *d_mass = nullptr, *d_inertia = nullptr;
   Wall* d_walls = nullptr;
@@ -515,7 +522,7 @@ static int alloc_buf(sdem_ctx* c, Buf& b) {
   size_t cap = (size_t)c->cap, hc = (size_t)c->hcap;
   hipError_t e = hipSuccess;
   auto A = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
-  A(&b.x, cap); A(&b.y, cap); A(&b.z, cap); A(&b.r, cap);
+  A(&b.pos, cap);
   A(&b.lv, cap * 3 * c->tsize); A(&b.la, cap * 3 * c->ta);
   A(&b.th, cap * 3); A(&b.rv, cap * 3 * c->rsize); A(&b.ra, cap * 3 * c->tr);
   A(&b.id, cap); A(&b.type, cap); A(&b.mark, cap); A(&b.wmask, cap);
@@ -559,7 +566,7 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   // (re)allocate state buffers on first use
   if (c->allocs_state.empty()) {
     c->cap = c->o.num_prtcl;
-    c->hcap = history_capacity(c);
+    c->hcap = c->hist_slots_hint > 0 ? c->hist_slots_hint : history_capacity(c);
     for (int b = 0; b < 2; b++)
       if (int r = alloc_buf(c, c->buf[b])) return r;
     hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
@@ -568,8 +575,12 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm_tmp, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->pk, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vcnt, (size_t)c->cap + 2);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->voff, (size_t)c->cap + 2);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ref, c->cap);
+    c->vcap = c->list_entries_hint > 0 ? c->list_entries_hint : (int64_t)c->cap * 32 + 1024;
+    if (e == hipSuccess) e = hipMalloc((void**)&c->vlist, (size_t)c->vcap * 4);
+    if (e == hipSuccess) c->dev_bytes += c->vcap * 4;
     if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(sort)");
   }
   const int cap = c->cap, ts = c->tsize, rs = c->rsize;
@@ -611,7 +622,9 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   auto H = [&](void* dst, const void* src, size_t bytes) {
     if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
-  H(A.x, x.data(), n * 8); H(A.y, y.data(), n * 8); H(A.z, z.data(), n * 8); H(A.r, r.data(), n * 8);
+  std::vector<double4> p4(n);
+  for (int i = 0; i < n; i++) p4[i] = make_double4(x[i], y[i], z[i], r[i]);
+  H(A.pos, p4.data(), (size_t)n * sizeof(double4));
   H(A.lv, lv.data(), lv.size() * 8); H(A.la, la.data(), la.size() * 8); H(A.th, th.data(), th.size() * 8);
   H(A.rv, rv.data(), rv.size() * 8); H(A.ra, ra.data(), ra.size() * 8);
   H(A.id, id, n * 4); H(A.type, ptype, n * 4); H(A.mark, mark.data(), n * 4);
@@ -624,6 +637,7 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   s.next_wall_update = 0;
   s.use_host_max_a = 1;
   s.host_max_a = host_max_a;
+  s.need_rebuild = 1;
   H(c->S, &s, sizeof(s));
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
   // search levels: radii never change, so the per-level largest radius is
@@ -640,6 +654,7 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   c->have_state = true;
   c->stepped = false;
   c->prepared = false;
+  c->first_step_pending = true;
   // plan the grids now so configuration errors surface here
   free_list(c, c->allocs_grid);
   int64_t base = 0;
@@ -653,8 +668,8 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   c->dead_key = base;
   e = dalloc(c, c->allocs_grid, &c->counts, (size_t)base + 2);
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)base + 2);
-  c->scan_bytes = scan_temp_bytes(base + 2);
-  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, (char**)&c->scan_tmp, c->scan_bytes);
+  int64_t nt = std::max(scan_tiles(base + 2), scan_tiles((int64_t)c->cap + 2));
+  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)nt + 1);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(grid)");
   return SDEM_OK;
 }
@@ -734,6 +749,9 @@ static int prepare(sdem_ctx* c) {
   for (auto& w : c->walls) mwv = fmx(mwv, vnorm(v3(w.trans_vel)));
   P.max_wall_vel = mwv;
   P.wall_max_update_iter = c->o.wall_max_update_iter;
+  P.skin = c->skin_factor * maxd;
+  P.rebuild_disp = 0.5 * P.skin * (1.0 - 1.0e-6);
+  P.vcap = (int)std::min<int64_t>(c->vcap, INT32_MAX);
   for (auto& w : c->walls)
     if (w.wall_type > c->nwtype) return c->fail(SDEM_E_ARG, "wall_type exceeds numWall_type");
   std::vector<double> mass(c->ntype), inertia(c->ntype);
@@ -799,57 +817,157 @@ static int prepare(sdem_ctx* c) {
   return SDEM_OK;
 }
 
+template <class T>
+static hipError_t dl(std::vector<T>& v, const T* src, size_t n) {
+  v.resize(n);
+  if (!n) return hipSuccess;
+  return hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
+}
+
+// Replace a tracked device allocation by a larger one (contents not kept).
+template <class T>
+static hipError_t regrow(sdem_ctx* c, T** p, size_t n) {
+  auto& L = c->allocs_state;
+  auto it = std::find(L.begin(), L.end(), (void*)*p);
+  if (it != L.end()) L.erase(it);
+  (void)hipFree(*p);
+  *p = nullptr;
+  return dalloc(c, L, p, n);
+}
+
+// History slots per particle doubled; the history of buffer A is re-strided.
+static int grow_history(sdem_ctx* c) {
+  const int old = c->hcap, neu = std::min(old * 2, 4096);
+  if (neu <= old) return c->fail(SDEM_E_CAPACITY, "contact history slots exhausted for a particle");
+  const size_t n = (size_t)c->n_upper;
+  Buf& A = c->buf[c->cur];
+  std::vector<int> hc, hk;
+  std::vector<double4> hv;
+  hipError_t e = dl(hc, A.hcnt, n);
+  if (e == hipSuccess) e = dl(hk, A.hkey, n * old);
+  if (e == hipSuccess) e = dl(hv, A.hval, n * old);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
+  std::vector<int> hk2(n * neu, 0);
+  std::vector<double4> hv2(n * neu);
+  for (size_t k = 0; k < n; k++)
+    for (int q = 0; q < std::min(hc[k], old); q++) {
+      hk2[k * neu + q] = hk[k * old + q];
+      hv2[k * neu + q] = hv[k * old + q];
+    }
+  for (int b = 0; b < 2; b++) {
+    if (e == hipSuccess) e = regrow(c, &c->buf[b].hkey, (size_t)c->cap * neu);
+    if (e == hipSuccess) e = regrow(c, &c->buf[b].hval, (size_t)c->cap * neu);
+  }
+  if (e == hipSuccess) e = hipMemcpy(A.hkey, hk2.data(), hk2.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(A.hval, hv2.data(), hv2.size() * sizeof(double4), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "grow history");
+  c->hcap = neu;
+  c->P.hcap = neu;
+  return SDEM_OK;
+}
+
+// Candidate-list storage sized to what the failed rebuild counted.
+static int grow_vlist(sdem_ctx* c) {
+  int need = 0;
+  hipError_t e = hipMemcpy(&need, c->voff + c->n_upper, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(voff)");
+  int64_t neu = std::max<int64_t>((int64_t)need + need / 4 + 1024, c->vcap * 2);
+  if (neu > INT32_MAX) return c->fail(SDEM_E_CAPACITY, "candidate lists exceed 2^31 entries");
+  (void)hipFree(c->vlist);
+  c->dev_bytes -= c->vcap * 4;
+  c->vlist = nullptr;
+  e = hipMalloc((void**)&c->vlist, (size_t)neu * 4);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
+  c->dev_bytes += neu * 4;
+  c->vcap = neu;
+  c->P.vcap = (int)neu;
+  return SDEM_OK;
+}
+
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
   if (!c) return SDEM_E_ARG;
   if (!c->have_state) return c->fail(SDEM_E_STATE, "state must be set before stepping");
   if (nsteps < 0) return c->fail(SDEM_E_ARG, "nsteps < 0");
   if (int rc = prepare(c)) return rc;
   const int NE = SDEM_NUM_PHASES + 1;
-  hipEvent_t ev[SDEM_NUM_PHASES + 1] = {};
-  std::vector<hipEvent_t> evs;
-  if (c->prof) evs.resize((size_t)nsteps * NE);
-  for (auto& x : evs)
-    if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
-  hipError_t e = hipSuccess;
-  for (int it = 0; it < nsteps && e == hipSuccess; it++) {
-    Buf& A = c->buf[c->cur];
-    Buf& B = c->buf[1 - c->cur];
-    if (c->prof) for (int q = 0; q < NE; q++) ev[q] = evs[(size_t)it * NE + q];
-    if (c->prof) (void)hipEventRecord(ev[0], c->stream);
-    e = launch_begin(c->S, c->stream);
-    if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->n_upper, c->stream);
-    if (c->prof) (void)hipEventRecord(ev[1], c->stream);
-    if (e == hipSuccess)
-      e = launch_bin(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
-                     c->perm_tmp, c->perm, c->pk, c->scan_tmp, c->scan_bytes, c->stream);
-    if (c->prof) (void)hipEventRecord(ev[2], c->stream);
-    if (e == hipSuccess)
-      e = launch_step(c->P, A, B, c->S, c->n_upper, c->perm, c->pk, c->start, c->clist, c->ccnt,
-                      c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->stream,
-                      c->prof ? ev[3] : nullptr);
-    if (c->prof) (void)hipEventRecord(ev[4], c->stream);
-    c->cur = 1 - c->cur;
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) {
-    for (auto& x : evs) (void)hipEventDestroy(x);
-    return hip_fail(c, e, "sdem_step");
-  }
-  if (nsteps > 0) c->stepped = true;
-  for (int it = 0; it < (int)evs.size() / NE; it++) {
-    for (int q = 0; q < SDEM_NUM_PHASES; q++) {
-      float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, evs[(size_t)it * NE + q], evs[(size_t)it * NE + q + 1]) == hipSuccess) {
-        c->prof_ms[q] += ms;
-        c->prof_n[q]++;
-      }
+  int remaining = nsteps, retries = 0;
+  while (remaining > 0) {
+    std::vector<hipEvent_t> evs;
+    if (c->prof) evs.resize((size_t)remaining * NE);
+    for (auto& x : evs)
+      if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
+    const int c0 = c->cur;
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < remaining && e == hipSuccess; it++) {
+      Buf& A = c->buf[c->cur];
+      Buf& B = c->buf[1 - c->cur];
+      hipEvent_t* ev = c->prof ? &evs[(size_t)it * NE] : nullptr;
+      if (ev) (void)hipEventRecord(ev[0], c->stream);
+      e = launch_begin(c->P, c->S, c->stream);
+      if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->n_upper, c->stream);
+      if (ev) (void)hipEventRecord(ev[1], c->stream);
+      if (e == hipSuccess)
+        e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
+                        c->tile_sum, c->perm_tmp, c->perm, c->pk, c->stream);
+      if (ev) (void)hipEventRecord(ev[2], c->stream);
+      if (e == hipSuccess)
+        e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->voff, c->vlist,
+                         c->tile_sum, c->stream);
+      if (ev) (void)hipEventRecord(ev[3], c->stream);
+      if (e == hipSuccess)
+        e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->ref, c->d_fpk,
+                         c->d_fid, c->d_ftype, c->d_fstart, c->stream);
+      if (ev) (void)hipEventRecord(ev[4], c->stream);
+      c->cur = 1 - c->cur;
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      for (auto& x : evs) (void)hipEventDestroy(x);
+      return hip_fail(c, e, "sdem_step");
+    }
+    Scalars s;
+    e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      for (auto& x : evs) (void)hipEventDestroy(x);
+      return hip_fail(c, e, "hipMemcpy(scalars)");
+    }
+    const int ran = remaining - s.halted_steps;  // steps that did work, the last one failed if overflow
+    const int ok_steps = s.overflow ? ran - 1 : ran;
+    for (int it = 0; it < ok_steps && c->prof; it++)
+      for (int q = 0; q < SDEM_NUM_PHASES; q++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, evs[(size_t)it * NE + q], evs[(size_t)it * NE + q + 1]) == hipSuccess) {
+          c->prof_ms[q] += ms;
+          c->prof_n[q]++;
+        }
+      }
+    for (auto& x : evs) (void)hipEventDestroy(x);
+    if (ok_steps > 0) {
+      c->stepped = true;
+      c->first_step_pending = false;
+    }
+    if (!s.overflow) break;
+    // a capacity ran out in step `ok_steps` of this batch: its input buffer
+    // is intact; grow the capacity and replay from there
+    if (++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");
+    c->cur = (c0 + ok_steps) % 2;
+    const int kind = s.overflow;
+    s.iter = s.iter_at_begin;
+    s.next_wall_update = s.next_wall_at_begin;
+    s.n_alive = s.n_stored;
+    s.need_rebuild = 1;
+    s.max_disp_bits = 0ull;
+    s.n_deleted = 0;
+    s.overflow = 0;
+    s.halt = 0;
+    s.halted_steps = 0;
+    if (c->first_step_pending) s.use_host_max_a = 1;
+    e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
+    int rc = kind == 2 ? grow_vlist(c) : grow_history(c);
+    if (rc) return rc;
+    remaining -= ok_steps;
   }
-  for (auto& x : evs) (void)hipEventDestroy(x);
-  Scalars s;
-  e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
-  if (s.overflow) return c->fail(SDEM_E_CAPACITY, "contact history slots exhausted for a particle");
   return SDEM_OK;
 }
 
@@ -882,13 +1000,6 @@ int sdem_get_num_particles(sdem_ctx* c, int32_t* n) {
   return SDEM_OK;
 }
 
-template <class T>
-static hipError_t dl(std::vector<T>& v, const T* src, size_t n) {
-  v.resize(n);
-  if (!n) return hipSuccess;
-  return hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
-}
-
 int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_t* ptype,
                    int32_t* usr_mark, double* posr, double* lin_vel, double* lin_acc, double* theta,
                    double* rot_vel, double* rot_acc) {
@@ -902,12 +1013,10 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
   Buf& A = c->buf[c->cur];
   const size_t cap = c->cap;
   const int ts = c->tsize, rs = c->rsize;
-  std::vector<double> x, y, z, r, lv, la, th, rv, ra;
+  std::vector<double> lv, la, th, rv, ra;
+  std::vector<double4> p4;
   std::vector<int> ty, mk, ids;
-  hipError_t e = dl(x, A.x, cap);
-  if (e == hipSuccess) e = dl(y, A.y, cap);
-  if (e == hipSuccess) e = dl(z, A.z, cap);
-  if (e == hipSuccess) e = dl(r, A.r, cap);
+  hipError_t e = dl(p4, A.pos, cap);
   if (e == hipSuccess) e = dl(lv, A.lv, cap * 3 * ts);
   if (e == hipSuccess) e = dl(la, A.la, cap * 3 * c->ta);
   if (e == hipSuccess) e = dl(th, A.th, cap * 3);
@@ -922,7 +1031,7 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
     if (id) id[q] = idx[q].first;
     if (ptype) ptype[q] = ty[k];
     if (usr_mark) usr_mark[q] = mk[k];
-    if (posr) { posr[4 * q] = x[k]; posr[4 * q + 1] = y[k]; posr[4 * q + 2] = z[k]; posr[4 * q + 3] = r[k]; }
+    if (posr) { posr[4 * q] = p4[k].x; posr[4 * q + 1] = p4[k].y; posr[4 * q + 2] = p4[k].z; posr[4 * q + 3] = p4[k].w; }
     for (int l = 0; l < ts; l++)
       for (int d = 0; d < 3; d++) {
         if (lin_vel) lin_vel[((size_t)q * ts + l) * 3 + d] = lv[((size_t)l * 3 + d) * cap + k];
@@ -1077,8 +1186,13 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES], int64_t launches[S
   return SDEM_OK;
 }
 
-int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], int32_t keep_forces) {
+int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor, int32_t keep_forces) {
   if (!c) return SDEM_E_ARG;
+  if (skin_factor >= 0.0) {
+    if (skin_factor > 0.45) return c->fail(SDEM_E_ARG, "skin factor must be below 0.45");
+    c->skin_factor = skin_factor;
+    c->prepared = false;
+  }
   if (cell_factor) {
     for (int d = 0; d < 3; d++)
       if (cell_factor[d] < 0.0 || cell_factor[d] > 64.0) return c->fail(SDEM_E_ARG, "cell factor out of range");
@@ -1089,6 +1203,16 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], int32_t keep_force
     c->keep_forces = keep_forces ? 1 : 0;
     c->P.keep_forces = c->keep_forces;
   }
+  return SDEM_OK;
+}
+
+int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int64_t list_entries) {
+  if (!c) return SDEM_E_ARG;
+  if (!c->allocs_state.empty()) return c->fail(SDEM_E_STATE, "capacities must be set before the state");
+  if (hist_slots < 0 || hist_slots > 4096 || list_entries < 0 || list_entries > INT32_MAX)
+    return c->fail(SDEM_E_ARG, "capacity out of range");
+  c->hist_slots_hint = hist_slots;
+  c->list_entries_hint = list_entries;
   return SDEM_OK;
 }
 
@@ -1103,6 +1227,7 @@ const char* sdem_last_error(const sdem_ctx* c) { return c ? c->err.c_str() : "nu
 void sdem_destroy(sdem_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->vlist) (void)hipFree(c->vlist);
   free_list(c, c->allocs_state);
   free_list(c, c->allocs_grid);
   free_list(c, c->allocs_tab);

@@ -51,7 +51,7 @@ constexpr int MAX_LEVELS = 6;
 
 // one SoA particle buffer
 struct Buf {
-  double *x, *y, *z, *r;
+  double4* pos;   // x, y, z, radius (GPrtcl_PosR)
   double *lv;     // (tsize*3) x cap  : lv[(l*3+c)*cap + i]
   double *la;     // (ta*3) x cap     : stored acceleration levels
   double *th;     // 3 x cap
@@ -84,6 +84,9 @@ struct Params {
   Grid fgrid;                 // grid of the fixed particles
   int nfix;
   double dxmax1, dxmax2, max_wall_vel;
+  double skin;                // candidate-list skin (search tuning)
+  double rebuild_disp;        // rebuild when a particle moved further than this
+  int vcap;                   // capacity of the candidate-list entries
   int wall_max_update_iter;
   const Bnry* pp;             // ntype*ntype, row = type of the record's lower-id particle
   const Bnry* pw;             // ntype*nwtype
@@ -99,9 +102,16 @@ struct Scalars {
   int n_deleted;           // particles removed at the end of the last step
   int iter;                // iterNumber
   int next_wall_update;    // Next_iter_update
-  int overflow;            // history slots exhausted (first failing step+1)
+  int overflow;            // 1: history slots, 2: candidate-list entries
+  int halt;                // set by the step after an overflow: no more work
+  int halted_steps;        // steps of the call skipped after an overflow
+  int iter_at_begin;       // iterNumber / Next_iter_update at the start of
+  int next_wall_at_begin;  // the last step that ran
+  int rebuild;             // this step rebuilds the cell sort and the lists
+  int need_rebuild;        // forced rebuild at the next step
+  int rebuilds;            // rebuilds so far (statistics)
   int num_cntcts[4];
-  unsigned long long max_v_bits, max_a_bits;
+  unsigned long long max_v_bits, max_a_bits, max_disp_bits;
   int use_host_max_a;      // first step after set_state: max|a| from host copy
   double host_max_a;
 };

@@ -1,19 +1,27 @@
 // sdem_kernels.hip -- gfx950 kernels of one DEM time step.
 //
-// A step is a short pipeline on one stream:
-//   k_begin        1 thread : counters for this step
-//   k_wall_list    N        : near-wall bit masks + max|v|, max|a| (only on
-//                             update iterations, UpdateNearPrtclsPW)
-//   k_wall_cadence 1 thread : Next_iter_update
-//   k_bin          N        : (level, cell) key per particle, per-cell counts
-//   scan                    : exclusive scan of the counts (rocPRIM)
-//   k_scatter / k_rank      : deterministic counting sort (in-cell order =
-//                             storage order) + packed (x,y,z,r) in cell order
-//   k_search       N        : contact search into a per-particle partner list
-//   k_force        N        : history match, contact forces, accelerations,
-//                             integration, wrap/delete
+// A step is a short pipeline on one stream.  Candidate (Verlet) lists are
+// rebuilt, together with the cell sort, only when a particle has moved more
+// than half the skin since the last rebuild; the contact set itself is
+// decided every step with the reference's exact predicate, so it does not
+// depend on when lists are rebuilt.
 //
-// All floating-point kernels are built with -ffp-contract=off so that every
+//   k_begin        1 thread : step counters; decides whether to rebuild
+//   k_wall_list    N        : near-wall bit masks + max|v|, max|a| (update
+//                             iterations only, UpdateNearPrtclsPW)
+//   k_wall_cadence 1 thread : Next_iter_update; iterNumber + 1
+//   -- rebuild steps only --
+//   k_fill/k_bin            : (level, cell) key per particle, per-key counts
+//   k_scan*                 : exclusive scan of the counts
+//   k_scatter / k_rank      : deterministic counting sort (in-cell order =
+//                             storage order) + packed positions in cell order
+//   k_vlist (count, fill)   : CSR candidate lists, pairs within ri+rj+skin
+//   -- every step --
+//   k_force        N        : contact test, history match, contact forces,
+//                             accelerations, integration, wrap/delete, and
+//                             the displacement since the last rebuild
+//
+// Floating-point kernels are built with -ffp-contract=off so that every
 // product and sum is rounded on its own, in the order the reference's
 // operator functions impose (Prtcl_TypeDef.f90:105-288).
 #include <hip/hip_runtime.h>
@@ -219,39 +227,47 @@ __device__ __forceinline__ int64_t cell_key(const Grid& G, double x, double y, d
 }
 
 // ---------------------------------------------------------------------------
-// small kernels
+// step control
 // ---------------------------------------------------------------------------
-__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
-}
-
-__global__ void k_begin(Scalars* S) {
+// After a capacity overflow every later kernel of the call does nothing, so
+// the input buffer of the failing step stays intact and can be replayed.
+__global__ void k_begin(Params P, Scalars* S) {
+  if (S->overflow) {
+    S->halt = 1;
+    S->halted_steps++;
+    return;
+  }
+  const double disp = __longlong_as_double((long long)S->max_disp_bits);
+  S->rebuild = (S->need_rebuild || S->n_deleted > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
+  S->need_rebuild = 0;
+  S->max_disp_bits = 0ull;
   S->n_stored = S->n_alive;
   for (int k = 0; k < 4; k++) S->num_cntcts[k] = 0;
   S->n_deleted = 0;
+  S->iter_at_begin = S->iter;
+  S->next_wall_at_begin = S->next_wall_update;
+  if (S->rebuild) S->rebuilds++;
 }
 
 // UpdateNearPrtclsPW (Prtcl_ContactSearchPW.f90:159-235), particle part
 __global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) {
-  if (S->iter < S->next_wall_update) return;
+  if (S->halt || S->iter < S->next_wall_update) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mv = 0.0, ma = 0.0;
   if (s < S->n_stored) {
-    double x = A.x[s], y = A.y[s], z = A.z[s];
+    const double4 p = A.pos[s];
     uint64_t mask = 0;
     for (int w = 0; w < P.nwall; w++) {
       const Wall& W = P.walls[w];
       double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2, mnz = W.min_point[2] - P.dxmax2;
       double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2, mxz = W.max_point[2] + P.dxmax2;
-      if (x >= mnx && x <= mxx && y >= mny && y <= mxy && z >= mnz && z <= mxz) mask |= (1ull << w);
+      if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
+        mask |= (1ull << w);
     }
     A.wmask[s] = mask;
     mv = norm3(ld3(A.lv, P.cap, s));
     ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
   }
-  // wave then block max (non-negative doubles order like their bit patterns)
   for (int off = 32; off > 0; off >>= 1) {
     mv = fmax(mv, __shfl_down(mv, off, 64));
     ma = fmax(ma, __shfl_down(ma, off, 64));
@@ -267,9 +283,10 @@ __global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) 
   }
 }
 
-// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances (only
-// the two wall kernels read it)
+// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances
+// (only the two wall kernels read it)
 __global__ void k_wall_cadence(Params P, Scalars* S) {
+  if (S->halt) return;
   if (S->iter >= S->next_wall_update) {
     double max_v = __longlong_as_double((long long)S->max_v_bits) + P.max_wall_vel;
     max_v = max_v > 0.3 ? max_v : 0.3;
@@ -296,12 +313,23 @@ __global__ void k_wall_cadence(Params P, Scalars* S) {
   S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
 }
 
+// ---------------------------------------------------------------------------
+// cell sort (rebuild steps only)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_rebuild(int* __restrict__ p, int64_t n, int v, const Scalars* S) {
+  if (!S->rebuild || S->halt) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
 // (level, cell) key and per-key counts; dead particles go to the last key.
 // Lanes of a wave holding the same key (neighbours in storage order usually
 // share a cell) take their slots with one atomic per run.
 __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_t dead_key,
                                              int* __restrict__ keys, int* __restrict__ rank,
                                              int* __restrict__ counts) {
+  if (!S->rebuild || S->halt) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = s < S->n_stored;
@@ -312,7 +340,8 @@ __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_
       key = (int)dead_key;
     } else {
       const Grid& G = P.lvl[P.type_level[t - 1]];
-      key = (int)cell_key(G, A.x[s], A.y[s], A.z[s]);
+      const double4 p = A.pos[s];
+      key = (int)cell_key(G, p.x, p.y, p.z);
     }
     keys[s] = key;
   }
@@ -329,10 +358,63 @@ __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_
   if (valid) rank[s] = base + (lane - my_head);
 }
 
+// exclusive scan in three passes (block scans, scan of block sums, add),
+// skipped on steps that do not rebuild
+constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out,
+                                                        int* __restrict__ tile_sum, int64_t n,
+                                                        const Scalars* S) {
+  if (!S->rebuild || S->halt) return;
+  using BS = hipcub::BlockScan<int, SCAN_T>;
+  __shared__ typename BS::TempStorage tmp;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0;
+  int total;
+  BS(tmp).ExclusiveSum(v, v, total);
+  for (int i = 0; i < SCAN_ITEMS; i++)
+    if (base + i < n) out[base + i] = v[i];
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_sums(int* __restrict__ tile_sum, int ntiles,
+                                                       const Scalars* S) {
+  if (!S->rebuild || S->halt) return;
+  using BS = hipcub::BlockScan<int, SCAN_T>;
+  __shared__ typename BS::TempStorage tmp;
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int start = 0; start < ntiles; start += SCAN_TILE) {
+    const int base = start + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < ntiles) ? tile_sum[base + i] : 0;
+    int total;
+    BS(tmp).ExclusiveSum(v, v, total);
+    const int c0 = carry;
+    for (int i = 0; i < SCAN_ITEMS; i++)
+      if (base + i < ntiles) tile_sum[base + i] = v[i] + c0;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c0 + total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ out, const int* __restrict__ tile_sum,
+                                                      int64_t n, const Scalars* S) {
+  if (!S->rebuild || S->halt) return;
+  const int add = tile_sum[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_T)
+    if (base + i < n) out[base + i] += add;
+}
+
 __global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __restrict__ keys,
                                                  const int* __restrict__ rank,
                                                  const int* __restrict__ start,
                                                  int* __restrict__ perm_tmp) {
+  if (!S->rebuild || S->halt) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S->n_stored) return;
   perm_tmp[start[keys[s]] + rank[s]] = s;
@@ -345,6 +427,7 @@ __global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_ke
                                               const int* __restrict__ start,
                                               const int* __restrict__ perm_tmp,
                                               int* __restrict__ perm, double4* __restrict__ pk) {
+  if (!S->rebuild || S->halt) return;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k == 0) S->n_alive = start[dead_key];
   if (k >= S->n_stored) return;
@@ -354,11 +437,98 @@ __global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_ke
   int r = 0;
   for (int j = a; j < b; j++) r += (perm_tmp[j] < s);
   perm[a + r] = s;
-  pk[a + r] = make_double4(A.x[s], A.y[s], A.z[s], A.r[s]);
+  pk[a + r] = A.pos[s];
 }
 
 // ---------------------------------------------------------------------------
-// the fused step kernel
+// candidate (Verlet) lists, rebuild steps only
+// ---------------------------------------------------------------------------
+// Every particle lists, in traversal order, the particles j (any level,
+// periodic images included) with ((dx*dx + dy*dy) + dz*dz) <= (ri+rj+skin)^2
+// over the cells of each level overlapping [x - reach, x + reach], reach =
+// ri + (largest radius of the level) + skin.  Pass 0 counts, pass 1 fills
+// the CSR rows whose offsets the scan of the counts produced.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_vlist(Params P, Scalars* S, const double4* __restrict__ pk,
+                                               const int* __restrict__ cstart, int* __restrict__ vcnt,
+                                               const int* __restrict__ voff, int* __restrict__ vlist,
+                                               int n_upper) {
+  if (!S->rebuild || S->halt) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > n_upper) return;
+  const int n = S->n_alive;
+  if (k >= n) {
+    if (PASS == 0) vcnt[k] = 0;
+    return;
+  }
+  const double4 pi = pk[k];
+  const double xv[3] = {pi.x, pi.y, pi.z};
+  const double ri = pi.w;
+  int c = 0;
+  int w0 = 0;
+  if (PASS == 1) {
+    w0 = voff[k];
+    if (voff[k + 1] > P.vcap) {
+      atomicMax(&S->overflow, 2);
+      return;
+    }
+  }
+  for (int L = 0; L < P.nlvl; L++) {
+    const Grid& G = P.lvl[L];
+    const double reach = ri + G.rmax + P.skin;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      const double pad = 1.0e-6 * G.w[d];
+      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+      if (!P.periodic[d]) {
+        lo[d] = lo[d] < 0 ? 0 : lo[d];
+        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      }
+    }
+    for (int cz = lo[2]; cz <= hi[2]; cz++) {
+      int wz = cz;
+      double shz = 0.0;
+      if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
+      else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
+      for (int cy = lo[1]; cy <= hi[1]; cy++) {
+        int wy = cy;
+        double shy = 0.0;
+        if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
+        else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
+        const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
+        for (int seg = 0; seg < 3; seg++) {
+          int xa, xb;
+          double shx;
+          if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; shx = -P.simlen[0]; }
+          else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; shx = 0.0; }
+          else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; shx = P.simlen[0]; }
+          if (xa > xb) continue;
+          if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
+          if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
+          const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
+          const int ja = cstart[rowkey + xa], jb = cstart[rowkey + xb + 1];
+          for (int kj = ja; kj < jb; kj++) {
+            const double4 pj = pk[kj];
+            const double ddx = pi.x - (pj.x + shx);
+            const double ddy = pi.y - (pj.y + shy);
+            const double ddz = pi.z - (pj.z + shz);
+            const double dr = ri + pj.w + P.skin;
+            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (d2 > dr * dr) continue;
+            if (kj == k && !shifted) continue;
+            if (PASS == 1) vlist[w0 + c] = kj;
+            c++;
+          }
+        }
+      }
+    }
+  }
+  if (PASS == 0) vcnt[k] = c;
+}
+
+// ---------------------------------------------------------------------------
+// the per-step force kernel
 // ---------------------------------------------------------------------------
 struct Hist {
   d3 td;
@@ -402,116 +572,43 @@ __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, 
   acc.nkind[kind - 1]++;
 }
 
-// Contact search: every particle gathers all its contacts (both roles) over
-// the cells of every level that overlap [x - reach, x + reach], reach =
-// r_i + largest radius of the level.  The narrow-phase test is the
-// reference's (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557):
-// contact iff ((dx*dx + dy*dy) + dz*dz) <= (ri+rj)^2.  Found partners are
-// stored as (sorted index | periodic shift code) in traversal order.
-constexpr int SHIFT_BITS = 26;
-constexpr unsigned IDX_MASK = (1u << SHIFT_BITS) - 1u;
-
-__global__ __launch_bounds__(256) void k_search(Params P, Scalars* S, const double4* __restrict__ pk,
-                                                const int* __restrict__ cstart,
-                                                unsigned* __restrict__ clist, int* __restrict__ ccnt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S->n_alive) return;
-  const double4 pi = pk[k];
-  const double xv[3] = {pi.x, pi.y, pi.z};
-  const double ri = pi.w;
-  const int cap = P.hcap;
-  unsigned* out = clist + (size_t)k * cap;
-  int c = 0;
-  for (int L = 0; L < P.nlvl; L++) {
-    const Grid& G = P.lvl[L];
-    const double reach = ri + G.rmax;
-    int lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-      const double pad = 1.0e-6 * G.w[d];
-      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-      if (!P.periodic[d]) {
-        lo[d] = lo[d] < 0 ? 0 : lo[d];
-        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-      }
-    }
-    for (int cz = lo[2]; cz <= hi[2]; cz++) {
-      int wz = cz;
-      unsigned sz = 0;
-      double shz = 0.0;
-      if (wz < 0) { wz += G.n[2]; sz = 1; shz = -P.simlen[2]; }
-      else if (wz >= G.n[2]) { wz -= G.n[2]; sz = 2; shz = P.simlen[2]; }
-      for (int cy = lo[1]; cy <= hi[1]; cy++) {
-        int wy = cy;
-        unsigned sy = 0;
-        double shy = 0.0;
-        if (wy < 0) { wy += G.n[1]; sy = 1; shy = -P.simlen[1]; }
-        else if (wy >= G.n[1]) { wy -= G.n[1]; sy = 2; shy = P.simlen[1]; }
-        const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
-        for (int seg = 0; seg < 3; seg++) {
-          int xa, xb;
-          unsigned sx;
-          double shx;
-          if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; sx = 1; shx = -P.simlen[0]; }
-          else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; sx = 0; shx = 0.0; }
-          else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; sx = 2; shx = P.simlen[0]; }
-          if (xa > xb) continue;
-          if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
-          if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
-          const unsigned code = (sx | (sy << 2) | (sz << 4)) << SHIFT_BITS;
-          const bool shifted = code != 0u;
-          const int ja = cstart[rowkey + xa], jb = cstart[rowkey + xb + 1];
-          for (int kj = ja; kj < jb; kj++) {
-            const double4 pj = pk[kj];
-            // image position = stored position + shift, as the reference's
-            // ghost copies hold fl(x +- L) (Prtcl_Comm.f90:242,252)
-            const double xj = sx ? pj.x + shx : pj.x;
-            const double yj = sy ? pj.y + shy : pj.y;
-            const double zj = sz ? pj.z + shz : pj.z;
-            const double dr = ri + pj.w;
-            const double dr2 = dr * dr;
-            const double ddx = pi.x - xj;
-            double d2 = ddx * ddx;
-            if (d2 > dr2) continue;
-            const double ddy = pi.y - yj;
-            d2 = ddy * ddy + d2;
-            if (d2 > dr2) continue;
-            const double ddz = pi.z - zj;
-            d2 = ddz * ddz + d2;
-            if (d2 > dr2) continue;
-            if (kj == k && !shifted) continue;
-            if (c < cap) out[c] = (unsigned)kj | code;
-            c++;
-          }
-        }
-      }
-    }
+// image of a partner coordinate: the reference's ghost copies hold
+// fl(x + L) or fl(x - L) (Prtcl_Comm.f90:242,252); the image within half a
+// period of the particle is the only one that can touch it
+__device__ __forceinline__ double image(double xi, double xj, double L, int periodic, int* shifted) {
+  if (periodic) {
+    const double dx = xi - xj;
+    if (dx > 0.5 * L) { *shifted = 1; return xj + L; }
+    if (dx < -0.5 * L) { *shifted = 1; return xj + (-L); }
   }
-  ccnt[k] = c;
-  if (c > cap) atomicMax(&S->overflow, 1);
+  return xj;
 }
 
 __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* S,
                                                const int* __restrict__ perm,
-                                               const double4* __restrict__ pk,
-                                               const unsigned* __restrict__ clist,
-                                               const int* __restrict__ ccnt,
+                                               const int* __restrict__ voff,
+                                               const int* __restrict__ vlist,
+                                               double4* __restrict__ ref,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
                                                const int* __restrict__ fstart) {
+  if (S->halt) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = S->n_alive;
+  const bool rb = S->rebuild != 0;
   Acc acc;
   acc.F = D3(0.0, 0.0, 0.0);
   acc.T = D3(0.0, 0.0, 0.0);
   acc.hc = 0;
   acc.nkind[0] = acc.nkind[1] = acc.nkind[2] = acc.nkind[3] = 0;
   acc.overflow = 0;
+  double disp2 = 0.0;
+  bool dead = false;
   if (k < n) {
     const int cap = P.cap;
-    const int s = perm[k];
-    const double4 pi = pk[k];
+    const int s = rb ? perm[k] : k;
+    const double4 pi = A.pos[s];
     const d3 xi = D3(pi.x, pi.y, pi.z);
     const double ri = pi.w;
     const int idi = A.id[s];
@@ -519,30 +616,37 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
     const d3 vi = ld3(A.lv, cap, s);
     const d3 wi = ld3(A.rv, cap, s);
     const double xv[3] = {pi.x, pi.y, pi.z};
+    double4 r0;
+    if (rb) {
+      ref[k] = pi;
+      r0 = pi;
+    } else {
+      r0 = ref[k];
+    }
 
-    // ---------------- moving particles found by k_search -----------------
-    const int nc = min(ccnt[k], P.hcap);
-    const unsigned* lst = clist + (size_t)k * P.hcap;
-    for (int q = 0; q < nc; q++) {
-      const unsigned e = lst[q];
-      const int kj = (int)(e & IDX_MASK);
-      const unsigned code = e >> SHIFT_BITS;
-      const bool shifted = code != 0u;
-      const double4 pj = pk[kj];
-      double xj = pj.x, yj = pj.y, zj = pj.z;
-      if (shifted) {
-        const unsigned sx = code & 3u, sy = (code >> 2) & 3u, sz = (code >> 4) & 3u;
-        if (sx) xj = pj.x + (sx == 1u ? -P.simlen[0] : P.simlen[0]);
-        if (sy) yj = pj.y + (sy == 1u ? -P.simlen[1] : P.simlen[1]);
-        if (sz) zj = pj.z + (sz == 1u ? -P.simlen[2] : P.simlen[2]);
-      }
+    // ---------------- moving particles of the candidate list --------------
+    const int qa = voff[k], qb = voff[k + 1];
+    for (int q = qa; q < qb; q++) {
+      const int kj = vlist[q];
+      const int sj = rb ? perm[kj] : kj;
+      const double4 pj = A.pos[sj];
+      int shifted = 0;
+      const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
+      const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
+      const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
+      // narrow phase, NBSM_FineSearch2/3 (Prtcl_NBS_Munjiza.f90:503-557)
       const double dr = ri + pj.w;
-      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
+      const double dr2 = dr * dr;
+      const double ddx = xi.x - xj;
       double d2 = ddx * ddx;
+      if (d2 > dr2) continue;
+      const double ddy = xi.y - yj;
       d2 = ddy * ddy + d2;
+      if (d2 > dr2) continue;
+      const double ddz = xi.z - zj;
       d2 = ddz * ddz + d2;
+      if (d2 > dr2) continue;
       const double ovrlp = dr - sqrt(d2);
-      const int sj = perm[kj];
       const int idj = A.id[sj];
       const int tj = A.type[sj];
       const d3 vj = ld3(A.lv, cap, sj);
@@ -584,7 +688,6 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
         push_hist(P, B, k, acc, 2, idj, cf);
       }
     }
-
     // ---------------- fixed particles (NBS only) --------------------------
     if (P.search_fixed && P.nfix > 0) {
       const Grid& G = P.fgrid;
@@ -740,7 +843,6 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
 
     // ---------------- periodic wrap / deletion (Comm_For_Exchange) --------
     double pv[3] = {pos.x, pos.y, pos.z};
-    bool dead = false;
     for (int d = 0; d < 3; d++) {
       if (P.periodic[d]) {
         if (pv[d] >= P.dmax[d]) pv[d] = pv[d] - P.simlen[d];
@@ -749,10 +851,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
         dead = true;
       }
     }
-    B.x[k] = pv[0];
-    B.y[k] = pv[1];
-    B.z[k] = pv[2];
-    B.r[k] = ri;
+    B.pos[k] = make_double4(pv[0], pv[1], pv[2], ri);
     B.id[k] = idi;
     B.type[k] = dead ? -ti : ti;
     B.mark[k] = A.mark[s];
@@ -762,13 +861,31 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       st3(B.force, cap, k, acc.F);
       st3(B.force + 3 * cap, cap, k, acc.T);
     }
-    if (dead) atomicAdd(&S->n_deleted, 1);
+    // displacement since the last list rebuild (nearest periodic image)
+    const double r0v[3] = {r0.x, r0.y, r0.z};
+    for (int d = 0; d < 3; d++) {
+      double dd = pv[d] - r0v[d];
+      if (P.periodic[d]) {
+        if (dd > 0.5 * P.simlen[d]) dd -= P.simlen[d];
+        else if (dd < -0.5 * P.simlen[d]) dd += P.simlen[d];
+      }
+      disp2 += dd * dd;
+    }
   }
-  // counters: wave reduction then one atomic per wave
+  // per-wave reductions, one atomic per wave
   for (int q = 0; q < 4; q++) {
     int v = acc.nkind[q];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(&S->num_cntcts[q], v);
+  }
+  int nd = dead ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    disp2 = fmax(disp2, __shfl_down(disp2, off, 64));
+    nd += __shfl_down(nd, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (disp2 > 0.0) atomicMax(&S->max_disp_bits, (unsigned long long)__double_as_longlong(sqrt(disp2)));
+    if (nd) atomicAdd(&S->n_deleted, nd);
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
 }
@@ -778,16 +895,17 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
-  return hipGetLastError();
+static void scan(const int* in, int* out, int* tile_sum, int64_t n, const Scalars* S, hipStream_t st) {
+  const unsigned nt = nblk(n, SCAN_TILE);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, in, out, tile_sum, n, S);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, st, tile_sum, (int)nt, S);
+  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, out, tile_sum, n, S);
 }
 
-hipError_t launch_begin(Scalars* S, hipStream_t st) {
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, S);
+int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+hipError_t launch_begin(const Params& P, Scalars* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, P, S);
   return hipGetLastError();
 }
 
@@ -798,17 +916,17 @@ hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, int n_upp
   return hipGetLastError();
 }
 
-hipError_t launch_bin(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
-                      int* keys, int* rank, int* counts, int* start, int* perm_tmp, int* perm,
-                      double4* pk, void* scan_tmp, size_t scan_bytes, hipStream_t st) {
-  hipError_t e = launch_fill_i32(counts, dead_key + 2, 0, st);
-  if (e != hipSuccess) return e;
+hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
+                       int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
+                       int* perm, double4* pk, hipStream_t st) {
+  int64_t nfill = dead_key + 2;
+  unsigned fb = nblk(nfill, 256);
+  if (fb > 65536) fb = 65536;
+  hipLaunchKernelGGL(k_fill_rebuild, dim3(fb), dim3(256), 0, st, counts, nfill, 0, S);
   if (n_upper > 0)
     hipLaunchKernelGGL(k_bin, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, dead_key, keys,
                        rank, counts);
-  size_t bytes = scan_bytes;
-  e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, counts, start, (int)(dead_key + 2), st);
-  if (e != hipSuccess) return e;
+  scan(counts, start, tile_sum, nfill, S, st);
   if (n_upper > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n_upper, 256)), dim3(256), 0, st, S, keys, rank, start,
                        perm_tmp);
@@ -818,24 +936,24 @@ hipError_t launch_bin(const Params& P, const Buf& A, Scalars* S, int n_upper, in
   return hipGetLastError();
 }
 
-size_t scan_temp_bytes(int64_t nitems) {
-  size_t bytes = 0;
-  if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int*)nullptr, (int*)nullptr, (int)nitems) != hipSuccess)
-    return 0;
-  return bytes;
+hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
+                        const int* cstart, int* vcnt, int* voff, int* vlist, int* tile_sum,
+                        hipStream_t st) {
+  if (n_upper <= 0) return hipSuccess;
+  const unsigned nb = nblk((int64_t)n_upper + 1, 256);
+  hipLaunchKernelGGL(k_vlist<0>, dim3(nb), dim3(256), 0, st, P, S, pk, cstart, vcnt, voff, vlist, n_upper);
+  scan(vcnt, voff, tile_sum, (int64_t)n_upper + 1, S, st);
+  hipLaunchKernelGGL(k_vlist<1>, dim3(nb), dim3(256), 0, st, P, S, pk, cstart, vcnt, voff, vlist, n_upper);
+  return hipGetLastError();
 }
 
-hipError_t launch_step(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                       const int* perm, const double4* pk, const int* cstart, unsigned* clist,
-                       int* ccnt, const double4* fpk, const int* fid, const int* ftype,
-                       const int* fstart, hipStream_t st, hipEvent_t mid) {
-  if (n_upper > 0) {
-    hipLaunchKernelGGL(k_search, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, S, pk, cstart, clist,
-                       ccnt);
-    if (mid) (void)hipEventRecord(mid, st);
-    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, pk,
-                       clist, ccnt, fpk, fid, ftype, fstart);
-  }
+hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
+                        const int* perm, const int* voff, const int* vlist, double4* ref,
+                        const double4* fpk, const int* fid, const int* ftype, const int* fstart,
+                        hipStream_t st) {
+  if (n_upper > 0)
+    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, voff,
+                       vlist, ref, fpk, fid, ftype, fstart);
   return hipGetLastError();
 }
 

@@ -110,7 +110,8 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "set_profiling": ([V, C.c_int32], C.c_int),
         "get_profile": ([V, _DP, _LP], C.c_int),
         "get_device_bytes": ([V, _LP], C.c_int),
-        "set_tuning": ([V, _DP, C.c_int32], C.c_int),
+        "set_tuning": ([V, _DP, C.c_double, C.c_int32], C.c_int),
+        "set_capacity": ([V, C.c_int32, C.c_int64], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = f(name)
@@ -282,9 +283,13 @@ class Engine:
     def set_profiling(self, enable: bool) -> None:
         self._check(self._fn("set_profiling")(self.h, int(bool(enable))), "set_profiling")
 
-    def set_tuning(self, cell_factor=None, keep_forces=-1) -> None:
+    def set_tuning(self, cell_factor=None, skin=-1.0, keep_forces=-1) -> None:
         cf = None if cell_factor is None else _f64(cell_factor)
-        self._check(self._fn("set_tuning")(self.h, _dptr(cf), int(keep_forces)), "set_tuning")
+        self._check(self._fn("set_tuning")(self.h, _dptr(cf), float(skin), int(keep_forces)),
+                    "set_tuning")
+
+    def set_capacity(self, hist_slots=0, list_entries=0) -> None:
+        self._check(self._fn("set_capacity")(self.h, int(hist_slots), int(list_entries)), "set_capacity")
 
     def get_profile(self):
         ms = np.zeros(len(PHASE_NAMES))
@@ -322,13 +327,16 @@ def make_options(lib: C.CDLL, prefix: str = "sdem_", **kw) -> Options:
     return o
 
 
-def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None) -> Engine:
+def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None,
+                 capacity: Optional[dict] = None) -> Engine:
     """Create an engine and push a scenario (see sdem_amd.scenarios) into it,
     in the reference's initialisation order (DEMS_Initialize :58-195)."""
     o = make_options(lib, prefix, **scenario["options"])
     eng = Engine(lib, o, prefix)
     if tuning and hasattr(lib, prefix + "set_tuning"):
         eng.set_tuning(**tuning)
+    if capacity and hasattr(lib, prefix + "set_capacity"):
+        eng.set_capacity(**capacity)
     pt = scenario["ptypes"]
     eng.set_particle_property(pt["diameter"], pt["density"], pt["youngs"], pt["poisson"], *scenario["pp"])
     wt = scenario["wtypes"]

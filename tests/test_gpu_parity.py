@@ -51,3 +51,44 @@ def test_many_steps_parity(name, hip_lib, oracle_lib):
         ref.iterate(10)
         errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
         assert not errs, (k, errs)
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.02, 0.3])
+def test_candidate_list_skin_does_not_change_results(skin, hip_lib, oracle_lib):
+    """Lists rebuilt every step (skin 0), rarely (0.02) or seldom (0.3 of
+    d_max): the contact set and the state stay the oracle's over 30 steps."""
+    print(f"skin {skin}", flush=True)
+    sc = scenarios.random_cloud(n=300, seed=21, periodic=(1, 1, 0), walls="z", infinite_walls=True)
+    gpu = native.build_engine(hip_lib, sc, tuning=dict(skin=skin, cell_factor=(0.5, 0.75, 1.0)))
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    for k in range(3):
+        gpu.iterate(10)
+        ref.iterate(10)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, (k, errs)
+
+
+def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
+    """Tiny history and list capacities force overflows; the library grows
+    them and replays the failed step from its intact input, so the result is
+    the oracle's."""
+    sc = scenarios.random_cloud(n=300, seed=4)
+    gpu = native.build_engine(hip_lib, sc, capacity=dict(hist_slots=1, list_entries=64))
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    gpu.iterate(12)
+    ref.iterate(12)
+    errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+    assert not errs, errs
+    assert gpu.get_iter() == ref.get_iter()
+
+
+def test_two_runs_are_bitwise_identical(hip_lib):
+    sc = scenarios.random_cloud(n=2000, seed=8, periodic=(1, 0, 0), walls="yYzZ")
+    out = []
+    for _ in range(2):
+        g = native.build_engine(hip_lib, sc)
+        g.iterate(25)
+        st, ct = g.get_state(), g.get_contacts()
+        out.append((st["posr"].tobytes(), st["lin_vel"].tobytes(), ct["tan_delta"].tobytes()))
+        g.close()
+    assert out[0] == out[1]
