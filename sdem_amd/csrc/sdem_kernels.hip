@@ -593,7 +593,9 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
                                                const int* __restrict__ fstart) {
-  if (S->halt) return;
+  // a candidate-list overflow in this step's rebuild leaves voff pointing
+  // past the list storage: do nothing, the host regrows and replays
+  if (S->halt || S->overflow == 2) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = S->n_alive;
   const bool rb = S->rebuild != 0;
@@ -625,7 +627,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
     }
 
     // ---------------- moving particles of the candidate list --------------
-    const int qa = voff[k], qb = voff[k + 1];
+    const int qa = voff[k], qb = min(voff[k + 1], P.vcap);
     for (int q = qa; q < qb; q++) {
       const int kj = vlist[q];
       const int sj = rb ? perm[kj] : kj;

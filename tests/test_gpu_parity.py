@@ -72,6 +72,7 @@ def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
     """Tiny history and list capacities force overflows; the library grows
     them and replays the failed step from its intact input, so the result is
     the oracle's."""
+    print("capacity replay: hist_slots=1, list_entries=64", flush=True)
     sc = scenarios.random_cloud(n=300, seed=4)
     gpu = native.build_engine(hip_lib, sc, capacity=dict(hist_slots=1, list_entries=64))
     ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
@@ -83,6 +84,7 @@ def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
 
 
 def test_two_runs_are_bitwise_identical(hip_lib):
+    print("determinism: n=2000, 25 steps twice", flush=True)
     sc = scenarios.random_cloud(n=2000, seed=8, periodic=(1, 0, 0), walls="yYzZ")
     out = []
     for _ in range(2):
