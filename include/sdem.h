@@ -61,10 +61,11 @@ extern "C" {
 
 /* profiling phases reported by sdem_get_profile */
 #define SDEM_PHASE_WALL_LIST  0   /* UpdateNearPrtclsPW                         */
-#define SDEM_PHASE_BIN        1   /* cell keys + counting sort                  */
-#define SDEM_PHASE_SEARCH     2   /* contact search kernel                      */
-#define SDEM_PHASE_FORCE      3   /* history/force/accel/integrate kernel       */
-#define SDEM_NUM_PHASES       4
+#define SDEM_PHASE_BIN        1   /* cell keys + counting sort (rebuild steps)  */
+#define SDEM_PHASE_SEARCH     2   /* candidate lists (rebuild steps)            */
+#define SDEM_PHASE_CONTACT    3   /* contact test over the candidate lists      */
+#define SDEM_PHASE_FORCE      4   /* history/force/accel/integrate kernel       */
+#define SDEM_NUM_PHASES       5
 
 /*
  * The DEMS_Options fields the step reads (Prtcl_Parameters.f90:45-95,

@@ -28,9 +28,9 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
                         const int* cstart, int* vcnt, int* voff, int* vlist, int* tile_sum,
                         hipStream_t st);
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* voff, const int* vlist, double4* ref,
-                        const double4* fpk, const int* fid, const int* ftype, const int* fstart,
-                        hipStream_t st);
+                        const int* perm, const int* voff, const int* vlist, unsigned* clist,
+                        int* ccnt, double4* ref, const double4* fpk, const int* fid,
+                        const int* ftype, const int* fstart, hipStream_t st, hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
 }  // namespace sdem
 
@@ -205,6 +205,8 @@ struct sdem_ctx {
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
   int *vcnt = nullptr, *voff = nullptr, *vlist = nullptr;
+  unsigned* clist = nullptr;
+  int* ccnt = nullptr;
   double4* ref = nullptr;
   int* vlist_own = nullptr;
   int64_t vcap = 0;
@@ -578,6 +580,8 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vcnt, (size_t)c->cap + 2);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->voff, (size_t)c->cap + 2);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ref, c->cap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
     c->vcap = c->list_entries_hint > 0 ? c->list_entries_hint : (int64_t)c->cap * 32 + 1024;
     if (e == hipSuccess) e = hipMalloc((void**)&c->vlist, (size_t)c->vcap * 4);
     if (e == hipSuccess) c->dev_bytes += c->vcap * 4;
@@ -858,6 +862,7 @@ static int grow_history(sdem_ctx* c) {
     if (e == hipSuccess) e = regrow(c, &c->buf[b].hkey, (size_t)c->cap * neu);
     if (e == hipSuccess) e = regrow(c, &c->buf[b].hval, (size_t)c->cap * neu);
   }
+  if (e == hipSuccess) e = regrow(c, &c->clist, (size_t)c->cap * neu);
   if (e == hipSuccess) e = hipMemcpy(A.hkey, hk2.data(), hk2.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(A.hval, hv2.data(), hv2.size() * sizeof(double4), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(c, e, "grow history");
@@ -915,9 +920,10 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
                          c->tile_sum, c->stream);
       if (ev) (void)hipEventRecord(ev[3], c->stream);
       if (e == hipSuccess)
-        e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->ref, c->d_fpk,
-                         c->d_fid, c->d_ftype, c->d_fstart, c->stream);
-      if (ev) (void)hipEventRecord(ev[4], c->stream);
+        e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->clist, c->ccnt,
+                         c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->stream,
+                         ev ? ev[4] : nullptr);
+      if (ev) (void)hipEventRecord(ev[5], c->stream);
       c->cur = 1 - c->cur;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

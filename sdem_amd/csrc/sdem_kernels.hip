@@ -584,10 +584,56 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
   return xj;
 }
 
+// Contact test of one step: every particle walks its candidate list and keeps
+// the partners that touch it under the reference's predicate
+// (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557), as A-buffer indices
+// with a periodic-image flag in the top bit, in candidate order.  A light
+// kernel (few registers) so that many waves hide the gather latency.
+__global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
+                                                 const int* __restrict__ perm,
+                                                 const int* __restrict__ voff,
+                                                 const int* __restrict__ vlist,
+                                                 unsigned* __restrict__ clist,
+                                                 int* __restrict__ ccnt) {
+  if (S->halt || S->overflow == 2) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S->n_alive) return;
+  const bool rb = S->rebuild != 0;
+  const int s = rb ? perm[k] : k;
+  const double4 pi = A.pos[s];
+  const int qa = voff[k], qb = min(voff[k + 1], P.vcap);
+  unsigned* out = clist + (size_t)k * P.hcap;
+  int c = 0;
+  for (int q = qa; q < qb; q++) {
+    const int kj = vlist[q];
+    const int sj = rb ? perm[kj] : kj;
+    const double4 pj = A.pos[sj];
+    int shifted = 0;
+    const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
+    const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
+    const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
+    const double dr = pi.w + pj.w;
+    const double dr2 = dr * dr;
+    const double ddx = pi.x - xj;
+    double d2 = ddx * ddx;
+    if (d2 > dr2) continue;
+    const double ddy = pi.y - yj;
+    d2 = ddy * ddy + d2;
+    if (d2 > dr2) continue;
+    const double ddz = pi.z - zj;
+    d2 = ddz * ddz + d2;
+    if (d2 > dr2) continue;
+    if (c < P.hcap) out[c] = (unsigned)sj | (shifted ? 0x80000000u : 0u);
+    c++;
+  }
+  ccnt[k] = c;
+  if (c > P.hcap) atomicMax(&S->overflow, 1);
+}
+
 __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* S,
                                                const int* __restrict__ perm,
-                                               const int* __restrict__ voff,
-                                               const int* __restrict__ vlist,
+                                               const unsigned* __restrict__ clist,
+                                               const int* __restrict__ ccnt,
                                                double4* __restrict__ ref,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
@@ -626,28 +672,22 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       r0 = ref[k];
     }
 
-    // ---------------- moving particles of the candidate list --------------
-    const int qa = voff[k], qb = min(voff[k + 1], P.vcap);
-    for (int q = qa; q < qb; q++) {
-      const int kj = vlist[q];
-      const int sj = rb ? perm[kj] : kj;
+    // ---------------- moving particles touching it (k_contact) ------------
+    const int nc = min(ccnt[k], P.hcap);
+    const unsigned* lst = clist + (size_t)k * P.hcap;
+    for (int q = 0; q < nc; q++) {
+      const unsigned e = lst[q];
+      const int sj = (int)(e & 0x7fffffffu);
       const double4 pj = A.pos[sj];
       int shifted = 0;
       const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
       const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
       const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
-      // narrow phase, NBSM_FineSearch2/3 (Prtcl_NBS_Munjiza.f90:503-557)
       const double dr = ri + pj.w;
-      const double dr2 = dr * dr;
-      const double ddx = xi.x - xj;
+      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
       double d2 = ddx * ddx;
-      if (d2 > dr2) continue;
-      const double ddy = xi.y - yj;
       d2 = ddy * ddy + d2;
-      if (d2 > dr2) continue;
-      const double ddz = xi.z - zj;
       d2 = ddz * ddz + d2;
-      if (d2 > dr2) continue;
       const double ovrlp = dr - sqrt(d2);
       const int idj = A.id[sj];
       const int tj = A.type[sj];
@@ -950,12 +990,16 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
 }
 
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* voff, const int* vlist, double4* ref,
-                        const double4* fpk, const int* fid, const int* ftype, const int* fstart,
-                        hipStream_t st) {
-  if (n_upper > 0)
-    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, voff,
-                       vlist, ref, fpk, fid, ftype, fstart);
+                        const int* perm, const int* voff, const int* vlist, unsigned* clist,
+                        int* ccnt, double4* ref, const double4* fpk, const int* fid,
+                        const int* ftype, const int* fstart, hipStream_t st, hipEvent_t mid) {
+  if (n_upper > 0) {
+    hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, voff,
+                       vlist, clist, ccnt);
+    if (mid) (void)hipEventRecord(mid, st);
+    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, clist,
+                       ccnt, ref, fpk, fid, ftype, fstart);
+  }
   return hipGetLastError();
 }
 
