@@ -19,8 +19,11 @@
 #include "sdem_internal.h"
 
 namespace sdem {
-hipError_t launch_begin(const Params& P, Scalars* S, hipStream_t st);
-hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, int n_upper, hipStream_t st);
+hipError_t launch_begin(const Params& P, Scalars* S, const BlockPart* parts, int n_upper,
+                        hipStream_t st);
+hipError_t launch_finish(Scalars* S, const BlockPart* parts, int n_upper, hipStream_t st);
+hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, WallPart* wparts, int n_upper,
+                            hipStream_t st);
 hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
                        int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
                        int* perm, double4* pk, hipStream_t st);
@@ -30,7 +33,8 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
                         const int* perm, const int* voff, const int* vlist, unsigned* clist,
                         int* ccnt, double4* ref, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, hipStream_t st, hipEvent_t mid);
+                        const int* ftype, const int* fstart, BlockPart* parts, hipStream_t st,
+                        hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
 }  // namespace sdem
 
@@ -207,6 +211,8 @@ struct sdem_ctx {
   int *vcnt = nullptr, *voff = nullptr, *vlist = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
+  BlockPart* parts = nullptr;
+  WallPart* wparts = nullptr;
   double4* ref = nullptr;
   int* vlist_own = nullptr;
   int64_t vcap = 0;
@@ -582,6 +588,8 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ref, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->parts, (size_t)(c->cap + 255) / 256 + 1);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->wparts, (size_t)(c->cap + 255) / 256 + 1);
     c->vcap = c->list_entries_hint > 0 ? c->list_entries_hint : (int64_t)c->cap * 32 + 1024;
     if (e == hipSuccess) e = hipMalloc((void**)&c->vlist, (size_t)c->vcap * 4);
     if (e == hipSuccess) c->dev_bytes += c->vcap * 4;
@@ -908,8 +916,8 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       Buf& B = c->buf[1 - c->cur];
       hipEvent_t* ev = c->prof ? &evs[(size_t)it * NE] : nullptr;
       if (ev) (void)hipEventRecord(ev[0], c->stream);
-      e = launch_begin(c->P, c->S, c->stream);
-      if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->n_upper, c->stream);
+      e = launch_begin(c->P, c->S, c->parts, c->n_upper, c->stream);
+      if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->wparts, c->n_upper, c->stream);
       if (ev) (void)hipEventRecord(ev[1], c->stream);
       if (e == hipSuccess)
         e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
@@ -921,11 +929,12 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       if (ev) (void)hipEventRecord(ev[3], c->stream);
       if (e == hipSuccess)
         e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->clist, c->ccnt,
-                         c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->stream,
+                         c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->parts, c->stream,
                          ev ? ev[4] : nullptr);
       if (ev) (void)hipEventRecord(ev[5], c->stream);
       c->cur = 1 - c->cur;
     }
+    if (e == hipSuccess) e = launch_finish(c->S, c->parts, c->n_upper, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
       for (auto& x : evs) (void)hipEventDestroy(x);
@@ -962,7 +971,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.next_wall_update = s.next_wall_at_begin;
     s.n_alive = s.n_stored;
     s.need_rebuild = 1;
-    s.max_disp_bits = 0ull;
+    s.have_parts = 0;
     s.n_deleted = 0;
     s.overflow = 0;
     s.halt = 0;

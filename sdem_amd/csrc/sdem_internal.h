@@ -95,6 +95,20 @@ struct Params {
   const Wall* walls;          // nwall
 };
 
+// per-block partial results of the force kernel (reduced by the next step's
+// k_begin and by k_finish; no same-address atomics on the hot path)
+struct BlockPart {
+  double disp;      // largest displacement since the last list rebuild
+  int cnt[4];       // contacts by kind
+  int ndel;         // particles removed
+  int pad;
+};
+
+// per-block partial maxima of the near-wall list kernel
+struct WallPart {
+  double max_v, max_a;
+};
+
 // device-side scalars of one context
 struct Scalars {
   int n_stored;            // entries of buffer A at the start of the step
@@ -110,8 +124,9 @@ struct Scalars {
   int rebuild;             // this step rebuilds the cell sort and the lists
   int need_rebuild;        // forced rebuild at the next step
   int rebuilds;            // rebuilds so far (statistics)
+  int have_parts;          // the force kernel's block partials are current
   int num_cntcts[4];
-  unsigned long long max_v_bits, max_a_bits, max_disp_bits;
+  unsigned long long max_v_bits, max_a_bits;  // unused, kept for layout
   int use_host_max_a;      // first step after set_state: max|a| from host copy
   double host_max_a;
 };

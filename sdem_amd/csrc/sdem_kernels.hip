@@ -231,26 +231,81 @@ __device__ __forceinline__ int64_t cell_key(const Grid& G, double x, double y, d
 // ---------------------------------------------------------------------------
 // After a capacity overflow every later kernel of the call does nothing, so
 // the input buffer of the failing step stays intact and can be replayed.
-__global__ void k_begin(Params P, Scalars* S) {
+// block reduction helpers (256 threads = 4 waves)
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmax(r, sh[w]);
+  return r;
+}
+__device__ __forceinline__ long long block_sum(long long v, long long* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  long long r = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += sh[w];
+  return r;
+}
+
+// Reduce the previous step's force-kernel partials (one block of 256).
+__device__ void reduce_parts(const BlockPart* parts, int nparts, double* disp, long long cnt[4],
+                             long long* ndel) {
+  __shared__ double shd[4];
+  __shared__ long long shl[4];
+  double d = 0.0;
+  long long c[4] = {0, 0, 0, 0}, nd = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+    d = fmax(d, parts[b].disp);
+    for (int q = 0; q < 4; q++) c[q] += parts[b].cnt[q];
+    nd += parts[b].ndel;
+  }
+  *disp = block_max(d, shd);
+  for (int q = 0; q < 4; q++) cnt[q] = block_sum(c[q], shl);
+  *ndel = block_sum(nd, shl);
+}
+
+// After a capacity overflow every later kernel of the call does nothing, so
+// the input buffer of the failing step stays intact and can be replayed.
+__global__ __launch_bounds__(256) void k_begin(Params P, Scalars* S, const BlockPart* parts, int nparts) {
   if (S->overflow) {
-    S->halt = 1;
-    S->halted_steps++;
+    if (threadIdx.x == 0) {
+      S->halt = 1;
+      S->halted_steps++;
+    }
     return;
   }
-  const double disp = __longlong_as_double((long long)S->max_disp_bits);
-  S->rebuild = (S->need_rebuild || S->n_deleted > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
+  double disp = 0.0;
+  long long cnt[4], ndel = 0;
+  const bool have = S->have_parts != 0;
+  if (have) reduce_parts(parts, nparts, &disp, cnt, &ndel);
+  if (threadIdx.x != 0) return;
+  S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
   S->need_rebuild = 0;
-  S->max_disp_bits = 0ull;
+  S->have_parts = 0;
   S->n_stored = S->n_alive;
-  for (int k = 0; k < 4; k++) S->num_cntcts[k] = 0;
-  S->n_deleted = 0;
   S->iter_at_begin = S->iter;
   S->next_wall_at_begin = S->next_wall_update;
   if (S->rebuild) S->rebuilds++;
 }
 
+// counters of the last step for the host (once per call)
+__global__ __launch_bounds__(256) void k_finish(Scalars* S, const BlockPart* parts, int nparts) {
+  if (S->halt || !S->have_parts) return;
+  double disp;
+  long long cnt[4], ndel;
+  reduce_parts(parts, nparts, &disp, cnt, &ndel);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; q++) S->num_cntcts[q] = (int)cnt[q];
+    S->n_deleted = (int)ndel;
+  }
+}
+
 // UpdateNearPrtclsPW (Prtcl_ContactSearchPW.f90:159-235), particle part
-__global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) {
+__global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S, WallPart* wparts) {
   if (S->halt || S->iter < S->next_wall_update) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mv = 0.0, ma = 0.0;
@@ -268,49 +323,54 @@ __global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S) 
     mv = norm3(ld3(A.lv, P.cap, s));
     ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    mv = fmax(mv, __shfl_down(mv, off, 64));
-    ma = fmax(ma, __shfl_down(ma, off, 64));
-  }
-  __shared__ double smv[4], sma[4];
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { smv[wid] = mv; sma[wid] = ma; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < (int)(blockDim.x >> 6); k++) { mv = fmax(mv, smv[k]); ma = fmax(ma, sma[k]); }
-    atomicMax(&S->max_v_bits, (unsigned long long)__double_as_longlong(mv));
-    atomicMax(&S->max_a_bits, (unsigned long long)__double_as_longlong(ma));
-  }
+  __shared__ double sh[4];
+  mv = block_max(mv, sh);
+  ma = block_max(ma, sh);
+  if (threadIdx.x == 0) wparts[blockIdx.x] = WallPart{mv, ma};
 }
 
 // UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances
 // (only the two wall kernels read it)
-__global__ void k_wall_cadence(Params P, Scalars* S) {
+__global__ __launch_bounds__(256) void k_wall_cadence(Params P, Scalars* S, const WallPart* wparts,
+                                                      int nparts) {
   if (S->halt) return;
-  if (S->iter >= S->next_wall_update) {
-    double max_v = __longlong_as_double((long long)S->max_v_bits) + P.max_wall_vel;
-    max_v = max_v > 0.3 ? max_v : 0.3;
-    max_v = max_v * 1.5;
-    double max_a = S->use_host_max_a ? S->host_max_a : __longlong_as_double((long long)S->max_a_bits);
-    double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
-    max_a = max_a > gn ? max_a : gn;
-    max_a = max_a * 1.5;
-    int num_iter;
-    if (max_a > 0.0) {
-      double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
-      double q = t / P.dt;
-      num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
-    } else {
-      num_iter = P.wall_max_update_iter;
+  const bool update = S->iter >= S->next_wall_update;
+  __syncthreads();
+  if (update) {
+    __shared__ double sh[4];
+    double mv = 0.0, ma = 0.0;
+    if (P.nwall > 0)
+      for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        mv = fmax(mv, wparts[b].max_v);
+        ma = fmax(ma, wparts[b].max_a);
+      }
+    mv = block_max(mv, sh);
+    ma = block_max(ma, sh);
+    if (threadIdx.x == 0) {
+      double max_v = mv + P.max_wall_vel;
+      max_v = max_v > 0.3 ? max_v : 0.3;
+      max_v = max_v * 1.5;
+      double max_a = S->use_host_max_a ? S->host_max_a : ma;
+      double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
+      max_a = max_a > gn ? max_a : gn;
+      max_a = max_a * 1.5;
+      int num_iter;
+      if (max_a > 0.0) {
+        double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
+        double q = t / P.dt;
+        num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
+      } else {
+        num_iter = P.wall_max_update_iter;
+      }
+      if (num_iter < 1) num_iter = 1;
+      if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
+      S->next_wall_update = S->iter + num_iter;
     }
-    if (num_iter < 1) num_iter = 1;
-    if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
-    S->next_wall_update = S->iter + num_iter;
-    S->max_v_bits = 0ull;
-    S->max_a_bits = 0ull;
   }
-  S->use_host_max_a = 0;
-  S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
+  if (threadIdx.x == 0) {
+    S->use_host_max_a = 0;
+    S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -638,7 +698,8 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
-                                               const int* __restrict__ fstart) {
+                                               const int* __restrict__ fstart,
+                                               BlockPart* __restrict__ parts) {
   // a candidate-list overflow in this step's rebuild leaves voff pointing
   // past the list storage: do nothing, the host regrows and replays
   if (S->halt || S->overflow == 2) return;
@@ -914,20 +975,21 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       disp2 += dd * dd;
     }
   }
-  // per-wave reductions, one atomic per wave
-  for (int q = 0; q < 4; q++) {
-    int v = acc.nkind[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&S->num_cntcts[q], v);
-  }
-  int nd = dead ? 1 : 0;
-  for (int off = 32; off > 0; off >>= 1) {
-    disp2 = fmax(disp2, __shfl_down(disp2, off, 64));
-    nd += __shfl_down(nd, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (disp2 > 0.0) atomicMax(&S->max_disp_bits, (unsigned long long)__double_as_longlong(sqrt(disp2)));
-    if (nd) atomicAdd(&S->n_deleted, nd);
+  // block partials for the next step's k_begin (no same-address atomics)
+  __shared__ double shd[4];
+  __shared__ long long shl[4];
+  const double bdisp = block_max(disp2, shd);
+  long long bc[4];
+  for (int q = 0; q < 4; q++) bc[q] = block_sum(acc.nkind[q], shl);
+  const long long bdel = block_sum(dead ? 1 : 0, shl);
+  if (threadIdx.x == 0) {
+    BlockPart bp;
+    bp.disp = sqrt(bdisp);
+    for (int q = 0; q < 4; q++) bp.cnt[q] = (int)bc[q];
+    bp.ndel = (int)bdel;
+    bp.pad = 0;
+    parts[blockIdx.x] = bp;
+    if (blockIdx.x == 0) S->have_parts = 1;
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
 }
@@ -946,15 +1008,22 @@ static void scan(const int* in, int* out, int* tile_sum, int64_t n, const Scalar
 
 int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_begin(const Params& P, Scalars* S, hipStream_t st) {
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, P, S);
+hipError_t launch_begin(const Params& P, Scalars* S, const BlockPart* parts, int n_upper,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(256), 0, st, P, S, parts, (int)nblk(n_upper, 256));
   return hipGetLastError();
 }
 
-hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, int n_upper, hipStream_t st) {
+hipError_t launch_finish(Scalars* S, const BlockPart* parts, int n_upper, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, S, parts, (int)nblk(n_upper, 256));
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, WallPart* wparts, int n_upper,
+                            hipStream_t st) {
   if (P.nwall > 0 && n_upper > 0)
-    hipLaunchKernelGGL(k_wall_list, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S);
-  hipLaunchKernelGGL(k_wall_cadence, dim3(1), dim3(1), 0, st, P, S);
+    hipLaunchKernelGGL(k_wall_list, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, wparts);
+  hipLaunchKernelGGL(k_wall_cadence, dim3(1), dim3(256), 0, st, P, S, wparts, (int)nblk(n_upper, 256));
   return hipGetLastError();
 }
 
@@ -992,13 +1061,14 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
                         const int* perm, const int* voff, const int* vlist, unsigned* clist,
                         int* ccnt, double4* ref, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, hipStream_t st, hipEvent_t mid) {
+                        const int* ftype, const int* fstart, BlockPart* parts, hipStream_t st,
+                        hipEvent_t mid) {
   if (n_upper > 0) {
     hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, voff,
                        vlist, clist, ccnt);
     if (mid) (void)hipEventRecord(mid, st);
     hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, clist,
-                       ccnt, ref, fpk, fid, ftype, fstart);
+                       ccnt, ref, fpk, fid, ftype, fstart, parts);
   }
   return hipGetLastError();
 }
