@@ -220,6 +220,8 @@ struct sdem_ctx {
   double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
   int keep_forces = 1;
   bool first_step_pending = false;
+  bool use_graph = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
   int hist_slots_hint = 0;
   int64_t list_entries_hint = 0;
   int* tile_sum = nullptr;
@@ -253,6 +255,14 @@ namespace {
 
 int hip_fail(sdem_ctx* c, hipError_t e, const char* where) {
   return c->fail(SDEM_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+void invalidate_graphs(sdem_ctx* c) {
+  for (auto& g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 template <class T>
@@ -728,6 +738,7 @@ int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind, const int32_t
 
 static int prepare(sdem_ctx* c) {
   if (c->prepared) return SDEM_OK;
+  invalidate_graphs(c);
   free_list(c, c->allocs_tab);
   Params& P = c->P;
   P.dt = c->o.dt;
@@ -876,6 +887,7 @@ static int grow_history(sdem_ctx* c) {
   if (e != hipSuccess) return hip_fail(c, e, "grow history");
   c->hcap = neu;
   c->P.hcap = neu;
+  invalidate_graphs(c);
   return SDEM_OK;
 }
 
@@ -894,7 +906,54 @@ static int grow_vlist(sdem_ctx* c) {
   c->dev_bytes += neu * 4;
   c->vcap = neu;
   c->P.vcap = (int)neu;
+  invalidate_graphs(c);
   return SDEM_OK;
+}
+
+// Queue one step on the context's stream; ev (SDEM_NUM_PHASES+1 events) may
+// be null.  The buffers swap roles afterwards.
+static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
+  Buf& A = c->buf[c->cur];
+  Buf& B = c->buf[1 - c->cur];
+  if (ev) (void)hipEventRecord(ev[0], c->stream);
+  hipError_t e = launch_begin(c->P, c->S, c->parts, c->n_upper, c->stream);
+  if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->wparts, c->n_upper, c->stream);
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess)
+    e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
+                    c->tile_sum, c->perm_tmp, c->perm, c->pk, c->stream);
+  if (ev) (void)hipEventRecord(ev[2], c->stream);
+  if (e == hipSuccess)
+    e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->voff, c->vlist, c->tile_sum,
+                     c->stream);
+  if (ev) (void)hipEventRecord(ev[3], c->stream);
+  if (e == hipSuccess)
+    e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->clist, c->ccnt,
+                     c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->parts, c->stream,
+                     ev ? ev[4] : nullptr);
+  if (ev) (void)hipEventRecord(ev[5], c->stream);
+  c->cur = 1 - c->cur;
+  return e;
+}
+
+// A graph of two consecutive steps starting from buffer c->cur (the kernels
+// read their control flags from device memory, so the graph is fixed).
+static hipError_t pair_graph(sdem_ctx* c, hipGraphExec_t* out) {
+  if (c->gexec[c->cur]) {
+    *out = c->gexec[c->cur];
+    return hipSuccess;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t e1 = enqueue_step(c, nullptr);
+  if (e1 == hipSuccess) e1 = enqueue_step(c, nullptr);
+  e = hipStreamEndCapture(c->stream, &g);
+  if (e1 != hipSuccess) e = e1;
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec[c->cur], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e == hipSuccess) *out = c->gexec[c->cur];
+  return e;
 }
 
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
@@ -911,29 +970,16 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
     const int c0 = c->cur;
     hipError_t e = hipSuccess;
-    for (int it = 0; it < remaining && e == hipSuccess; it++) {
-      Buf& A = c->buf[c->cur];
-      Buf& B = c->buf[1 - c->cur];
-      hipEvent_t* ev = c->prof ? &evs[(size_t)it * NE] : nullptr;
-      if (ev) (void)hipEventRecord(ev[0], c->stream);
-      e = launch_begin(c->P, c->S, c->parts, c->n_upper, c->stream);
-      if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->wparts, c->n_upper, c->stream);
-      if (ev) (void)hipEventRecord(ev[1], c->stream);
-      if (e == hipSuccess)
-        e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
-                        c->tile_sum, c->perm_tmp, c->perm, c->pk, c->stream);
-      if (ev) (void)hipEventRecord(ev[2], c->stream);
-      if (e == hipSuccess)
-        e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->voff, c->vlist,
-                         c->tile_sum, c->stream);
-      if (ev) (void)hipEventRecord(ev[3], c->stream);
-      if (e == hipSuccess)
-        e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->clist, c->ccnt,
-                         c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->parts, c->stream,
-                         ev ? ev[4] : nullptr);
-      if (ev) (void)hipEventRecord(ev[5], c->stream);
-      c->cur = 1 - c->cur;
+    int it = 0;
+    if (!c->prof && c->use_graph) {
+      for (; it + 2 <= remaining && e == hipSuccess; it += 2) {
+        hipGraphExec_t g = nullptr;
+        e = pair_graph(c, &g);
+        if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
+      }
     }
+    for (; it < remaining && e == hipSuccess; it++)
+      e = enqueue_step(c, c->prof ? &evs[(size_t)it * NE] : nullptr);
     if (e == hipSuccess) e = launch_finish(c->S, c->parts, c->n_upper, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -1217,6 +1263,7 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
   if (keep_forces >= 0) {
     c->keep_forces = keep_forces ? 1 : 0;
     c->P.keep_forces = c->keep_forces;
+    invalidate_graphs(c);
   }
   return SDEM_OK;
 }
@@ -1242,6 +1289,7 @@ const char* sdem_last_error(const sdem_ctx* c) { return c ? c->err.c_str() : "nu
 void sdem_destroy(sdem_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  invalidate_graphs(c);
   if (c->vlist) (void)hipFree(c->vlist);
   free_list(c, c->allocs_state);
   free_list(c, c->allocs_grid);
