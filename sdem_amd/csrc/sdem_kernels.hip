@@ -647,7 +647,7 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // Contact test of one step: every particle walks its candidate list and keeps
 // the partners that touch it under the reference's predicate
 // (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557), as A-buffer indices
-// with a periodic-image flag in the top bit, in candidate order.  A light
+// with a periodic-image flag in the top bit, sorted by partner id.  A light
 // kernel (few registers) so that many waves hide the gather latency.
 __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
                                                  const int* __restrict__ perm,
@@ -687,7 +687,22 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
     c++;
   }
   ccnt[k] = c;
-  if (c > P.hcap) atomicMax(&S->overflow, 1);
+  if (c > P.hcap) {
+    atomicMax(&S->overflow, 1);
+    return;
+  }
+  // canonical order: ascending partner id, so the sums of a particle's
+  // contact forces do not depend on the grid, the skin or the storage order
+  for (int a = 1; a < c; a++) {
+    const unsigned e = out[a];
+    const int ida = A.id[e & 0x7fffffffu];
+    int b = a - 1;
+    while (b >= 0 && A.id[out[b] & 0x7fffffffu] > ida) {
+      out[b + 1] = out[b];
+      b--;
+    }
+    out[b + 1] = e;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* S,
