@@ -206,10 +206,11 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
 /* Initial capacities (no counterpart in the reference, whose contact list
  * holds CntctList_Size * mlocal records in one pool): hist_slots history
  * records per particle (0 = a kissing-number bound from the type radii) and
- * list_entries candidate-list entries in total (0 = 32 per particle).  Both
- * grow on demand: a step that runs out is replayed from its intact input
- * after the storage has been enlarged.  Call before sdem_set_state. */
-int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int64_t list_entries);
+ * list_slots candidate-list entries per particle (0 = twice hist_slots, at
+ * least 48).  Both grow on demand: a step that runs out is replayed from its
+ * intact input after the storage has been enlarged.  Call before
+ * sdem_set_state. */
+int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots);
 
 /* Memory the context holds on the device, in bytes. */
 int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);

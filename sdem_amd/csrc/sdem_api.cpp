@@ -19,21 +19,18 @@
 #include "sdem_internal.h"
 
 namespace sdem {
-hipError_t launch_begin(const Params& P, Scalars* S, const BlockPart* parts, int n_upper,
-                        hipStream_t st);
-hipError_t launch_finish(Scalars* S, const BlockPart* parts, int n_upper, hipStream_t st);
-hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, WallPart* wparts, int n_upper,
-                            hipStream_t st);
+hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
+                      hipStream_t st);
+hipError_t launch_finish(Scalars* S, const Slot* slots, hipStream_t st);
 hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
                        int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
                        int* perm, double4* pk, hipStream_t st);
 hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
-                        const int* cstart, int* vcnt, int* voff, int* vlist, int* tile_sum,
-                        hipStream_t st);
+                        const int* cstart, int* vcnt, int* vlist, hipStream_t st);
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* voff, const int* vlist, unsigned* clist,
+                        const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
                         int* ccnt, double4* ref, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, BlockPart* parts, hipStream_t st,
+                        const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
                         hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
 }  // namespace sdem
@@ -208,14 +205,12 @@ struct sdem_ctx {
   int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr;
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
-  int *vcnt = nullptr, *voff = nullptr, *vlist = nullptr;
+  int *vcnt = nullptr, *vlist = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
-  BlockPart* parts = nullptr;
-  WallPart* wparts = nullptr;
+  Slot* slots = nullptr;
   double4* ref = nullptr;
-  int* vlist_own = nullptr;
-  int64_t vcap = 0;
+  int lcap = 0;
   double cell_factor[3] = {1.0, 1.0, 1.0};
   double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
   int keep_forces = 1;
@@ -223,7 +218,7 @@ struct sdem_ctx {
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
   int hist_slots_hint = 0;
-  int64_t list_entries_hint = 0;
+  int list_slots_hint = 0;
   int* tile_sum = nullptr;
   Bnry *d_pp = nullptr, *d_pw = nullptr;
   double *d_mass = nullptr, *d_inertia = nullptr;
@@ -594,15 +589,12 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->pk, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vcnt, (size_t)c->cap + 2);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->voff, (size_t)c->cap + 2);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ref, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->parts, (size_t)(c->cap + 255) / 256 + 1);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->wparts, (size_t)(c->cap + 255) / 256 + 1);
-    c->vcap = c->list_entries_hint > 0 ? c->list_entries_hint : (int64_t)c->cap * 32 + 1024;
-    if (e == hipSuccess) e = hipMalloc((void**)&c->vlist, (size_t)c->vcap * 4);
-    if (e == hipSuccess) c->dev_bytes += c->vcap * 4;
+    c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vlist, (size_t)c->cap * c->lcap);
+    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->slots, NSLOT);
     if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(sort)");
   }
   const int cap = c->cap, ts = c->tsize, rs = c->rsize;
@@ -661,6 +653,7 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   s.host_max_a = host_max_a;
   s.need_rebuild = 1;
   H(c->S, &s, sizeof(s));
+  if (e == hipSuccess) e = hipMemset(c->slots, 0, sizeof(Slot) * NSLOT);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
   // search levels: radii never change, so the per-level largest radius is
   // fixed by the state
@@ -689,8 +682,9 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   if (base + 2 > INT32_MAX) return c->fail(SDEM_E_ARG, "search grid too large");
   c->dead_key = base;
   e = dalloc(c, c->allocs_grid, &c->counts, (size_t)base + 2);
+  if (e == hipSuccess) e = hipMemset(c->counts, 0, ((size_t)base + 2) * sizeof(int));
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)base + 2);
-  int64_t nt = std::max(scan_tiles(base + 2), scan_tiles((int64_t)c->cap + 2));
+  int64_t nt = scan_tiles(base + 2);
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)nt + 1);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(grid)");
   return SDEM_OK;
@@ -717,7 +711,7 @@ int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind, const int32_t
     if (types[s] >= 1) idx.push_back({ids[s], s});
   std::sort(idx.begin(), idx.end());
   std::vector<int> hcnt(n, 0), hkey((size_t)n * c->hcap, 0);
-  std::vector<double4> hval((size_t)n * c->hcap);
+  std::vector<double4> hval((size_t)n * c->hcap);  // record q of particle s at q*n+s here
   for (int64_t k = 0; k < m; k++) {
     auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(owner_id[k], -1));
     if (it == idx.end() || it->first != owner_id[k]) return c->fail(SDEM_E_ARG, "owner id not present");
@@ -725,13 +719,17 @@ int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind, const int32_t
     if (partner_id[k] < 0 || partner_id[k] > KEY_MASK) return c->fail(SDEM_E_ARG, "partner id out of range");
     int s = it->second;
     if (hcnt[s] >= c->hcap) return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
-    size_t slot = (size_t)s * c->hcap + hcnt[s]++;
+    size_t slot = (size_t)hcnt[s]++ * n + s;
     hkey[slot] = partner_id[k] | (kind[k] << KEY_BITS);
     hval[slot] = make_double4(tan_delta[3 * k], tan_delta[3 * k + 1], tan_delta[3 * k + 2], vrel_init[k]);
   }
   e = hipMemcpy(A.hcnt, hcnt.data(), (size_t)n * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(A.hkey, hkey.data(), hkey.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(A.hval, hval.data(), hval.size() * sizeof(double4), hipMemcpyHostToDevice);
+  for (int q = 0; q < c->hcap && e == hipSuccess; q++) {
+    e = hipMemcpy(A.hkey + (size_t)q * c->cap, hkey.data() + (size_t)q * n, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(A.hval + (size_t)q * c->cap, hval.data() + (size_t)q * n, (size_t)n * sizeof(double4),
+                    hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(contacts)");
   return SDEM_OK;
 }
@@ -774,7 +772,7 @@ static int prepare(sdem_ctx* c) {
   P.wall_max_update_iter = c->o.wall_max_update_iter;
   P.skin = c->skin_factor * maxd;
   P.rebuild_disp = 0.5 * P.skin * (1.0 - 1.0e-6);
-  P.vcap = (int)std::min<int64_t>(c->vcap, INT32_MAX);
+  P.lcap = c->lcap;
   for (auto& w : c->walls)
     if (w.wall_type > c->nwtype) return c->fail(SDEM_E_ARG, "wall_type exceeds numWall_type");
   std::vector<double> mass(c->ntype), inertia(c->ntype);
@@ -858,32 +856,33 @@ static hipError_t regrow(sdem_ctx* c, T** p, size_t n) {
   return dalloc(c, L, p, n);
 }
 
-// History slots per particle doubled; the history of buffer A is re-strided.
+// History slots per particle doubled.  Record q of particle s sits at
+// q*cap+s, so the records of buffer A keep their places in the larger arrays.
 static int grow_history(sdem_ctx* c) {
   const int old = c->hcap, neu = std::min(old * 2, 4096);
   if (neu <= old) return c->fail(SDEM_E_CAPACITY, "contact history slots exhausted for a particle");
-  const size_t n = (size_t)c->n_upper;
+  const size_t cap = (size_t)c->cap;
   Buf& A = c->buf[c->cur];
-  std::vector<int> hc, hk;
-  std::vector<double4> hv;
-  hipError_t e = dl(hc, A.hcnt, n);
-  if (e == hipSuccess) e = dl(hk, A.hkey, n * old);
-  if (e == hipSuccess) e = dl(hv, A.hval, n * old);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
-  std::vector<int> hk2(n * neu, 0);
-  std::vector<double4> hv2(n * neu);
-  for (size_t k = 0; k < n; k++)
-    for (int q = 0; q < std::min(hc[k], old); q++) {
-      hk2[k * neu + q] = hk[k * old + q];
-      hv2[k * neu + q] = hv[k * old + q];
-    }
-  for (int b = 0; b < 2; b++) {
-    if (e == hipSuccess) e = regrow(c, &c->buf[b].hkey, (size_t)c->cap * neu);
-    if (e == hipSuccess) e = regrow(c, &c->buf[b].hval, (size_t)c->cap * neu);
-  }
-  if (e == hipSuccess) e = regrow(c, &c->clist, (size_t)c->cap * neu);
-  if (e == hipSuccess) e = hipMemcpy(A.hkey, hk2.data(), hk2.size() * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(A.hval, hv2.data(), hv2.size() * sizeof(double4), hipMemcpyHostToDevice);
+  int* hk = nullptr;
+  double4* hv = nullptr;
+  hipError_t e = dalloc(c, c->allocs_state, &hk, cap * neu);
+  if (e == hipSuccess) e = dalloc(c, c->allocs_state, &hv, cap * neu);
+  if (e == hipSuccess) e = hipMemcpy(hk, A.hkey, cap * old * sizeof(int), hipMemcpyDeviceToDevice);
+  if (e == hipSuccess) e = hipMemcpy(hv, A.hval, cap * old * sizeof(double4), hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "grow history");
+  auto drop = [&](void* p) {
+    auto it = std::find(c->allocs_state.begin(), c->allocs_state.end(), p);
+    if (it != c->allocs_state.end()) c->allocs_state.erase(it);
+    (void)hipFree(p);
+  };
+  drop(A.hkey);
+  drop(A.hval);
+  A.hkey = hk;
+  A.hval = hv;
+  Buf& B = c->buf[1 - c->cur];
+  e = regrow(c, &B.hkey, cap * neu);
+  if (e == hipSuccess) e = regrow(c, &B.hval, cap * neu);
+  if (e == hipSuccess) e = regrow(c, &c->clist, cap * neu);
   if (e != hipSuccess) return hip_fail(c, e, "grow history");
   c->hcap = neu;
   c->P.hcap = neu;
@@ -891,21 +890,17 @@ static int grow_history(sdem_ctx* c) {
   return SDEM_OK;
 }
 
-// Candidate-list storage sized to what the failed rebuild counted.
+// Candidate-list slots per particle sized to what the failed rebuild counted.
 static int grow_vlist(sdem_ctx* c) {
-  int need = 0;
-  hipError_t e = hipMemcpy(&need, c->voff + c->n_upper, sizeof(int), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(voff)");
-  int64_t neu = std::max<int64_t>((int64_t)need + need / 4 + 1024, c->vcap * 2);
-  if (neu > INT32_MAX) return c->fail(SDEM_E_CAPACITY, "candidate lists exceed 2^31 entries");
-  (void)hipFree(c->vlist);
-  c->dev_bytes -= c->vcap * 4;
-  c->vlist = nullptr;
-  e = hipMalloc((void**)&c->vlist, (size_t)neu * 4);
+  Scalars s;
+  hipError_t e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
+  int neu = std::max(s.list_need + s.list_need / 4 + 8, 2 * c->lcap);
+  if ((int64_t)neu * c->cap > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
+  e = regrow(c, &c->vlist, (size_t)c->cap * neu);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
-  c->dev_bytes += neu * 4;
-  c->vcap = neu;
-  c->P.vcap = (int)neu;
+  c->lcap = neu;
+  c->P.lcap = neu;
   invalidate_graphs(c);
   return SDEM_OK;
 }
@@ -916,20 +911,17 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
   Buf& A = c->buf[c->cur];
   Buf& B = c->buf[1 - c->cur];
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipError_t e = launch_begin(c->P, c->S, c->parts, c->n_upper, c->stream);
-  if (e == hipSuccess) e = launch_wall_list(c->P, A, c->S, c->wparts, c->n_upper, c->stream);
+  hipError_t e = launch_pre(c->P, A, c->S, c->slots, c->n_upper, c->stream);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (e == hipSuccess)
     e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
                     c->tile_sum, c->perm_tmp, c->perm, c->pk, c->stream);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
-  if (e == hipSuccess)
-    e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->voff, c->vlist, c->tile_sum,
-                     c->stream);
+  if (e == hipSuccess) e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->vlist, c->stream);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
   if (e == hipSuccess)
-    e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->voff, c->vlist, c->clist, c->ccnt,
-                     c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->parts, c->stream,
+    e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->vcnt, c->vlist, c->clist, c->ccnt,
+                     c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->slots, c->stream,
                      ev ? ev[4] : nullptr);
   if (ev) (void)hipEventRecord(ev[5], c->stream);
   c->cur = 1 - c->cur;
@@ -980,7 +972,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     for (; it < remaining && e == hipSuccess; it++)
       e = enqueue_step(c, c->prof ? &evs[(size_t)it * NE] : nullptr);
-    if (e == hipSuccess) e = launch_finish(c->S, c->parts, c->n_upper, c->stream);
+    if (e == hipSuccess) e = launch_finish(c->S, c->slots, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
       for (auto& x : evs) (void)hipEventDestroy(x);
@@ -1019,6 +1011,9 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.need_rebuild = 1;
     s.have_parts = 0;
     s.n_deleted = 0;
+    s.ticket = 0;
+    s.ticket_scan = 0;
+    s.list_need = 0;
     s.overflow = 0;
     s.halt = 0;
     s.halted_steps = 0;
@@ -1156,14 +1151,14 @@ static int collect(sdem_ctx* c, std::vector<Rec>& recs, std::vector<double4>& hv
   std::vector<int> hc, hk;
   size_t n = (size_t)c->n_upper;
   hipError_t e = dl(hc, A.hcnt, n);
-  if (e == hipSuccess) e = dl(hk, A.hkey, n * c->hcap);
-  if (e == hipSuccess) e = dl(hv, A.hval, n * c->hcap);
+  if (e == hipSuccess) e = dl(hk, A.hkey, (size_t)c->cap * c->hcap);
+  if (e == hipSuccess) e = dl(hv, A.hval, (size_t)c->cap * c->hcap);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
   recs.clear();
   for (auto& pr : idx) {
     int k = pr.second;
     for (int q = 0; q < hc[k]; q++) {
-      size_t slot = (size_t)k * c->hcap + q;
+      size_t slot = (size_t)q * c->cap + k;
       recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), slot});
     }
   }
@@ -1268,13 +1263,13 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
   return SDEM_OK;
 }
 
-int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int64_t list_entries) {
+int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots) {
   if (!c) return SDEM_E_ARG;
   if (!c->allocs_state.empty()) return c->fail(SDEM_E_STATE, "capacities must be set before the state");
-  if (hist_slots < 0 || hist_slots > 4096 || list_entries < 0 || list_entries > INT32_MAX)
+  if (hist_slots < 0 || hist_slots > 4096 || list_slots < 0 || list_slots > 65536)
     return c->fail(SDEM_E_ARG, "capacity out of range");
   c->hist_slots_hint = hist_slots;
-  c->list_entries_hint = list_entries;
+  c->list_slots_hint = list_slots;
   return SDEM_OK;
 }
 
@@ -1290,7 +1285,6 @@ void sdem_destroy(sdem_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   invalidate_graphs(c);
-  if (c->vlist) (void)hipFree(c->vlist);
   free_list(c, c->allocs_state);
   free_list(c, c->allocs_grid);
   free_list(c, c->allocs_tab);

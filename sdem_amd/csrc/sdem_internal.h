@@ -60,8 +60,8 @@ struct Buf {
   int *id, *type, *mark;
   uint64_t *wmask;  // near-wall list as a bit mask over wall indices
   int *hcnt;        // history records owned by the particle
-  int *hkey;        // cap*hcap partner keys
-  double4 *hval;    // cap*hcap (TanDelta.x,.y,.z, VelRel_Init)
+  int *hkey;        // hcap x cap partner keys, record c of particle s at c*cap+s
+  double4 *hval;    // hcap x cap (TanDelta.x,.y,.z, VelRel_Init), same layout
   double *force;    // 6 x cap (F, T) when forces are kept
 };
 
@@ -86,7 +86,7 @@ struct Params {
   double dxmax1, dxmax2, max_wall_vel;
   double skin;                // candidate-list skin (search tuning)
   double rebuild_disp;        // rebuild when a particle moved further than this
-  int vcap;                   // capacity of the candidate-list entries
+  int lcap;                   // candidate-list entries per particle
   int wall_max_update_iter;
   const Bnry* pp;             // ntype*ntype, row = type of the record's lower-id particle
   const Bnry* pw;             // ntype*nwtype
@@ -95,18 +95,18 @@ struct Params {
   const Wall* walls;          // nwall
 };
 
-// per-block partial results of the force kernel (reduced by the next step's
-// k_begin and by k_finish; no same-address atomics on the hot path)
-struct BlockPart {
-  double disp;      // largest displacement since the last list rebuild
-  int cnt[4];       // contacts by kind
-  int ndel;         // particles removed
+// Partial results of the per-particle kernels, accumulated with atomics
+// spread over NSLOT addresses (block b uses slot b % NSLOT) so that no
+// single address takes thousands of atomics; reduced by the next step's
+// control block and by k_finish.
+constexpr int NSLOT = 64;
+struct Slot {
+  unsigned long long disp_bits;  // largest displacement since the last list rebuild
+  unsigned long long max_v_bits; // near-wall list: max |v|, max |a| (as bits of
+  unsigned long long max_a_bits; // non-negative doubles)
+  int cnt[4];                    // contacts by kind
+  int ndel;                      // particles removed
   int pad;
-};
-
-// per-block partial maxima of the near-wall list kernel
-struct WallPart {
-  double max_v, max_a;
 };
 
 // device-side scalars of one context
@@ -124,7 +124,10 @@ struct Scalars {
   int rebuild;             // this step rebuilds the cell sort and the lists
   int need_rebuild;        // forced rebuild at the next step
   int rebuilds;            // rebuilds so far (statistics)
-  int have_parts;          // the force kernel's block partials are current
+  int have_parts;          // the force kernel's slots are current
+  unsigned ticket;         // last-block election in k_pre
+  unsigned ticket_scan;    // last-block election in k_scan_tiles
+  int list_need;           // largest candidate count seen when a list overflowed
   int num_cntcts[4];
   unsigned long long max_v_bits, max_a_bits;  // unused, kept for layout
   int use_host_max_a;      // first step after set_state: max|a| from host copy

@@ -6,20 +6,21 @@
 // decided every step with the reference's exact predicate, so it does not
 // depend on when lists are rebuilt.
 //
-//   k_begin        1 thread : step counters; decides whether to rebuild
-//   k_wall_list    N        : near-wall bit masks + max|v|, max|a| (update
-//                             iterations only, UpdateNearPrtclsPW)
-//   k_wall_cadence 1 thread : Next_iter_update; iterNumber + 1
-//   -- rebuild steps only --
-//   k_fill/k_bin            : (level, cell) key per particle, per-key counts
-//   k_scan*                 : exclusive scan of the counts
+//   k_pre                   : near-wall bit masks (update iterations); its
+//                             last block decides the rebuild, runs the
+//                             near-wall cadence and advances iterNumber
+//   -- rebuild steps only (the kernels return at once otherwise) --
+//   k_bin                   : (level, cell) key per particle, per-key counts
+//   k_scan_tiles/k_scan_add : exclusive scan of the counts
 //   k_scatter / k_rank      : deterministic counting sort (in-cell order =
 //                             storage order) + packed positions in cell order
-//   k_vlist (count, fill)   : CSR candidate lists, pairs within ri+rj+skin
+//   k_vlist                 : candidate lists, pairs within ri+rj+skin
 //   -- every step --
-//   k_force        N        : contact test, history match, contact forces,
-//                             accelerations, integration, wrap/delete, and
-//                             the displacement since the last rebuild
+//   k_contact               : exact contact test over the candidate lists,
+//                             partners sorted by id
+//   k_force                 : history match, contact forces, accelerations,
+//                             integration, wrap/delete, and the displacement
+//                             since the last rebuild
 //
 // Floating-point kernels are built with -ffp-contract=off so that every
 // product and sum is rounded on its own, in the order the reference's
@@ -227,11 +228,8 @@ __device__ __forceinline__ int64_t cell_key(const Grid& G, double x, double y, d
 }
 
 // ---------------------------------------------------------------------------
-// step control
+// step control + near-wall list: k_pre
 // ---------------------------------------------------------------------------
-// After a capacity overflow every later kernel of the call does nothing, so
-// the input buffer of the failing step stays intact and can be replayed.
-// block reduction helpers (256 threads = 4 waves)
 __device__ __forceinline__ double block_max(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
   __syncthreads();
@@ -241,151 +239,152 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmax(r, sh[w]);
   return r;
 }
-__device__ __forceinline__ long long block_sum(long long v, long long* sh) {
+__device__ __forceinline__ int block_sum(int v, int* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  long long r = 0;
+  int r = 0;
   for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += sh[w];
   return r;
 }
-
-// Reduce the previous step's force-kernel partials (one block of 256).
-__device__ void reduce_parts(const BlockPart* parts, int nparts, double* disp, long long cnt[4],
-                             long long* ndel) {
-  __shared__ double shd[4];
-  __shared__ long long shl[4];
-  double d = 0.0;
-  long long c[4] = {0, 0, 0, 0}, nd = 0;
-  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-    d = fmax(d, parts[b].disp);
-    for (int q = 0; q < 4; q++) c[q] += parts[b].cnt[q];
-    nd += parts[b].ndel;
-  }
-  *disp = block_max(d, shd);
-  for (int q = 0; q < 4; q++) cnt[q] = block_sum(c[q], shl);
-  *ndel = block_sum(nd, shl);
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
 }
+__device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// After a capacity overflow every later kernel of the call does nothing, so
-// the input buffer of the failing step stays intact and can be replayed.
-__global__ __launch_bounds__(256) void k_begin(Params P, Scalars* S, const BlockPart* parts, int nparts) {
-  if (S->overflow) {
+// One kernel per step for the control work:
+//  * on update iterations every block builds the near-wall bit masks of its
+//    particles and folds max|v|, max|a| into the slots (UpdateNearPrtclsPW,
+//    Prtcl_ContactSearchPW.f90:159-235);
+//  * the last block to finish reduces the previous step's force slots,
+//    decides whether the cell sort and the candidate lists are rebuilt,
+//    computes the near-wall cadence (:201-229) and advances iterNumber.
+// After a capacity overflow the last block only marks the call halted, so
+// every later kernel of the call does nothing and the failing step's input
+// buffer stays intact for the replay.
+__global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* slots) {
+  __shared__ double shd[4];
+  __shared__ bool last;
+  if (S->halt) return;
+  const int iter = S->iter, next = S->next_wall_update;
+  const bool update = iter >= next;  // max|v|, max|a| are taken even without walls
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (update) {
+    double mv = 0.0, ma = 0.0;
+    if (s < S->n_alive) {
+      const double4 p = A.pos[s];
+      uint64_t mask = 0;
+      for (int w = 0; w < P.nwall; w++) {
+        const Wall& W = P.walls[w];
+        const double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2;
+        const double mnz = W.min_point[2] - P.dxmax2;
+        const double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2;
+        const double mxz = W.max_point[2] + P.dxmax2;
+        if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
+          mask |= (1ull << w);
+      }
+      A.wmask[s] = mask;
+      mv = norm3(ld3(A.lv, P.cap, s));
+      ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
+    }
+    mv = block_max(mv, shd);
+    ma = block_max(ma, shd);
     if (threadIdx.x == 0) {
+      Slot& sl = slots[blockIdx.x % NSLOT];
+      atomicMax(&sl.max_v_bits, dbits(mv));
+      atomicMax(&sl.max_a_bits, dbits(ma));
+    }
+  }
+  // last-block election
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&S->ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last || threadIdx.x >= 64) return;
+  __threadfence();
+  const int lane = threadIdx.x;  // one wave does the control work
+  if (S->overflow) {
+    if (lane == 0) {
       S->halt = 1;
       S->halted_steps++;
+      S->ticket = 0;
     }
     return;
   }
-  double disp = 0.0;
-  long long cnt[4], ndel = 0;
+  Slot& sl = slots[lane];
+  double disp = bitsd(sl.disp_bits), mv = bitsd(sl.max_v_bits), ma = bitsd(sl.max_a_bits);
+  int ndel = sl.ndel;
+  for (int off = 32; off > 0; off >>= 1) {
+    disp = fmax(disp, __shfl_down(disp, off, 64));
+    mv = fmax(mv, __shfl_down(mv, off, 64));
+    ma = fmax(ma, __shfl_down(ma, off, 64));
+    ndel += __shfl_down(ndel, off, 64);
+  }
+  sl.disp_bits = 0ull;
+  sl.max_v_bits = 0ull;
+  sl.max_a_bits = 0ull;
+  sl.ndel = 0;
+  for (int q = 0; q < 4; q++) sl.cnt[q] = 0;
+  if (lane != 0) return;
   const bool have = S->have_parts != 0;
-  if (have) reduce_parts(parts, nparts, &disp, cnt, &ndel);
-  if (threadIdx.x != 0) return;
   S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
   S->need_rebuild = 0;
   S->have_parts = 0;
   S->n_stored = S->n_alive;
-  S->iter_at_begin = S->iter;
-  S->next_wall_at_begin = S->next_wall_update;
+  S->iter_at_begin = iter;
+  S->next_wall_at_begin = next;
   if (S->rebuild) S->rebuilds++;
+  if (iter >= next) {
+    double max_v = mv + P.max_wall_vel;
+    max_v = max_v > 0.3 ? max_v : 0.3;
+    max_v = max_v * 1.5;
+    double max_a = S->use_host_max_a ? S->host_max_a : ma;
+    const double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
+    max_a = max_a > gn ? max_a : gn;
+    max_a = max_a * 1.5;
+    int num_iter;
+    if (max_a > 0.0) {
+      const double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
+      const double q = t / P.dt;
+      num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
+    } else {
+      num_iter = P.wall_max_update_iter;
+    }
+    if (num_iter < 1) num_iter = 1;
+    if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
+    S->next_wall_update = iter + num_iter;
+  }
+  S->use_host_max_a = 0;
+  S->iter = iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
+  S->ticket = 0;
 }
 
 // counters of the last step for the host (once per call)
-__global__ __launch_bounds__(256) void k_finish(Scalars* S, const BlockPart* parts, int nparts) {
+__global__ __launch_bounds__(64) void k_finish(Scalars* S, const Slot* slots) {
   if (S->halt || !S->have_parts) return;
-  double disp;
-  long long cnt[4], ndel;
-  reduce_parts(parts, nparts, &disp, cnt, &ndel);
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 4; q++) S->num_cntcts[q] = (int)cnt[q];
-    S->n_deleted = (int)ndel;
+  const int lane = threadIdx.x;
+  int c[4], nd = slots[lane].ndel;
+  for (int q = 0; q < 4; q++) c[q] = slots[lane].cnt[q];
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int q = 0; q < 4; q++) c[q] += __shfl_down(c[q], off, 64);
+    nd += __shfl_down(nd, off, 64);
   }
-}
-
-// UpdateNearPrtclsPW (Prtcl_ContactSearchPW.f90:159-235), particle part
-__global__ __launch_bounds__(256) void k_wall_list(Params P, Buf A, Scalars* S, WallPart* wparts) {
-  if (S->halt || S->iter < S->next_wall_update) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double mv = 0.0, ma = 0.0;
-  if (s < S->n_stored) {
-    const double4 p = A.pos[s];
-    uint64_t mask = 0;
-    for (int w = 0; w < P.nwall; w++) {
-      const Wall& W = P.walls[w];
-      double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2, mnz = W.min_point[2] - P.dxmax2;
-      double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2, mxz = W.max_point[2] + P.dxmax2;
-      if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
-        mask |= (1ull << w);
-    }
-    A.wmask[s] = mask;
-    mv = norm3(ld3(A.lv, P.cap, s));
-    ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
-  }
-  __shared__ double sh[4];
-  mv = block_max(mv, sh);
-  ma = block_max(ma, sh);
-  if (threadIdx.x == 0) wparts[blockIdx.x] = WallPart{mv, ma};
-}
-
-// UpdateNearPrtclsPW, cadence part (:201-229); then iterNumber advances
-// (only the two wall kernels read it)
-__global__ __launch_bounds__(256) void k_wall_cadence(Params P, Scalars* S, const WallPart* wparts,
-                                                      int nparts) {
-  if (S->halt) return;
-  const bool update = S->iter >= S->next_wall_update;
-  __syncthreads();
-  if (update) {
-    __shared__ double sh[4];
-    double mv = 0.0, ma = 0.0;
-    if (P.nwall > 0)
-      for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        mv = fmax(mv, wparts[b].max_v);
-        ma = fmax(ma, wparts[b].max_a);
-      }
-    mv = block_max(mv, sh);
-    ma = block_max(ma, sh);
-    if (threadIdx.x == 0) {
-      double max_v = mv + P.max_wall_vel;
-      max_v = max_v > 0.3 ? max_v : 0.3;
-      max_v = max_v * 1.5;
-      double max_a = S->use_host_max_a ? S->host_max_a : ma;
-      double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
-      max_a = max_a > gn ? max_a : gn;
-      max_a = max_a * 1.5;
-      int num_iter;
-      if (max_a > 0.0) {
-        double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
-        double q = t / P.dt;
-        num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
-      } else {
-        num_iter = P.wall_max_update_iter;
-      }
-      if (num_iter < 1) num_iter = 1;
-      if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
-      S->next_wall_update = S->iter + num_iter;
-    }
-  }
-  if (threadIdx.x == 0) {
-    S->use_host_max_a = 0;
-    S->iter = S->iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
+  if (lane == 0) {
+    for (int q = 0; q < 4; q++) S->num_cntcts[q] = c[q];
+    S->n_deleted = nd;
   }
 }
 
 // ---------------------------------------------------------------------------
 // cell sort (rebuild steps only)
 // ---------------------------------------------------------------------------
-__global__ void k_fill_rebuild(int* __restrict__ p, int64_t n, int v, const Scalars* S) {
-  if (!S->rebuild || S->halt) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
-}
-
 // (level, cell) key and per-key counts; dead particles go to the last key.
 // Lanes of a wave holding the same key (neighbours in storage order usually
-// share a cell) take their slots with one atomic per run.
+// share a cell) take their slots with one atomic per run.  The counts array
+// is all zero on entry (zeroed at allocation and by k_scan_add).
 __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_t dead_key,
                                              int* __restrict__ keys, int* __restrict__ rank,
                                              int* __restrict__ counts) {
@@ -418,16 +417,18 @@ __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_
   if (valid) rank[s] = base + (lane - my_head);
 }
 
-// exclusive scan in three passes (block scans, scan of block sums, add),
-// skipped on steps that do not rebuild
+// Exclusive scan of the counts in two kernels: tile scans (the last tile
+// block then scans the tile sums), and the tile offsets added (while the
+// counts are zeroed for the next rebuild).
 constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out,
                                                         int* __restrict__ tile_sum, int64_t n,
-                                                        const Scalars* S) {
+                                                        Scalars* S) {
   if (!S->rebuild || S->halt) return;
   using BS = hipcub::BlockScan<int, SCAN_T>;
   __shared__ typename BS::TempStorage tmp;
+  __shared__ bool last;
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0;
@@ -435,39 +436,47 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ i
   BS(tmp).ExclusiveSum(v, v, total);
   for (int i = 0; i < SCAN_ITEMS; i++)
     if (base + i < n) out[base + i] = v[i];
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(SCAN_T) void k_scan_sums(int* __restrict__ tile_sum, int ntiles,
-                                                       const Scalars* S) {
-  if (!S->rebuild || S->halt) return;
-  using BS = hipcub::BlockScan<int, SCAN_T>;
-  __shared__ typename BS::TempStorage tmp;
+  if (threadIdx.x == 0) {
+    tile_sum[blockIdx.x] = total;
+    __threadfence();
+    last = atomicAdd(&S->ticket_scan, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the last tile block scans the tile sums
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
+  const int ntiles = gridDim.x;
   for (int start = 0; start < ntiles; start += SCAN_TILE) {
-    const int base = start + threadIdx.x * SCAN_ITEMS;
-    int v[SCAN_ITEMS];
-    for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < ntiles) ? tile_sum[base + i] : 0;
-    int total;
-    BS(tmp).ExclusiveSum(v, v, total);
+    const int b0 = start + threadIdx.x * SCAN_ITEMS;
+    int w[SCAN_ITEMS];
+    for (int i = 0; i < SCAN_ITEMS; i++) w[i] = (b0 + i < ntiles) ? tile_sum[b0 + i] : 0;
+    int tot;
+    __syncthreads();
+    BS(tmp).ExclusiveSum(w, w, tot);
     const int c0 = carry;
     for (int i = 0; i < SCAN_ITEMS; i++)
-      if (base + i < ntiles) tile_sum[base + i] = v[i] + c0;
+      if (b0 + i < ntiles) tile_sum[b0 + i] = w[i] + c0;
     __syncthreads();
-    if (threadIdx.x == 0) carry = c0 + total;
+    if (threadIdx.x == 0) carry = c0 + tot;
     __syncthreads();
   }
+  if (threadIdx.x == 0) S->ticket_scan = 0;
 }
 
-__global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ out, const int* __restrict__ tile_sum,
-                                                      int64_t n, const Scalars* S) {
+__global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ in, int* __restrict__ out,
+                                                      const int* __restrict__ tile_sum, int64_t n,
+                                                      const Scalars* S) {
   if (!S->rebuild || S->halt) return;
   const int add = tile_sum[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_T)
-    if (base + i < n) out[base + i] += add;
+    if (base + i < n) {
+      out[base + i] += add;
+      in[base + i] = 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __restrict__ keys,
@@ -506,33 +515,19 @@ __global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_ke
 // Every particle lists, in traversal order, the particles j (any level,
 // periodic images included) with ((dx*dx + dy*dy) + dz*dz) <= (ri+rj+skin)^2
 // over the cells of each level overlapping [x - reach, x + reach], reach =
-// ri + (largest radius of the level) + skin.  Pass 0 counts, pass 1 fills
-// the CSR rows whose offsets the scan of the counts produced.
-template <int PASS>
+// ri + (largest radius of the level) + skin.  Entry q of particle k sits at
+// vlist[q*cap + k] (lanes of a wave read consecutive words).
 __global__ __launch_bounds__(256) void k_vlist(Params P, Scalars* S, const double4* __restrict__ pk,
                                                const int* __restrict__ cstart, int* __restrict__ vcnt,
-                                               const int* __restrict__ voff, int* __restrict__ vlist,
-                                               int n_upper) {
+                                               int* __restrict__ vlist) {
   if (!S->rebuild || S->halt) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > n_upper) return;
-  const int n = S->n_alive;
-  if (k >= n) {
-    if (PASS == 0) vcnt[k] = 0;
-    return;
-  }
+  if (k >= S->n_alive) return;
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
+  const int cap = P.cap, lcap = P.lcap;
   int c = 0;
-  int w0 = 0;
-  if (PASS == 1) {
-    w0 = voff[k];
-    if (voff[k + 1] > P.vcap) {
-      atomicMax(&S->overflow, 2);
-      return;
-    }
-  }
   for (int L = 0; L < P.nlvl; L++) {
     const Grid& G = P.lvl[L];
     const double reach = ri + G.rmax + P.skin;
@@ -577,14 +572,18 @@ __global__ __launch_bounds__(256) void k_vlist(Params P, Scalars* S, const doubl
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
             if (kj == k && !shifted) continue;
-            if (PASS == 1) vlist[w0 + c] = kj;
+            if (c < lcap) vlist[(size_t)c * cap + k] = kj;
             c++;
           }
         }
       }
     }
   }
-  if (PASS == 0) vcnt[k] = c;
+  vcnt[k] = c;
+  if (c > lcap) {
+    atomicMax(&S->overflow, 2);
+    atomicMax(&S->list_need, c);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -595,15 +594,15 @@ struct Hist {
   double vri;
 };
 
-__device__ __forceinline__ Hist find_hist(const Buf& A, int hcap, int owner_s, int partner) {
+__device__ __forceinline__ Hist find_hist(const Buf& A, int cap, int owner_s, int partner) {
   Hist h;
   h.td = D3(0.0, 0.0, 0.0);
   h.vri = -2.0;
   int c = A.hcnt[owner_s];
-  int base = owner_s * hcap;
   for (int k = 0; k < c; k++) {
-    if ((A.hkey[base + k] & KEY_MASK) == partner) {
-      double4 v = A.hval[base + k];
+    const size_t slot = (size_t)k * cap + owner_s;
+    if ((A.hkey[slot] & KEY_MASK) == partner) {
+      double4 v = A.hval[slot];
       h.td = D3(v.x, v.y, v.z);
       h.vri = v.w;
       break;
@@ -622,7 +621,7 @@ struct Acc {
 __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, Acc& acc, int kind,
                                           int partner, const CF& cf) {
   if (acc.hc < P.hcap) {
-    int slot = k * P.hcap + acc.hc;
+    const size_t slot = (size_t)acc.hc * P.cap + k;
     B.hkey[slot] = partner | (kind << KEY_BITS);
     B.hval[slot] = make_double4(cf.tandel.x, cf.tandel.y, cf.tandel.z, cf.vri);
     acc.hc++;
@@ -651,7 +650,7 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // kernel (few registers) so that many waves hide the gather latency.
 __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
                                                  const int* __restrict__ perm,
-                                                 const int* __restrict__ voff,
+                                                 const int* __restrict__ vcnt,
                                                  const int* __restrict__ vlist,
                                                  unsigned* __restrict__ clist,
                                                  int* __restrict__ ccnt) {
@@ -661,11 +660,11 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
   const bool rb = S->rebuild != 0;
   const int s = rb ? perm[k] : k;
   const double4 pi = A.pos[s];
-  const int qa = voff[k], qb = min(voff[k + 1], P.vcap);
-  unsigned* out = clist + (size_t)k * P.hcap;
+  const int nq = min(vcnt[k], P.lcap);
+  const size_t cap = P.cap;
   int c = 0;
-  for (int q = qa; q < qb; q++) {
-    const int kj = vlist[q];
+  for (int q = 0; q < nq; q++) {
+    const int kj = vlist[(size_t)q * cap + k];
     const int sj = rb ? perm[kj] : kj;
     const double4 pj = A.pos[sj];
     int shifted = 0;
@@ -683,7 +682,7 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
     const double ddz = pi.z - zj;
     d2 = ddz * ddz + d2;
     if (d2 > dr2) continue;
-    if (c < P.hcap) out[c] = (unsigned)sj | (shifted ? 0x80000000u : 0u);
+    if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)sj | (shifted ? 0x80000000u : 0u);
     c++;
   }
   ccnt[k] = c;
@@ -694,14 +693,16 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
   // canonical order: ascending partner id, so the sums of a particle's
   // contact forces do not depend on the grid, the skin or the storage order
   for (int a = 1; a < c; a++) {
-    const unsigned e = out[a];
+    const unsigned e = clist[(size_t)a * cap + k];
     const int ida = A.id[e & 0x7fffffffu];
     int b = a - 1;
-    while (b >= 0 && A.id[out[b] & 0x7fffffffu] > ida) {
-      out[b + 1] = out[b];
+    while (b >= 0) {
+      const unsigned eb = clist[(size_t)b * cap + k];
+      if (A.id[eb & 0x7fffffffu] <= ida) break;
+      clist[(size_t)(b + 1) * cap + k] = eb;
       b--;
     }
-    out[b + 1] = e;
+    clist[(size_t)(b + 1) * cap + k] = e;
   }
 }
 
@@ -714,7 +715,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
                                                const int* __restrict__ fstart,
-                                               BlockPart* __restrict__ parts) {
+                                               Slot* __restrict__ slots) {
   // a candidate-list overflow in this step's rebuild leaves voff pointing
   // past the list storage: do nothing, the host regrows and replays
   if (S->halt || S->overflow == 2) return;
@@ -750,9 +751,8 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
 
     // ---------------- moving particles touching it (k_contact) ------------
     const int nc = min(ccnt[k], P.hcap);
-    const unsigned* lst = clist + (size_t)k * P.hcap;
     for (int q = 0; q < nc; q++) {
-      const unsigned e = lst[q];
+      const unsigned e = clist[(size_t)q * cap + k];
       const int sj = (int)(e & 0x7fffffffu);
       const double4 pj = A.pos[sj];
       int shifted = 0;
@@ -774,7 +774,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
         // PP: record in the lower id's bucket, force from its side
         // (FineSearch2 :525-529, ContactForce_PP :732-756)
         const bool ilo = idi < idj;
-        const Hist h = ilo ? find_hist(A, P.hcap, s, idj) : find_hist(A, P.hcap, sj, idi);
+        const Hist h = ilo ? find_hist(A, P.cap, s, idj) : find_hist(A, P.cap, sj, idi);
         const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
         const Bnry& Bp = P.pp[(tlo - 1) * P.ntype + (thi - 1)];
         CF cf;
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       } else {
         // PPG: each side keeps its own record against the other's image
         // (ContactForce_PPG :761-798); row type = the local particle
-        const Hist h = find_hist(A, P.hcap, s, idj);
+        const Hist h = find_hist(A, P.cap, s, idj);
         const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (tj - 1)];
         CF cf;
         if (idi < idj) {
@@ -853,7 +853,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
               if (d2 > dr2) continue;
               const double ovrlp = dr - sqrt(d2);
               const int idj = fid[kj];
-              const Hist h = find_hist(A, P.hcap, s, idj);
+              const Hist h = find_hist(A, P.cap, s, idj);
               const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (ftype[kj] - 1)];
               const d3 z3 = D3(0.0, 0.0, 0.0);
               CF cf = contact_force(P, Bp, nv3(D3(xj, yj, zj), xi), ri, pj.w, vi, z3, wi, z3,
@@ -878,7 +878,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       d3 nv;
       if (pw_in_contact(W, xi, ri, &ovrlp, &nv)) {
         // ContactForce_PW (Prtcl_CL_and_CF.f90:832-856)
-        const Hist h = find_hist(A, P.hcap, s, W.wall_id);
+        const Hist h = find_hist(A, P.cap, s, W.wall_id);
         const Bnry& Bp = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
         const d3 norm_v = mul3(nv, -1.0);
         CF cf = contact_force(P, Bp, norm_v, ri, 1.00e20 * ri, vi, wp(W.trans_vel), wi,
@@ -990,20 +990,19 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
       disp2 += dd * dd;
     }
   }
-  // block partials for the next step's k_begin (no same-address atomics)
+  // block reductions, then one atomic per block on slot (block % NSLOT)
   __shared__ double shd[4];
-  __shared__ long long shl[4];
+  __shared__ int shi[4];
   const double bdisp = block_max(disp2, shd);
-  long long bc[4];
-  for (int q = 0; q < 4; q++) bc[q] = block_sum(acc.nkind[q], shl);
-  const long long bdel = block_sum(dead ? 1 : 0, shl);
+  int bc[4];
+  for (int q = 0; q < 4; q++) bc[q] = block_sum(acc.nkind[q], shi);
+  const int bdel = block_sum(dead ? 1 : 0, shi);
   if (threadIdx.x == 0) {
-    BlockPart bp;
-    bp.disp = sqrt(bdisp);
-    for (int q = 0; q < 4; q++) bp.cnt[q] = (int)bc[q];
-    bp.ndel = (int)bdel;
-    bp.pad = 0;
-    parts[blockIdx.x] = bp;
+    Slot& sl = slots[blockIdx.x % NSLOT];
+    if (bdisp > 0.0) atomicMax(&sl.disp_bits, dbits(sqrt(bdisp)));
+    for (int q = 0; q < 4; q++)
+      if (bc[q]) atomicAdd(&sl.cnt[q], bc[q]);
+    if (bdel) atomicAdd(&sl.ndel, bdel);
     if (blockIdx.x == 0) S->have_parts = 1;
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
@@ -1014,45 +1013,29 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-static void scan(const int* in, int* out, int* tile_sum, int64_t n, const Scalars* S, hipStream_t st) {
-  const unsigned nt = nblk(n, SCAN_TILE);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, in, out, tile_sum, n, S);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, st, tile_sum, (int)nt, S);
-  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, out, tile_sum, n, S);
-}
-
 int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_begin(const Params& P, Scalars* S, const BlockPart* parts, int n_upper,
-                        hipStream_t st) {
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(256), 0, st, P, S, parts, (int)nblk(n_upper, 256));
+hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_pre, dim3(nblk(n_upper > 0 ? n_upper : 1, 256)), dim3(256), 0, st, P, A, S, slots);
   return hipGetLastError();
 }
 
-hipError_t launch_finish(Scalars* S, const BlockPart* parts, int n_upper, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, S, parts, (int)nblk(n_upper, 256));
-  return hipGetLastError();
-}
-
-hipError_t launch_wall_list(const Params& P, const Buf& A, Scalars* S, WallPart* wparts, int n_upper,
-                            hipStream_t st) {
-  if (P.nwall > 0 && n_upper > 0)
-    hipLaunchKernelGGL(k_wall_list, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, wparts);
-  hipLaunchKernelGGL(k_wall_cadence, dim3(1), dim3(256), 0, st, P, S, wparts, (int)nblk(n_upper, 256));
+hipError_t launch_finish(Scalars* S, const Slot* slots, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, S, slots);
   return hipGetLastError();
 }
 
 hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
                        int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
                        int* perm, double4* pk, hipStream_t st) {
-  int64_t nfill = dead_key + 2;
-  unsigned fb = nblk(nfill, 256);
-  if (fb > 65536) fb = 65536;
-  hipLaunchKernelGGL(k_fill_rebuild, dim3(fb), dim3(256), 0, st, counts, nfill, 0, S);
+  const int64_t n = dead_key + 2;
+  const unsigned nt = nblk(n, SCAN_TILE);
   if (n_upper > 0)
     hipLaunchKernelGGL(k_bin, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, dead_key, keys,
                        rank, counts);
-  scan(counts, start, tile_sum, nfill, S, st);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, counts, start, tile_sum, n, S);
+  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, counts, start, tile_sum, n, S);
   if (n_upper > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n_upper, 256)), dim3(256), 0, st, S, keys, rank, start,
                        perm_tmp);
@@ -1063,27 +1046,23 @@ hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, i
 }
 
 hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
-                        const int* cstart, int* vcnt, int* voff, int* vlist, int* tile_sum,
-                        hipStream_t st) {
-  if (n_upper <= 0) return hipSuccess;
-  const unsigned nb = nblk((int64_t)n_upper + 1, 256);
-  hipLaunchKernelGGL(k_vlist<0>, dim3(nb), dim3(256), 0, st, P, S, pk, cstart, vcnt, voff, vlist, n_upper);
-  scan(vcnt, voff, tile_sum, (int64_t)n_upper + 1, S, st);
-  hipLaunchKernelGGL(k_vlist<1>, dim3(nb), dim3(256), 0, st, P, S, pk, cstart, vcnt, voff, vlist, n_upper);
+                        const int* cstart, int* vcnt, int* vlist, hipStream_t st) {
+  if (n_upper > 0)
+    hipLaunchKernelGGL(k_vlist, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, S, pk, cstart, vcnt, vlist);
   return hipGetLastError();
 }
 
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* voff, const int* vlist, unsigned* clist,
+                        const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
                         int* ccnt, double4* ref, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, BlockPart* parts, hipStream_t st,
+                        const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
                         hipEvent_t mid) {
   if (n_upper > 0) {
-    hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, voff,
+    hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, vcnt,
                        vlist, clist, ccnt);
     if (mid) (void)hipEventRecord(mid, st);
     hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, clist,
-                       ccnt, ref, fpk, fid, ftype, fstart, parts);
+                       ccnt, ref, fpk, fid, ftype, fstart, slots);
   }
   return hipGetLastError();
 }

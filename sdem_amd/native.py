@@ -29,7 +29,7 @@ CFT_LSD, CFT_NLIN = 1, 2
 CTM_CONSTANT_TORQUE, CTM_VARIABLE_TORQUE = 1, 2
 PIM_FE, PIM_AB2, PIM_AB3 = 1, 2, 3
 KIND_PP, KIND_PPG, KIND_PPFIX, KIND_PW = 1, 2, 3, 4
-PHASE_NAMES = ("wall_list", "bin_sort", "search", "contact", "force")
+PHASE_NAMES = ("pre", "bin_sort", "search", "contact", "force")
 
 
 class SdemError(RuntimeError):
@@ -111,7 +111,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "get_profile": ([V, _DP, _LP], C.c_int),
         "get_device_bytes": ([V, _LP], C.c_int),
         "set_tuning": ([V, _DP, C.c_double, C.c_int32], C.c_int),
-        "set_capacity": ([V, C.c_int32, C.c_int64], C.c_int),
+        "set_capacity": ([V, C.c_int32, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = f(name)
@@ -288,8 +288,8 @@ class Engine:
         self._check(self._fn("set_tuning")(self.h, _dptr(cf), float(skin), int(keep_forces)),
                     "set_tuning")
 
-    def set_capacity(self, hist_slots=0, list_entries=0) -> None:
-        self._check(self._fn("set_capacity")(self.h, int(hist_slots), int(list_entries)), "set_capacity")
+    def set_capacity(self, hist_slots=0, list_slots=0) -> None:
+        self._check(self._fn("set_capacity")(self.h, int(hist_slots), int(list_slots)), "set_capacity")
 
     def get_profile(self):
         ms = np.zeros(len(PHASE_NAMES))

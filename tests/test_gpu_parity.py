@@ -72,9 +72,9 @@ def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
     """Tiny history and list capacities force overflows; the library grows
     them and replays the failed step from its intact input, so the result is
     the oracle's."""
-    print("capacity replay: hist_slots=1, list_entries=64", flush=True)
+    print("capacity replay: hist_slots=1, list_slots=2", flush=True)
     sc = scenarios.random_cloud(n=300, seed=4)
-    gpu = native.build_engine(hip_lib, sc, capacity=dict(hist_slots=1, list_entries=64))
+    gpu = native.build_engine(hip_lib, sc, capacity=dict(hist_slots=1, list_slots=2))
     ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
     gpu.iterate(12)
     ref.iterate(12)
