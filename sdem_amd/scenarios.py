@@ -179,9 +179,9 @@ def random_cloud(n=300, seed=DEFAULT_SEED, box=None, diameters=(1.0e-3, 1.5e-3, 
         st["rot_acc"][:, 0, :] = st["rot_acc"][:, 1, :]
     st["theta"][:] = rng.random((n, 3))
     wl = box_walls((0, 0, 0), box, infinite=infinite_walls, faces=walls) if walls else []
+    okw.setdefault("dt", 2.0e-6)
     opts = base_options(domain_max=tuple(box), is_periodic=tuple(periodic), num_prtcl=n, cf_type=cf_type,
-                        ct_model=ct_model, pi_method=pi, pri_method=pri, cs_method=cs_method,
-                        dt=2.0e-6, **okw)
+                        ct_model=ct_model, pi_method=pi, pri_method=pri, cs_method=cs_method, **okw)
     sc = dict(options=opts,
               ptypes=dict(diameter=list(diam), density=[2500.0] * len(diam),
                           youngs=[5.0e7] * len(diam), poisson=[0.3] * len(diam)),

@@ -20,6 +20,8 @@ CASES = {
     "fixed_particles": dict(nfixed=25),
     "hierarchical": dict(diameters=(0.6e-3, 1.2e-3, 3.0e-3), cs_method=native.CSM_NBS_MUNJIZA_HRCHL,
                          nfixed=10),
+    # no walls, fast particles: the near-wall cadence still follows max|v|
+    "no_walls_fast": dict(periodic=(1, 1, 1), walls="", vel=3.0, dt=1.0e-5),
 }
 
 
