@@ -891,13 +891,10 @@ static int grow_history(sdem_ctx* c) {
 }
 
 // Candidate-list slots per particle sized to what the failed rebuild counted.
-static int grow_vlist(sdem_ctx* c) {
-  Scalars s;
-  hipError_t e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
-  int neu = std::max(s.list_need + s.list_need / 4 + 8, 2 * c->lcap);
+static int grow_vlist(sdem_ctx* c, int list_need) {
+  int neu = std::max(list_need + list_need / 4 + 8, 2 * c->lcap);
   if ((int64_t)neu * c->cap > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
-  e = regrow(c, &c->vlist, (size_t)c->cap * neu);
+  hipError_t e = regrow(c, &c->vlist, (size_t)c->cap * neu);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
   c->lcap = neu;
   c->P.lcap = neu;
@@ -1004,14 +1001,13 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     // is intact; grow the capacity and replay from there
     if (++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");
     c->cur = (c0 + ok_steps) % 2;
-    const int kind = s.overflow;
+    const int kind = s.overflow, list_need = s.list_need;
     s.iter = s.iter_at_begin;
     s.next_wall_update = s.next_wall_at_begin;
     s.n_alive = s.n_stored;
     s.need_rebuild = 1;
     s.have_parts = 0;
     s.n_deleted = 0;
-    s.ticket = 0;
     s.ticket_scan = 0;
     s.list_need = 0;
     s.overflow = 0;
@@ -1019,8 +1015,11 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.halted_steps = 0;
     if (c->first_step_pending) s.use_host_max_a = 1;
     e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
-    int rc = kind == 2 ? grow_vlist(c) : grow_history(c);
+    // the slots hold what the failing and the halted steps folded in; the
+    // replayed step rebuilds its lists, so only its own maxima are needed
+    if (e == hipSuccess) e = hipMemset(c->slots, 0, NSLOT * sizeof(Slot));
+    if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
+    int rc = kind == 2 ? grow_vlist(c, list_need) : grow_history(c);
     if (rc) return rc;
     remaining -= ok_steps;
   }

@@ -125,11 +125,9 @@ struct Scalars {
   int need_rebuild;        // forced rebuild at the next step
   int rebuilds;            // rebuilds so far (statistics)
   int have_parts;          // the force kernel's slots are current
-  unsigned ticket;         // last-block election in k_pre
   unsigned ticket_scan;    // last-block election in k_scan_tiles
   int list_need;           // largest candidate count seen when a list overflowed
   int num_cntcts[4];
-  unsigned long long max_v_bits, max_a_bits;  // unused, kept for layout
   int use_host_max_a;      // first step after set_state: max|a| from host copy
   double host_max_a;
 };

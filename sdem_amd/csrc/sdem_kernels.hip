@@ -6,8 +6,9 @@
 // decided every step with the reference's exact predicate, so it does not
 // depend on when lists are rebuilt.
 //
-//   k_pre                   : near-wall bit masks (update iterations); its
-//                             last block decides the rebuild, runs the
+//   k_pre                   : near-wall bit masks, max|v|, max|a| (update
+//                             iterations)
+//   k_ctl                   : one wave: decides the rebuild, runs the
 //                             near-wall cadence and advances iterNumber
 //   -- rebuild steps only (the kernels return at once otherwise) --
 //   k_bin                   : (level, cell) key per particle, per-key counts
@@ -253,67 +254,65 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
 }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// One kernel per step for the control work:
-//  * on update iterations every block builds the near-wall bit masks of its
-//    particles and folds max|v|, max|a| into the slots (UpdateNearPrtclsPW,
-//    Prtcl_ContactSearchPW.f90:159-235);
-//  * the last block to finish reduces the previous step's force slots,
-//    decides whether the cell sort and the candidate lists are rebuilt,
-//    computes the near-wall cadence (:201-229) and advances iterNumber.
-// After a capacity overflow the last block only marks the call halted, so
-// every later kernel of the call does nothing and the failing step's input
-// buffer stays intact for the replay.
+// Near-wall list, every block (UpdateNearPrtclsPW,
+// Prtcl_ContactSearchPW.f90:159-235): on update iterations the near-wall bit
+// mask of each particle is rebuilt and max|v|, max|a| are folded into the
+// slots (one atomic per block, spread over NSLOT addresses).  After a
+// capacity overflow the step is halted: block 0 counts it and nothing runs.
 __global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* slots) {
   __shared__ double shd[4];
-  __shared__ bool last;
-  if (S->halt) return;
-  const int iter = S->iter, next = S->next_wall_update;
-  const bool update = iter >= next;  // max|v|, max|a| are taken even without walls
+  if (S->halt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->halted_steps++;
+    return;
+  }
+  if (S->iter < S->next_wall_update) return;  // max|v|, max|a| are taken even without walls
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (update) {
-    double mv = 0.0, ma = 0.0;
-    if (s < S->n_alive) {
-      const double4 p = A.pos[s];
-      uint64_t mask = 0;
-      for (int w = 0; w < P.nwall; w++) {
-        const Wall& W = P.walls[w];
-        const double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2;
-        const double mnz = W.min_point[2] - P.dxmax2;
-        const double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2;
-        const double mxz = W.max_point[2] + P.dxmax2;
-        if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
-          mask |= (1ull << w);
-      }
-      A.wmask[s] = mask;
-      mv = norm3(ld3(A.lv, P.cap, s));
-      ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
+  double mv = 0.0, ma = 0.0;
+  if (s < S->n_alive) {
+    const double4 p = A.pos[s];
+    uint64_t mask = 0;
+    for (int w = 0; w < P.nwall; w++) {
+      const Wall& W = P.walls[w];
+      const double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2;
+      const double mnz = W.min_point[2] - P.dxmax2;
+      const double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2;
+      const double mxz = W.max_point[2] + P.dxmax2;
+      if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
+        mask |= (1ull << w);
     }
-    mv = block_max(mv, shd);
-    ma = block_max(ma, shd);
-    if (threadIdx.x == 0) {
-      Slot& sl = slots[blockIdx.x % NSLOT];
-      atomicMax(&sl.max_v_bits, dbits(mv));
-      atomicMax(&sl.max_a_bits, dbits(ma));
-    }
+    A.wmask[s] = mask;
+    mv = norm3(ld3(A.lv, P.cap, s));
+    ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
   }
-  // last-block election
-  __syncthreads();
+  mv = block_max(mv, shd);
+  ma = block_max(ma, shd);
   if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(&S->ticket, 1u) == gridDim.x - 1;
+    Slot& sl = slots[blockIdx.x % NSLOT];
+    atomicMax(&sl.max_v_bits, dbits(mv));
+    atomicMax(&sl.max_a_bits, dbits(ma));
   }
-  __syncthreads();
-  if (!last || threadIdx.x >= 64) return;
-  __threadfence();
-  const int lane = threadIdx.x;  // one wave does the control work
+}
+
+// Step control, one wave: reduces the slots (the previous step's
+// displacement and deletions, this step's max|v| and max|a|), decides
+// whether the cell sort and the candidate lists are rebuilt, computes the
+// near-wall cadence (Prtcl_ContactSearchPW.f90:201-229) and advances
+// iterNumber.  After a capacity overflow it only marks the call halted, so
+// every later kernel of the call does nothing and the failing step's input
+// buffer stays intact for the replay; the halted steps are counted (this
+// one here, the later ones in k_pre) so that the host knows how many steps
+// completed.
+__global__ __launch_bounds__(64) void k_ctl(Params P, Scalars* S, Slot* slots) {
+  if (S->halt) return;
+  const int lane = threadIdx.x;
   if (S->overflow) {
     if (lane == 0) {
       S->halt = 1;
       S->halted_steps++;
-      S->ticket = 0;
     }
     return;
   }
+  const int iter = S->iter, next = S->next_wall_update;
   Slot& sl = slots[lane];
   double disp = bitsd(sl.disp_bits), mv = bitsd(sl.max_v_bits), ma = bitsd(sl.max_a_bits);
   int ndel = sl.ndel;
@@ -359,7 +358,6 @@ __global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* 
   }
   S->use_host_max_a = 0;
   S->iter = iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
-  S->ticket = 0;
 }
 
 // counters of the last step for the host (once per call)
@@ -716,8 +714,8 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const int* __restrict__ ftype,
                                                const int* __restrict__ fstart,
                                                Slot* __restrict__ slots) {
-  // a candidate-list overflow in this step's rebuild leaves voff pointing
-  // past the list storage: do nothing, the host regrows and replays
+  // a candidate-list overflow in this step's rebuild leaves the lists
+  // incomplete: do nothing, the host regrows them and replays the step
   if (S->halt || S->overflow == 2) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = S->n_alive;
@@ -1018,6 +1016,7 @@ int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
                       hipStream_t st) {
   hipLaunchKernelGGL(k_pre, dim3(nblk(n_upper > 0 ? n_upper : 1, 256)), dim3(256), 0, st, P, A, S, slots);
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, P, S, slots);
   return hipGetLastError();
 }
 
