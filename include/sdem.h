@@ -60,7 +60,7 @@ extern "C" {
 #define SDEM_KIND_PW     4
 
 /* profiling phases reported by sdem_get_profile */
-#define SDEM_PHASE_WALL_LIST  0   /* UpdateNearPrtclsPW                         */
+#define SDEM_PHASE_CONTROL    0   /* rebuild decision, near-wall cadence        */
 #define SDEM_PHASE_BIN        1   /* cell keys + counting sort (rebuild steps)  */
 #define SDEM_PHASE_SEARCH     2   /* candidate lists (rebuild steps)            */
 #define SDEM_PHASE_CONTACT    3   /* contact test over the candidate lists      */
@@ -194,7 +194,8 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
  * Prtcl_cs_ratio * d_max) in dimension d, 0 keeps the current value.
  * skin_factor (fraction of d_max, below 0.45; negative keeps the current
  * value, default 0.1) is the margin of the candidate lists, which are rebuilt
- * with the cell sort only after some particle has moved half the skin; the
+ * with the cell sort only once the sum over steps of the largest move of a
+ * particle passes half the skin (or a particle was removed); the
  * contact set is decided every step with the exact predicate, so it is the
  * same for any factors.  keep_forces = 0 stops writing the per-particle
  * contact force / torque arrays that sdem_get_forces reads (the forces are

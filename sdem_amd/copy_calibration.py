@@ -1,7 +1,11 @@
-"""A plain device-to-device copy of known size, used to calibrate the HBM
-byte counters (FETCH_SIZE, WRITE_SIZE) that rocprofv3 reports for the DEM
-kernels: run it under the same `rocprofv3 --pmc` command and divide the
-counter by the bytes moved.  Uses torch only for the allocation and copy.
+"""A plain device-to-device copy of known size.
+
+It calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) that rocprofv3
+reports for the DEM kernels: run it under the same `rocprofv3 --pmc` command
+and divide the counter by the bytes moved.  It also prints the copy's own
+bandwidth (bytes read + written per second), the achievable HBM rate that
+the kernels' figures are compared with.  Uses torch only for the allocation
+and the copy.
 
     python3 -m sdem_amd.copy_calibration [MiB]
 """
@@ -17,7 +21,16 @@ def main(mib: int = 512) -> None:
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(f"copied {mib} MiB five times: {n * 8} bytes read and written per copy", flush=True)
+    reps = 20
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    sec = t0.elapsed_time(t1) / 1e3 / reps
+    print(f"copied {mib} MiB {5 + reps} times: {n * 8} bytes read and written per copy; "
+          f"{2 * n * 8 / sec / 1e9:.1f} GB/s (read + written)", flush=True)
 
 
 if __name__ == "__main__":

@@ -22,6 +22,7 @@ namespace sdem {
 hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
                       hipStream_t st);
 hipError_t launch_finish(Scalars* S, const Slot* slots, hipStream_t st);
+hipError_t launch_ctl(const Params& P, Scalars* S, Slot* slots, hipStream_t st);
 hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
                        int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
                        int* perm, double4* pk, hipStream_t st);
@@ -29,7 +30,7 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
                         const int* cstart, int* vcnt, int* vlist, hipStream_t st);
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
                         const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
-                        int* ccnt, double4* ref, const double4* fpk, const int* fid,
+                        int* ccnt, const double4* fpk, const int* fid,
                         const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
                         hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
@@ -209,12 +210,12 @@ struct sdem_ctx {
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
   Slot* slots = nullptr;
-  double4* ref = nullptr;
   int lcap = 0;
   double cell_factor[3] = {1.0, 1.0, 1.0};
   double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
   int keep_forces = 1;
   bool first_step_pending = false;
+  bool need_pre = false;  // the state in buffer A has no near-wall pass yet
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
   int hist_slots_hint = 0;
@@ -589,7 +590,6 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->pk, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vcnt, (size_t)c->cap + 2);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ref, c->cap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
     if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
     c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
@@ -670,6 +670,7 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
   c->stepped = false;
   c->prepared = false;
   c->first_step_pending = true;
+  c->need_pre = true;
   // plan the grids now so configuration errors surface here
   free_list(c, c->allocs_grid);
   int64_t base = 0;
@@ -908,7 +909,7 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
   Buf& A = c->buf[c->cur];
   Buf& B = c->buf[1 - c->cur];
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipError_t e = launch_pre(c->P, A, c->S, c->slots, c->n_upper, c->stream);
+  hipError_t e = launch_ctl(c->P, c->S, c->slots, c->stream);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (e == hipSuccess)
     e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
@@ -918,7 +919,7 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[3], c->stream);
   if (e == hipSuccess)
     e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->vcnt, c->vlist, c->clist, c->ccnt,
-                     c->ref, c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->slots, c->stream,
+                     c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->slots, c->stream,
                      ev ? ev[4] : nullptr);
   if (ev) (void)hipEventRecord(ev[5], c->stream);
   c->cur = 1 - c->cur;
@@ -959,6 +960,11 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
     const int c0 = c->cur;
     hipError_t e = hipSuccess;
+    if (c->need_pre) {
+      // near-wall pass of a state that no step has produced
+      e = launch_pre(c->P, c->buf[c->cur], c->S, c->slots, c->n_upper, c->stream);
+      c->need_pre = false;
+    }
     int it = 0;
     if (!c->prof && c->use_graph) {
       for (; it + 2 <= remaining && e == hipSuccess; it += 2) {
@@ -1014,6 +1020,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.halt = 0;
     s.halted_steps = 0;
     if (c->first_step_pending) s.use_host_max_a = 1;
+    c->need_pre = true;
     e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
     // the slots hold what the failing and the halted steps folded in; the
     // replayed step rebuilds its lists, so only its own maxima are needed

@@ -101,7 +101,7 @@ struct Params {
 // control block and by k_finish.
 constexpr int NSLOT = 64;
 struct Slot {
-  unsigned long long disp_bits;  // largest displacement since the last list rebuild
+  unsigned long long step_bits;  // largest distance a particle moved in the step
   unsigned long long max_v_bits; // near-wall list: max |v|, max |a| (as bits of
   unsigned long long max_a_bits; // non-negative doubles)
   int cnt[4];                    // contacts by kind
@@ -130,6 +130,8 @@ struct Scalars {
   int num_cntcts[4];
   int use_host_max_a;      // first step after set_state: max|a| from host copy
   double host_max_a;
+  double disp_acc;         // sum over steps since the last rebuild of the largest
+                           // move of a particle: bounds every displacement
 };
 
 }  // namespace sdem

@@ -1,13 +1,15 @@
 // sdem_kernels.hip -- gfx950 kernels of one DEM time step.
 //
 // A step is a short pipeline on one stream.  Candidate (Verlet) lists are
-// rebuilt, together with the cell sort, only when a particle has moved more
-// than half the skin since the last rebuild; the contact set itself is
-// decided every step with the reference's exact predicate, so it does not
-// depend on when lists are rebuilt.
+// rebuilt, together with the cell sort, only when the particles may have
+// moved half the skin since the last rebuild (the sum over steps of the
+// largest move of a particle passes half the skin); the contact set itself
+// is decided every step with the reference's exact predicate, so it does
+// not depend on when lists are rebuilt.
 //
-//   k_pre                   : near-wall bit masks, max|v|, max|a| (update
-//                             iterations)
+//   k_pre                   : near-wall bit masks, max|v|, max|a| of the
+//                             state as set or replayed (first step of a run
+//                             only; later steps get them from k_force)
 //   k_ctl                   : one wave: decides the rebuild, runs the
 //                             near-wall cadence and advances iterNumber
 //   -- rebuild steps only (the kernels return at once otherwise) --
@@ -20,8 +22,8 @@
 //   k_contact               : exact contact test over the candidate lists,
 //                             partners sorted by id
 //   k_force                 : history match, contact forces, accelerations,
-//                             integration, wrap/delete, and the displacement
-//                             since the last rebuild
+//                             integration, wrap/delete, the largest move of
+//                             the step, and the next step's near-wall masks
 //
 // Floating-point kernels are built with -ffp-contract=off so that every
 // product and sum is rounded on its own, in the order the reference's
@@ -254,33 +256,33 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
 }
 __device__ __forceinline__ double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// Near-wall list, every block (UpdateNearPrtclsPW,
-// Prtcl_ContactSearchPW.f90:159-235): on update iterations the near-wall bit
-// mask of each particle is rebuilt and max|v|, max|a| are folded into the
-// slots (one atomic per block, spread over NSLOT addresses).  After a
-// capacity overflow the step is halted: block 0 counts it and nothing runs.
+// Near-wall list (UpdateNearPrtclsPW, Prtcl_ContactSearchPW.f90:159-235)
+// of a state that no step has produced yet (after sdem_set_state or a
+// replay): on an update iteration the near-wall bit mask of each particle is
+// rebuilt and max|v|, max|a| are folded into the slots (one atomic per
+// block, spread over NSLOT addresses).  Every later step gets the same from
+// the k_force of the step before, which holds the new state in registers.
+__device__ __forceinline__ uint64_t near_wall_mask(const Params& P, double x, double y, double z) {
+  uint64_t mask = 0;
+  for (int w = 0; w < P.nwall; w++) {
+    const Wall& W = P.walls[w];
+    const double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2;
+    const double mnz = W.min_point[2] - P.dxmax2;
+    const double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2;
+    const double mxz = W.max_point[2] + P.dxmax2;
+    if (x >= mnx && x <= mxx && y >= mny && y <= mxy && z >= mnz && z <= mxz) mask |= (1ull << w);
+  }
+  return mask;
+}
+
 __global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* slots) {
   __shared__ double shd[4];
-  if (S->halt) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) S->halted_steps++;
-    return;
-  }
-  if (S->iter < S->next_wall_update) return;  // max|v|, max|a| are taken even without walls
+  if (S->halt || S->iter < S->next_wall_update) return;  // max|v|, max|a| are taken even without walls
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mv = 0.0, ma = 0.0;
-  if (s < S->n_alive) {
+  if (s < S->n_alive && A.type[s] >= 1) {
     const double4 p = A.pos[s];
-    uint64_t mask = 0;
-    for (int w = 0; w < P.nwall; w++) {
-      const Wall& W = P.walls[w];
-      const double mnx = W.min_point[0] - P.dxmax2, mny = W.min_point[1] - P.dxmax2;
-      const double mnz = W.min_point[2] - P.dxmax2;
-      const double mxx = W.max_point[0] + P.dxmax2, mxy = W.max_point[1] + P.dxmax2;
-      const double mxz = W.max_point[2] + P.dxmax2;
-      if (p.x >= mnx && p.x <= mxx && p.y >= mny && p.y <= mxy && p.z >= mnz && p.z <= mxz)
-        mask |= (1ull << w);
-    }
-    A.wmask[s] = mask;
+    A.wmask[s] = near_wall_mask(P, p.x, p.y, p.z);
     mv = norm3(ld3(A.lv, P.cap, s));
     ma = norm3(ld3(A.la, P.cap, s));  // stored level 0 == linAcc(1) at step start
   }
@@ -293,19 +295,17 @@ __global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* 
   }
 }
 
-// Step control, one wave: reduces the slots (the previous step's
-// displacement and deletions, this step's max|v| and max|a|), decides
-// whether the cell sort and the candidate lists are rebuilt, computes the
-// near-wall cadence (Prtcl_ContactSearchPW.f90:201-229) and advances
-// iterNumber.  After a capacity overflow it only marks the call halted, so
-// every later kernel of the call does nothing and the failing step's input
-// buffer stays intact for the replay; the halted steps are counted (this
-// one here, the later ones in k_pre) so that the host knows how many steps
-// completed.
+// Step control, one wave: reduces the slots (the previous step's largest
+// move and deletions, this step's max|v| and max|a|), decides whether the
+// cell sort and the candidate lists are rebuilt, computes the near-wall
+// cadence (Prtcl_ContactSearchPW.f90:201-229) and advances iterNumber.
+// After a capacity overflow it only marks the call halted, so every later
+// kernel of the call does nothing and the failing step's input buffer stays
+// intact for the replay; every halted step is counted so that the host
+// knows how many steps completed.
 __global__ __launch_bounds__(64) void k_ctl(Params P, Scalars* S, Slot* slots) {
-  if (S->halt) return;
   const int lane = threadIdx.x;
-  if (S->overflow) {
+  if (S->halt || S->overflow) {
     if (lane == 0) {
       S->halt = 1;
       S->halted_steps++;
@@ -314,22 +314,25 @@ __global__ __launch_bounds__(64) void k_ctl(Params P, Scalars* S, Slot* slots) {
   }
   const int iter = S->iter, next = S->next_wall_update;
   Slot& sl = slots[lane];
-  double disp = bitsd(sl.disp_bits), mv = bitsd(sl.max_v_bits), ma = bitsd(sl.max_a_bits);
+  double step = bitsd(sl.step_bits), mv = bitsd(sl.max_v_bits), ma = bitsd(sl.max_a_bits);
   int ndel = sl.ndel;
   for (int off = 32; off > 0; off >>= 1) {
-    disp = fmax(disp, __shfl_down(disp, off, 64));
+    step = fmax(step, __shfl_down(step, off, 64));
     mv = fmax(mv, __shfl_down(mv, off, 64));
     ma = fmax(ma, __shfl_down(ma, off, 64));
     ndel += __shfl_down(ndel, off, 64);
   }
-  sl.disp_bits = 0ull;
+  sl.step_bits = 0ull;
   sl.max_v_bits = 0ull;
   sl.max_a_bits = 0ull;
   sl.ndel = 0;
   for (int q = 0; q < 4; q++) sl.cnt[q] = 0;
   if (lane != 0) return;
   const bool have = S->have_parts != 0;
+  // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
+  const double disp = S->disp_acc + step;
   S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
+  S->disp_acc = S->rebuild ? 0.0 : disp;
   S->need_rebuild = 0;
   S->have_parts = 0;
   S->n_stored = S->n_alive;
@@ -612,7 +615,7 @@ __device__ __forceinline__ Hist find_hist(const Buf& A, int cap, int owner_s, in
 struct Acc {
   d3 F, T;
   int hc;
-  int nkind[4];
+  unsigned long long nkind;  // contacts by kind, 16 bits each (kind 1 lowest)
   int overflow;
 };
 
@@ -626,7 +629,7 @@ __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, 
   } else {
     acc.overflow = 1;
   }
-  acc.nkind[kind - 1]++;
+  acc.nkind += 1ull << (16 * (kind - 1));
 }
 
 // image of a partner coordinate: the reference's ghost copies hold
@@ -641,24 +644,45 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
   return xj;
 }
 
+// Contact-list entries: index of the partner in buffer A (moving) or in the
+// fixed-particle arrays, a flag for fixed partners and, for moving ones, a
+// flag for a periodic image.
+constexpr unsigned CL_INDEX = 0x3fffffffu, CL_FIXED = 0x40000000u, CL_SHIFT = 0x80000000u;
+
 // Contact test of one step: every particle walks its candidate list and keeps
 // the partners that touch it under the reference's predicate
-// (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557), as A-buffer indices
-// with a periodic-image flag in the top bit, sorted by partner id.  A light
-// kernel (few registers) so that many waves hide the gather latency.
+// (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557), then, with the plain
+// search (CS_Method 1), the fixed particles of its cells (FineSearch1
+// :478-498; the hierarchical search does not look for them).  The list is
+// sorted by partner id (fixed ids follow all moving ones), so that the sums
+// of a particle's contact forces do not depend on the grid, the skin or the
+// storage order.  A light kernel (few registers), so that many waves hide
+// the gather latency.
+__device__ __forceinline__ int cl_id(const Buf& A, const int* __restrict__ fid, unsigned e) {
+  return (e & CL_FIXED) ? fid[e & CL_INDEX] : A.id[e & CL_INDEX];
+}
+
 __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
                                                  const int* __restrict__ perm,
                                                  const int* __restrict__ vcnt,
                                                  const int* __restrict__ vlist,
                                                  unsigned* __restrict__ clist,
-                                                 int* __restrict__ ccnt) {
+                                                 int* __restrict__ ccnt,
+                                                 const double4* __restrict__ fpk,
+                                                 const int* __restrict__ fid,
+                                                 const int* __restrict__ fstart) {
   if (S->halt || S->overflow == 2) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= S->n_alive) return;
+  const int nq = min(vcnt[k], P.lcap);
+  const bool fixed = P.search_fixed && P.nfix > 0;
+  if (nq == 0 && !fixed) {  // loose beds: most particles have no candidate at all
+    ccnt[k] = 0;
+    return;
+  }
   const bool rb = S->rebuild != 0;
   const int s = rb ? perm[k] : k;
   const double4 pi = A.pos[s];
-  const int nq = min(vcnt[k], P.lcap);
   const size_t cap = P.cap;
   int c = 0;
   for (int q = 0; q < nq; q++) {
@@ -680,23 +704,71 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
     const double ddz = pi.z - zj;
     d2 = ddz * ddz + d2;
     if (d2 > dr2) continue;
-    if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)sj | (shifted ? 0x80000000u : 0u);
+    if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)sj | (shifted ? CL_SHIFT : 0u);
     c++;
+  }
+  if (fixed) {
+    const Grid& G = P.fgrid;
+    const double xv[3] = {pi.x, pi.y, pi.z};
+    const double reach = pi.w + G.rmax;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      const double pad = 1.0e-6 * G.w[d];
+      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+      if (!P.periodic[d]) {
+        lo[d] = lo[d] < 0 ? 0 : lo[d];
+        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      }
+    }
+    for (int cz = lo[2]; cz <= hi[2]; cz++) {
+      int wz = cz;
+      double shz = 0.0;
+      if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
+      else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
+      for (int cy = lo[1]; cy <= hi[1]; cy++) {
+        int wy = cy;
+        double shy = 0.0;
+        if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
+        else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
+        for (int cx = lo[0]; cx <= hi[0]; cx++) {
+          int wx = cx;
+          double shx = 0.0;
+          if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
+          else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
+          const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
+          for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
+            const double4 pj = fpk[kj];
+            const double dr = pi.w + pj.w;
+            const double dr2 = dr * dr;
+            const double ddx = pi.x - (pj.x + shx);
+            double d2 = ddx * ddx;
+            if (d2 > dr2) continue;
+            const double ddy = pi.y - (pj.y + shy);
+            d2 = ddy * ddy + d2;
+            if (d2 > dr2) continue;
+            const double ddz = pi.z - (pj.z + shz);
+            d2 = ddz * ddz + d2;
+            if (d2 > dr2) continue;
+            if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)kj | CL_FIXED;
+            c++;
+          }
+        }
+      }
+    }
   }
   ccnt[k] = c;
   if (c > P.hcap) {
     atomicMax(&S->overflow, 1);
     return;
   }
-  // canonical order: ascending partner id, so the sums of a particle's
-  // contact forces do not depend on the grid, the skin or the storage order
   for (int a = 1; a < c; a++) {
     const unsigned e = clist[(size_t)a * cap + k];
-    const int ida = A.id[e & 0x7fffffffu];
+    const int ida = cl_id(A, fid, e);
     int b = a - 1;
     while (b >= 0) {
       const unsigned eb = clist[(size_t)b * cap + k];
-      if (A.id[eb & 0x7fffffffu] <= ida) break;
+      if (cl_id(A, fid, eb) <= ida) break;
       clist[(size_t)(b + 1) * cap + k] = eb;
       b--;
     }
@@ -708,7 +780,6 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const int* __restrict__ perm,
                                                const unsigned* __restrict__ clist,
                                                const int* __restrict__ ccnt,
-                                               double4* __restrict__ ref,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
@@ -720,13 +791,15 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = S->n_alive;
   const bool rb = S->rebuild != 0;
+  // k_ctl has advanced iterNumber: is the next step a near-wall update?
+  const bool upd_next = S->iter >= S->next_wall_update;
   Acc acc;
   acc.F = D3(0.0, 0.0, 0.0);
   acc.T = D3(0.0, 0.0, 0.0);
   acc.hc = 0;
-  acc.nkind[0] = acc.nkind[1] = acc.nkind[2] = acc.nkind[3] = 0;
+  acc.nkind = 0ull;
   acc.overflow = 0;
-  double disp2 = 0.0;
+  double move2 = 0.0, vnext = 0.0, anext = 0.0;
   bool dead = false;
   if (k < n) {
     const int cap = P.cap;
@@ -738,153 +811,107 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
     const int ti = A.type[s];
     const d3 vi = ld3(A.lv, cap, s);
     const d3 wi = ld3(A.rv, cap, s);
-    const double xv[3] = {pi.x, pi.y, pi.z};
-    double4 r0;
-    if (rb) {
-      ref[k] = pi;
-      r0 = pi;
-    } else {
-      r0 = ref[k];
-    }
 
-    // ---------------- moving particles touching it (k_contact) ------------
+    // ---------------- every contact, one call site ------------------------
+    // The contacts k_contact found (moving partners, then fixed ones, by
+    // ascending partner id), then the walls of the near-wall list, newest
+    // first (FindContactsPW :298-319 walks the list from the highest index).
+    // Each contact is reduced to the arguments of the force routine from the
+    // perspective the reference evaluates it from (ContactForce_PP :732-756,
+    // _PPG :761-798, _PPFix :803-827, _PW :832-856), so the routine is
+    // inlined once.
     const int nc = min(ccnt[k], P.hcap);
-    for (int q = 0; q < nc; q++) {
-      const unsigned e = clist[(size_t)q * cap + k];
-      const int sj = (int)(e & 0x7fffffffu);
-      const double4 pj = A.pos[sj];
-      int shifted = 0;
-      const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
-      const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
-      const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
-      const double dr = ri + pj.w;
-      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
-      double d2 = ddx * ddx;
-      d2 = ddy * ddy + d2;
-      d2 = ddz * ddz + d2;
-      const double ovrlp = dr - sqrt(d2);
-      const int idj = A.id[sj];
-      const int tj = A.type[sj];
-      const d3 vj = ld3(A.lv, cap, sj);
-      const d3 wj = ld3(A.rv, cap, sj);
-      const d3 xjv = D3(xj, yj, zj);
-      if (!shifted) {
-        // PP: record in the lower id's bucket, force from its side
-        // (FineSearch2 :525-529, ContactForce_PP :732-756)
-        const bool ilo = idi < idj;
-        const Hist h = ilo ? find_hist(A, P.cap, s, idj) : find_hist(A, P.cap, sj, idi);
-        const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
-        const Bnry& Bp = P.pp[(tlo - 1) * P.ntype + (thi - 1)];
-        CF cf;
-        if (ilo) {
-          cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
-          acc.F = add3(acc.F, cf.fsum);
-          acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-          push_hist(P, B, k, acc, 1, idj, cf);
+    uint64_t mask = A.wmask[s];
+    const int ntot = nc + __popcll(mask);
+    for (int q = 0; q < ntot; q++) {
+      d3 nvec, va, vb, wa, wb;
+      double ra, rb, ovrlp;
+      Hist h;
+      const Bnry* Bp;
+      int kind, partner;
+      bool flip = false;
+      if (q < nc) {
+        const unsigned e = clist[(size_t)q * cap + k];
+        const int j = (int)(e & CL_INDEX);
+        double4 pj;
+        d3 vj, wj;
+        int idj, tj;
+        if (e & CL_FIXED) {
+          pj = fpk[j];
+          vj = D3(0.0, 0.0, 0.0);
+          wj = vj;
+          idj = fid[j];
+          tj = ftype[j];
         } else {
-          cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
-          acc.F = sub3(acc.F, cf.fsum);
-          acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+          pj = A.pos[j];
+          vj = ld3(A.lv, cap, j);
+          wj = ld3(A.rv, cap, j);
+          idj = A.id[j];
+          tj = A.type[j];
+        }
+        int shifted = 0;
+        const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
+        const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
+        const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
+        const double dr = ri + pj.w;
+        const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
+        double d2 = ddx * ddx;
+        d2 = ddy * ddy + d2;
+        d2 = ddz * ddz + d2;
+        ovrlp = dr - sqrt(d2);
+        const d3 xjv = D3(xj, yj, zj);
+        partner = idj;
+        if (e & CL_FIXED) {
+          // PPFix: the moving particle's record, its own perspective
+          kind = 3;
+          h = find_hist(A, cap, s, idj);
+          Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+        } else if (!shifted) {
+          // PP: the record sits in the lower id's bucket and the force is
+          // taken from that side (FineSearch2 :525-529)
+          const bool ilo = idi < idj;
+          kind = ilo ? 1 : 0;
+          h = ilo ? find_hist(A, cap, s, idj) : find_hist(A, cap, j, idi);
+          const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
+          Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
+          flip = !ilo;
+        } else {
+          // PPG: each side keeps its own record against the other's image,
+          // the force from the lower id's side; row type = the local particle
+          kind = 2;
+          h = find_hist(A, cap, s, idj);
+          Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+          flip = idi > idj;
+        }
+        if (!flip) {
+          nvec = nv3(xjv, xi);
+          ra = ri; rb = pj.w; va = vi; vb = vj; wa = wi; wb = wj;
+        } else {
+          nvec = nv3(xi, xjv);
+          ra = pj.w; rb = ri; va = vj; vb = vi; wa = wj; wb = wi;
         }
       } else {
-        // PPG: each side keeps its own record against the other's image
-        // (ContactForce_PPG :761-798); row type = the local particle
-        const Hist h = find_hist(A, P.cap, s, idj);
-        const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (tj - 1)];
-        CF cf;
-        if (idi < idj) {
-          cf = contact_force(P, Bp, nv3(xjv, xi), ri, pj.w, vi, vj, wi, wj, ovrlp, h.td, h.vri);
-          acc.F = add3(acc.F, cf.fsum);
-          acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-        } else {
-          cf = contact_force(P, Bp, nv3(xi, xjv), pj.w, ri, vj, vi, wj, wi, ovrlp, h.td, h.vri);
-          acc.F = sub3(acc.F, cf.fsum);
-          acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-        }
-        push_hist(P, B, k, acc, 2, idj, cf);
+        const int w = 63 - __clzll((long long)mask);
+        mask &= ~(1ull << w);
+        const Wall& W = P.walls[w];
+        d3 nv;
+        if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
+        kind = 4;
+        partner = W.wall_id;
+        h = find_hist(A, cap, s, W.wall_id);
+        Bp = &P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
+        nvec = mul3(nv, -1.0);  // Norm_v = -nv
+        ra = ri; rb = 1.00e20 * ri; va = vi; vb = wp(W.trans_vel); wa = wi; wb = D3(0.0, 0.0, 0.0);
       }
-    }
-    // ---------------- fixed particles (NBS only) --------------------------
-    if (P.search_fixed && P.nfix > 0) {
-      const Grid& G = P.fgrid;
-      const double reach = ri + G.rmax;
-      int lo[3], hi[3];
-      for (int d = 0; d < 3; d++) {
-        double pad = 1.0e-6 * G.w[d];
-        lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-        hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-        if (!P.periodic[d]) {
-          lo[d] = lo[d] < 0 ? 0 : lo[d];
-          hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-        }
-      }
-      for (int cz = lo[2]; cz <= hi[2]; cz++) {
-        int wz = cz;
-        double shz = 0.0;
-        if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
-        else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
-        for (int cy = lo[1]; cy <= hi[1]; cy++) {
-          int wy = cy;
-          double shy = 0.0;
-          if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
-          else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
-          for (int cx = lo[0]; cx <= hi[0]; cx++) {
-            int wx = cx;
-            double shx = 0.0;
-            if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
-            else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
-            const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
-            for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
-              const double4 pj = fpk[kj];
-              const double xj = pj.x + shx, yj = pj.y + shy, zj = pj.z + shz;
-              // NBSM_FineSearch1 (Prtcl_NBS_Munjiza.f90:478-498)
-              const double dr = ri + pj.w;
-              const double dr2 = dr * dr;
-              const double ddx = xi.x - xj;
-              double d2 = ddx * ddx;
-              if (d2 > dr2) continue;
-              const double ddy = xi.y - yj;
-              d2 = ddy * ddy + d2;
-              if (d2 > dr2) continue;
-              const double ddz = xi.z - zj;
-              d2 = ddz * ddz + d2;
-              if (d2 > dr2) continue;
-              const double ovrlp = dr - sqrt(d2);
-              const int idj = fid[kj];
-              const Hist h = find_hist(A, P.cap, s, idj);
-              const Bnry& Bp = P.pp[(ti - 1) * P.ntype + (ftype[kj] - 1)];
-              const d3 z3 = D3(0.0, 0.0, 0.0);
-              CF cf = contact_force(P, Bp, nv3(D3(xj, yj, zj), xi), ri, pj.w, vi, z3, wi, z3,
-                                    ovrlp, h.td, h.vri);
-              acc.F = add3(acc.F, cf.fsum);
-              acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-              push_hist(P, B, k, acc, 3, idj, cf);
-            }
-          }
-        }
-      }
-    }
-
-    // ---------------- walls of the near-wall list -------------------------
-    // (FindContactsPW :298-319 walks the list newest first = highest index)
-    uint64_t mask = A.wmask[s];
-    while (mask) {
-      const int w = 63 - __clzll((long long)mask);
-      mask &= ~(1ull << w);
-      const Wall& W = P.walls[w];
-      double ovrlp;
-      d3 nv;
-      if (pw_in_contact(W, xi, ri, &ovrlp, &nv)) {
-        // ContactForce_PW (Prtcl_CL_and_CF.f90:832-856)
-        const Hist h = find_hist(A, P.cap, s, W.wall_id);
-        const Bnry& Bp = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
-        const d3 norm_v = mul3(nv, -1.0);
-        CF cf = contact_force(P, Bp, norm_v, ri, 1.00e20 * ri, vi, wp(W.trans_vel), wi,
-                              D3(0.0, 0.0, 0.0), ovrlp, h.td, h.vri);
+      const CF cf = contact_force(P, *Bp, nvec, ra, rb, va, vb, wa, wb, ovrlp, h.td, h.vri);
+      if (!flip) {
         acc.F = add3(acc.F, cf.fsum);
         acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-        push_hist(P, B, k, acc, 4, W.wall_id, cf);
+      } else {
+        acc.F = sub3(acc.F, cf.fsum);
+        acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
       }
+      if (kind) push_hist(P, B, k, acc, kind, partner, cf);
     }
 
     // ---------------- acceleration (DEMS_clc_Acceleration :360-375) -------
@@ -957,6 +984,12 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
     }
     st3(B.th, cap, k, th);
 
+    // the move of this step, before any periodic wrap
+    {
+      const double mx = pos.x - xi.x, my = pos.y - xi.y, mz = pos.z - xi.z;
+      move2 = mx * mx + my * my + mz * mz;
+    }
+
     // ---------------- periodic wrap / deletion (Comm_For_Exchange) --------
     double pv[3] = {pos.x, pos.y, pos.z};
     for (int d = 0; d < 3; d++) {
@@ -971,33 +1004,38 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
     B.id[k] = idi;
     B.type[k] = dead ? -ti : ti;
     B.mark[k] = A.mark[s];
-    B.wmask[k] = A.wmask[s];
+    // the next step's near-wall list is built here from the new state
+    // (UpdateNearPrtclsPW runs at the start of that step, after deletion)
+    B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : A.wmask[s];
+    if (upd_next && !dead) {
+      vnext = norm3(lin1);
+      anext = norm3(a1);  // linAcc(1) at the start of the next step
+    }
     B.hcnt[k] = dead ? 0 : acc.hc;
     if (P.keep_forces) {
       st3(B.force, cap, k, acc.F);
       st3(B.force + 3 * cap, cap, k, acc.T);
     }
-    // displacement since the last list rebuild (nearest periodic image)
-    const double r0v[3] = {r0.x, r0.y, r0.z};
-    for (int d = 0; d < 3; d++) {
-      double dd = pv[d] - r0v[d];
-      if (P.periodic[d]) {
-        if (dd > 0.5 * P.simlen[d]) dd -= P.simlen[d];
-        else if (dd < -0.5 * P.simlen[d]) dd += P.simlen[d];
-      }
-      disp2 += dd * dd;
-    }
   }
   // block reductions, then one atomic per block on slot (block % NSLOT)
   __shared__ double shd[4];
   __shared__ int shi[4];
-  const double bdisp = block_max(disp2, shd);
+  const double bmove = block_max(move2, shd);
   int bc[4];
-  for (int q = 0; q < 4; q++) bc[q] = block_sum(acc.nkind[q], shi);
+  for (int q = 0; q < 4; q++) bc[q] = block_sum((int)((acc.nkind >> (16 * q)) & 0xffffull), shi);
   const int bdel = block_sum(dead ? 1 : 0, shi);
+  double bv = 0.0, ba = 0.0;
+  if (upd_next) {
+    bv = block_max(vnext, shd);
+    ba = block_max(anext, shd);
+  }
   if (threadIdx.x == 0) {
     Slot& sl = slots[blockIdx.x % NSLOT];
-    if (bdisp > 0.0) atomicMax(&sl.disp_bits, dbits(sqrt(bdisp)));
+    if (bmove > 0.0) atomicMax(&sl.step_bits, dbits(sqrt(bmove)));
+    if (upd_next) {
+      atomicMax(&sl.max_v_bits, dbits(bv));
+      atomicMax(&sl.max_a_bits, dbits(ba));
+    }
     for (int q = 0; q < 4; q++)
       if (bc[q]) atomicAdd(&sl.cnt[q], bc[q]);
     if (bdel) atomicAdd(&sl.ndel, bdel);
@@ -1016,6 +1054,10 @@ int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
                       hipStream_t st) {
   hipLaunchKernelGGL(k_pre, dim3(nblk(n_upper > 0 ? n_upper : 1, 256)), dim3(256), 0, st, P, A, S, slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_ctl(const Params& P, Scalars* S, Slot* slots, hipStream_t st) {
   hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, P, S, slots);
   return hipGetLastError();
 }
@@ -1053,15 +1095,15 @@ hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4*
 
 hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
                         const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
-                        int* ccnt, double4* ref, const double4* fpk, const int* fid,
+                        int* ccnt, const double4* fpk, const int* fid,
                         const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
                         hipEvent_t mid) {
   if (n_upper > 0) {
     hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, vcnt,
-                       vlist, clist, ccnt);
+                       vlist, clist, ccnt, fpk, fid, fstart);
     if (mid) (void)hipEventRecord(mid, st);
     hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, clist,
-                       ccnt, ref, fpk, fid, ftype, fstart, slots);
+                       ccnt, fpk, fid, ftype, fstart, slots);
   }
   return hipGetLastError();
 }

@@ -29,7 +29,7 @@ CFT_LSD, CFT_NLIN = 1, 2
 CTM_CONSTANT_TORQUE, CTM_VARIABLE_TORQUE = 1, 2
 PIM_FE, PIM_AB2, PIM_AB3 = 1, 2, 3
 KIND_PP, KIND_PPG, KIND_PPFIX, KIND_PW = 1, 2, 3, 4
-PHASE_NAMES = ("pre", "bin_sort", "search", "contact", "force")
+PHASE_NAMES = ("control", "bin_sort", "search", "contact", "force")
 
 
 class SdemError(RuntimeError):
