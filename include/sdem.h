@@ -46,6 +46,10 @@ extern "C" {
 #define SDEM_CSM_NBS_MUNJIZA_HRCHL  2
 #define SDEM_CFT_LSD   1
 #define SDEM_CFT_NLIN  2
+/* constant-stiffness linear spring-dashpot: no counterpart in the reference
+ * (its CF_Type has only the two models above); set with
+ * sdem_set_linear_stiffness */
+#define SDEM_CFT_LINEAR 3
 #define SDEM_CTM_CONSTANT_TORQUE  1
 #define SDEM_CTM_VARIABLE_TORQUE  2
 #define SDEM_PIM_FE   1
@@ -112,6 +116,17 @@ int sdem_set_particle_property(sdem_ctx* c, int32_t ntype,
                                const double* youngs_modulus, const double* poisson_ratio,
                                double fric_s, double fric_k, double fric_roll,
                                double rest_n, double rest_t);
+
+/* Constant-stiffness linear model (cf_type SDEM_CFT_LINEAR; no counterpart in
+ * the reference): normal stiffness kn [N/m] for every particle-particle and
+ * particle-wall pair, tangential stiffness kt = kt_over_kn * kn, normal
+ * damping from the pair's restitution e and effective mass m* as the
+ * reference's linear model has it at unit velocity factor,
+ * dn = -2 ln(e) sqrt(m* kn) / sqrt(pi^2 + ln(e)^2), which gives the
+ * restitution e exactly for a head-on collision.  Here m* is the true pair
+ * value (both types), and walls count as infinitely heavy.  Call before
+ * sdem_set_particle_property. */
+int sdem_set_linear_stiffness(sdem_ctx* c, double kn, double kt_over_kn);
 
 /* InitWallProperty (Prtcl_Property.f90:148-195): wall pure properties and the
  * particle-wall binary table. Call after sdem_set_particle_property. */

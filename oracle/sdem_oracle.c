@@ -116,6 +116,7 @@ typedef struct oracle_ctx {
   bnry_prop *pp_tab;  /* (i,j) -> [(i-1)*ntype + (j-1)] */
   bnry_prop *pw_tab;  /* (i,w) -> [(i-1)*nwtype + (w-1)] */
   int have_pprop, have_wprop;
+  double lin_kn, lin_kt_ratio; /* constant-stiffness linear model */
 
   /* geometry */
   int nwall, mwall, num_pwall_total;
@@ -215,7 +216,8 @@ int oracle_create(const sdem_options *o, oracle_ctx **out) {
   if (!o || !out) return SDEM_E_ARG;
   oracle_ctx *c = (oracle_ctx *)xcalloc(1, sizeof(oracle_ctx));
   c->o = *o;
-  if (o->pi_method < 1 || o->pi_method > 3 || o->pri_method < 1 || o->pri_method > 3) {
+  if (o->pi_method < 1 || o->pi_method > 3 || o->pri_method < 1 || o->pri_method > 3 ||
+      o->cf_type < 1 || o->cf_type > 3) {
     free(c);
     return SDEM_E_ARG;
   }
@@ -306,6 +308,13 @@ static bnry_prop clc_bnry(const oracle_ctx *c, const pure_prop *pi, const pure_p
     b.kt = kappa * b.kn;
     b.dn = -2.0 * log(en) * sqrt(b.mass_eff * b.kn) / sqrt(PI * PI + eta);
     b.dt = 0.0;
+  } else if (c->o.cf_type == SDEM_CFT_LINEAR) {
+    /* no counterpart in the reference: constant kn, kt = ratio*kn, and the
+     * damping of the LSD formula at unit velocity factor */
+    b.kn = c->lin_kn;
+    b.kt = c->lin_kt_ratio * b.kn;
+    b.dn = -2.0 * log(en) * sqrt(b.mass_eff * b.kn) / sqrt(PI * PI + eta);
+    b.dt = 0.0;
   } else {
     double kh = 4.0 / 3.0 * e_eff * sqrt(b.rad_eff);
     b.kn = kh;
@@ -316,11 +325,24 @@ static bnry_prop clc_bnry(const oracle_ctx *c, const pure_prop *pi, const pure_p
   return b;
 }
 
+/* Constant-stiffness linear model: no counterpart in the reference. */
+int oracle_set_linear_stiffness(oracle_ctx *c, double kn, double kt_over_kn) {
+  if (!c) return SDEM_E_ARG;
+  if (c->o.cf_type != SDEM_CFT_LINEAR) return fail(c, SDEM_E_STATE, "cf_type is not the linear model");
+  if (!(kn > 0.0) || !(kt_over_kn >= 0.0)) return fail(c, SDEM_E_ARG, "bad linear stiffness");
+  if (c->have_pprop) return fail(c, SDEM_E_STATE, "stiffness must be set before the properties");
+  c->lin_kn = kn;
+  c->lin_kt_ratio = kt_over_kn;
+  return SDEM_OK;
+}
+
 /* InitPrtclProperty :62-143 (property part; the bin counts are not needed) */
 int oracle_set_particle_property(oracle_ctx *c, int ntype, const double *diam, const double *dens,
                                  const double *E, const double *nu, double fs, double fk,
                                  double froll, double en, double et) {
   if (ntype < 1) return fail(c, SDEM_E_ARG, "ntype < 1");
+  if (c->o.cf_type == SDEM_CFT_LINEAR && !(c->lin_kn > 0.0))
+    return fail(c, SDEM_E_STATE, "linear model: set the stiffness first");
   const double PI = 3.141592653589793238462643383279502884;
   c->ntype = ntype;
   free(c->pprop); free(c->pp_tab); free(c->type_level);
@@ -340,9 +362,10 @@ int oracle_set_particle_property(oracle_ctx *c, int ntype, const double *diam, c
   }
   for (int i = 1; i <= ntype; i++)
     for (int j = 1; j <= ntype; j++) {
-      /* pari = parj = Prtcl_PureProp(i)  (:137-138) */
-      c->pp_tab[(i - 1) * ntype + (j - 1)] =
-          clc_bnry(c, &c->pprop[i], &c->pprop[i], fs, fk, froll, en, et, 0);
+      /* pari = parj = Prtcl_PureProp(i)  (:137-138); the linear model, which
+       * the reference does not have, takes the true pair */
+      const pure_prop *pj = c->o.cf_type == SDEM_CFT_LINEAR ? &c->pprop[j] : &c->pprop[i];
+      c->pp_tab[(i - 1) * ntype + (j - 1)] = clc_bnry(c, &c->pprop[i], pj, fs, fk, froll, en, et, 0);
     }
   c->have_pprop = 1;
   return SDEM_OK;
@@ -789,7 +812,12 @@ static cf_out contact_force_inc(oracle_ctx *c, const bnry_prop *P, r3 norm_v, do
 
   double fn, k_t, d_t;
   r3 ftij;
-  if (c->o.cf_type == SDEM_CFT_LSD) {
+  if (c->o.cf_type == SDEM_CFT_LINEAR) {
+    fn = -P->kn * ovrlp + -P->dn * vrn;
+    k_t = -P->kt;
+    d_t = 0.0;
+    ftij = add3(mul3(ovlp_t, k_t), mul3(vij_t, d_t));
+  } else if (c->o.cf_type == SDEM_CFT_LSD) {
     double vel_in = fmax_f(c->vrel[ind], 0.01);
     double k_n = -(P->kn * pow(vel_in, 0.4));
     double d_n = -(P->dn * pow(vel_in, 0.2));

@@ -143,7 +143,7 @@ bool wall_in_domain(const Wall& w, V3 mn, V3 mx) {  // PW_IsInDomain :114-180
 
 // clc_BnryPrtcl_Prop (Prtcl_Property.f90:200-247)
 Bnry bnry_prop(int cf_type, const PureProp& pi, const PureProp& pj, double fs, double fk,
-               double froll, double en, bool iswall) {
+               double froll, double en, bool iswall, double lin_kn, double lin_kt_ratio) {
   Bnry b;
   if (!iswall) {
     b.rad_eff = (pi.radius * pj.radius) / (pi.radius + pj.radius);
@@ -168,6 +168,12 @@ Bnry bnry_prop(int cf_type, const PureProp& pi, const PureProp& pj, double fs, d
   if (cf_type == SDEM_CFT_LSD) {
     b.kn = 1.2024 * std::pow(std::sqrt(b.mass_eff) * (e_eff * e_eff) * b.rad_eff, 0.4);
     b.kt = kappa * b.kn;
+    b.dn = -2.0 * std::log(en) * std::sqrt(b.mass_eff * b.kn) / std::sqrt(PI * PI + eta);
+  } else if (cf_type == SDEM_CFT_LINEAR) {
+    // no counterpart in the reference: constant kn, kt = ratio * kn, and the
+    // damping of the LSD formula at unit velocity factor
+    b.kn = lin_kn;
+    b.kt = lin_kt_ratio * lin_kn;
     b.dn = -2.0 * std::log(en) * std::sqrt(b.mass_eff * b.kn) / std::sqrt(PI * PI + eta);
   } else {
     const double kh = 4.0 / 3.0 * e_eff * std::sqrt(b.rad_eff);
@@ -215,6 +221,7 @@ struct sdem_ctx {
   double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
   int keep_forces = 1;
   bool first_step_pending = false;
+  double lin_kn = 0.0, lin_kt_ratio = 0.0;  // constant-stiffness linear model
   bool need_pre = false;  // the state in buffer A has no near-wall pass yet
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
@@ -398,7 +405,7 @@ int sdem_create(const sdem_options* o, sdem_ctx** out) {
   if (!o || !out) return SDEM_E_ARG;
   *out = nullptr;
   if (o->pi_method < 1 || o->pi_method > 3 || o->pri_method < 1 || o->pri_method > 3 ||
-      o->cf_type < 1 || o->cf_type > 2 || o->ct_model < 1 || o->ct_model > 2 ||
+      o->cf_type < 1 || o->cf_type > 3 || o->ct_model < 1 || o->ct_model > 2 ||
       o->cs_method < 1 || o->cs_method > 2 || o->num_prtcl < 1 || !(o->dt > 0.0) ||
       !(o->prtcl_cs_ratio >= 1.0) || o->wall_max_update_iter < 1)
     return SDEM_E_ARG;
@@ -425,6 +432,16 @@ int sdem_create(const sdem_options* o, sdem_ctx** out) {
   return SDEM_OK;
 }
 
+int sdem_set_linear_stiffness(sdem_ctx* c, double kn, double kt_over_kn) {
+  if (!c) return SDEM_E_ARG;
+  if (c->o.cf_type != SDEM_CFT_LINEAR) return c->fail(SDEM_E_STATE, "cf_type is not the linear model");
+  if (!(kn > 0.0) || !(kt_over_kn >= 0.0)) return c->fail(SDEM_E_ARG, "bad linear stiffness");
+  if (c->have_pprop) return c->fail(SDEM_E_STATE, "stiffness must be set before the properties");
+  c->lin_kn = kn;
+  c->lin_kt_ratio = kt_over_kn;
+  return SDEM_OK;
+}
+
 int sdem_set_particle_property(sdem_ctx* c, int32_t ntype, const double* diameter,
                                const double* density, const double* youngs, const double* poisson,
                                double fs, double fk, double froll, double en, double et) {
@@ -433,6 +450,8 @@ int sdem_set_particle_property(sdem_ctx* c, int32_t ntype, const double* diamete
   if (ntype < 1 || ntype > MAX_TYPES || !diameter || !density || !youngs || !poisson)
     return c->fail(SDEM_E_ARG, "bad particle property arguments");
   if (c->have_state) return c->fail(SDEM_E_STATE, "properties must be set before the state");
+  if (c->o.cf_type == SDEM_CFT_LINEAR && !(c->lin_kn > 0.0))
+    return c->fail(SDEM_E_STATE, "linear model: set the stiffness first");
   const double PI = 3.141592653589793238462643383279502884;
   c->ntype = ntype;
   c->pprop.assign(ntype, PureProp{});
@@ -450,7 +469,11 @@ int sdem_set_particle_property(sdem_ctx* c, int32_t ntype, const double* diamete
   c->pp_tab.assign((size_t)ntype * ntype, Bnry{});
   for (int i = 0; i < ntype; i++)
     for (int j = 0; j < ntype; j++)  // pari = parj = type i (Prtcl_Property.f90:137-138)
-      c->pp_tab[(size_t)i * ntype + j] = bnry_prop(c->o.cf_type, c->pprop[i], c->pprop[i], fs, fk, froll, en, false);
+      // the reference's table takes the row type only (Prtcl_Property.f90:137-138);
+      // the linear model, which it does not have, takes the true pair
+      c->pp_tab[(size_t)i * ntype + j] =
+          bnry_prop(c->o.cf_type, c->pprop[i], c->o.cf_type == SDEM_CFT_LINEAR ? c->pprop[j] : c->pprop[i],
+                    fs, fk, froll, en, false, c->lin_kn, c->lin_kt_ratio);
   c->have_pprop = true;
   c->have_wprop = false;
   return SDEM_OK;
@@ -473,7 +496,8 @@ int sdem_set_wall_property(sdem_ctx* c, int32_t nwtype, const double* youngs, co
   c->pw_tab.assign((size_t)c->ntype * nwtype, Bnry{});
   for (int i = 0; i < c->ntype; i++)
     for (int j = 0; j < nwtype; j++)
-      c->pw_tab[(size_t)i * nwtype + j] = bnry_prop(c->o.cf_type, c->pprop[i], c->wprop[j], fs, fk, froll, en, true);
+      c->pw_tab[(size_t)i * nwtype + j] =
+          bnry_prop(c->o.cf_type, c->pprop[i], c->wprop[j], fs, fk, froll, en, true, c->lin_kn, c->lin_kt_ratio);
   c->have_wprop = true;
   return SDEM_OK;
 }

@@ -102,7 +102,10 @@ __device__ __forceinline__ CF contact_force(const Params& P, const Bnry& B, d3 n
   double fn, k_t;
   const double d_t = 0.0;
   d3 ftij;
-  if (P.cf_type == 1) {  // CFT_LSD
+  if (P.cf_type == 3) {  // constant-stiffness linear model (not in the reference)
+    fn = -B.kn * ovrlp + -B.dn * vrn;
+    k_t = -B.kt;
+  } else if (P.cf_type == 1) {  // CFT_LSD
     double vel_in = vri > 0.01 ? vri : 0.01;
     double k_n = -(B.kn * pow(vel_in, 0.4));
     double d_n = -(B.dn * pow(vel_in, 0.2));

@@ -25,7 +25,7 @@ ERRORS = {-1: "SDEM_E_ARG", -2: "SDEM_E_HIP", -3: "SDEM_E_CAPACITY", -4: "SDEM_E
           -5: "SDEM_E_NODEV"}
 
 CSM_NBS_MUNJIZA, CSM_NBS_MUNJIZA_HRCHL = 1, 2
-CFT_LSD, CFT_NLIN = 1, 2
+CFT_LSD, CFT_NLIN, CFT_LINEAR = 1, 2, 3
 CTM_CONSTANT_TORQUE, CTM_VARIABLE_TORQUE = 1, 2
 PIM_FE, PIM_AB2, PIM_AB3 = 1, 2, 3
 KIND_PP, KIND_PPG, KIND_PPFIX, KIND_PW = 1, 2, 3, 4
@@ -90,6 +90,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "create": ([C.POINTER(Options), C.POINTER(V)], C.c_int),
         "set_particle_property": ([V, C.c_int32, _DP, _DP, _DP, _DP] + [C.c_double] * 5, C.c_int),
         "set_wall_property": ([V, C.c_int32, _DP, _DP] + [C.c_double] * 5, C.c_int),
+        "set_linear_stiffness": ([V, C.c_double, C.c_double], C.c_int),
         "add_plane_wall": ([V, _DP, _DP, _DP, _DP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _DP, _IP],
                            C.c_int),
         "set_fixed_particles": ([V, C.c_int32, _IP, _IP, _DP], C.c_int),
@@ -182,6 +183,9 @@ class Engine:
         self._check(self._fn("set_particle_property")(
             self.h, len(d), _dptr(d), _dptr(rho), _dptr(e), _dptr(nu),
             fric_s, fric_k, fric_roll, rest_n, rest_t), "set_particle_property")
+
+    def set_linear_stiffness(self, kn: float, kt_over_kn: float) -> None:
+        self._check(self._fn("set_linear_stiffness")(self.h, kn, kt_over_kn), "set_linear_stiffness")
 
     def set_wall_property(self, youngs, poisson, fric_s, fric_k, fric_roll, rest_n, rest_t) -> None:
         e, nu = _f64(youngs), _f64(poisson)
@@ -337,6 +341,8 @@ def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Opt
         eng.set_tuning(**tuning)
     if capacity and hasattr(lib, prefix + "set_capacity"):
         eng.set_capacity(**capacity)
+    if "linear" in scenario:
+        eng.set_linear_stiffness(*scenario["linear"])
     pt = scenario["ptypes"]
     eng.set_particle_property(pt["diameter"], pt["density"], pt["youngs"], pt["poisson"], *scenario["pp"])
     wt = scenario["wtypes"]

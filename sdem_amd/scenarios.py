@@ -23,7 +23,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from .native import (CFT_LSD, CFT_NLIN, CSM_NBS_MUNJIZA, CSM_NBS_MUNJIZA_HRCHL, CTM_CONSTANT_TORQUE,
+from .native import (CFT_LINEAR, CFT_LSD, CFT_NLIN, CSM_NBS_MUNJIZA, CSM_NBS_MUNJIZA_HRCHL, CTM_CONSTANT_TORQUE,
                      PIM_AB2, level_size)
 
 DEFAULT_SEED = 20261203
@@ -111,8 +111,10 @@ def base_options(**kw) -> dict:
 
 
 def two_spheres(d=2.0e-3, v=0.5, gap=0.0, cf_type=CFT_NLIN, rest=0.8, youngs=5.0e7, poisson=0.3,
-                density=2500.0, fric=0.4, roll=0.0, offset=(0.0, 0.0), **okw) -> Dict:
-    """Two equal spheres approaching head-on along x with speeds +v, -v."""
+                density=2500.0, fric=0.4, roll=0.0, offset=(0.0, 0.0), kn=1.0e5, kt_over_kn=0.8,
+                **okw) -> Dict:
+    """Two equal spheres approaching head-on along x with speeds +v, -v
+    (kn, kt_over_kn: the constant-stiffness linear model's parameters)."""
     L = 0.05
     c = np.array([L / 2, L / 2, L / 2])
     s = d + gap
@@ -122,11 +124,14 @@ def two_spheres(d=2.0e-3, v=0.5, gap=0.0, cf_type=CFT_NLIN, rest=0.8, youngs=5.0
                     lin_vel1=np.array([[v, 0, 0], [-v, 0, 0]]))
     opts = base_options(cf_type=cf_type, gravity=(0.0, 0.0, 0.0), num_prtcl=2,
                         domain_max=(L, L, L), **okw)
-    return dict(options=opts,
-                ptypes=dict(diameter=[d], density=[density], youngs=[youngs], poisson=[poisson]),
-                pp=(fric, fric, roll, rest, rest),
-                wtypes=dict(youngs=[youngs], poisson=[poisson]), pw=(fric, fric, roll, rest, rest),
-                walls=[], state=st)
+    sc = dict(options=opts,
+              ptypes=dict(diameter=[d], density=[density], youngs=[youngs], poisson=[poisson]),
+              pp=(fric, fric, roll, rest, rest),
+              wtypes=dict(youngs=[youngs], poisson=[poisson]), pw=(fric, fric, roll, rest, rest),
+              walls=[], state=st)
+    if cf_type == CFT_LINEAR:
+        sc["linear"] = (kn, kt_over_kn)
+    return sc
 
 
 def random_cloud(n=300, seed=DEFAULT_SEED, box=None, diameters=(1.0e-3, 1.5e-3, 2.0e-3),
@@ -188,6 +193,8 @@ def random_cloud(n=300, seed=DEFAULT_SEED, box=None, diameters=(1.0e-3, 1.5e-3, 
               pp=(0.4, 0.35, roll, 0.8, 0.8),
               wtypes=dict(youngs=[5.0e7], poisson=[0.3]), pw=(0.45, 0.4, roll, 0.7, 0.7),
               walls=wl, state=st)
+    if cf_type == CFT_LINEAR:
+        sc["linear"] = (5.0e4, 0.8)
     if nfixed:
         fr = diam.max() / 2
         fpos = rng.random((nfixed, 3)) * (box - 2 * fr) + fr
@@ -234,12 +241,15 @@ def settling(n=1_000_000, seed=DEFAULT_SEED, phi=0.3, ntypes=32, area=None, cs_m
                 walls=walls, state=st, meta=dict(height=height, phi=phi, side=side))
 
 
-def sandbox(n=10_000, seed=DEFAULT_SEED, box=(0.05, 0.05, 0.2), ntypes=11, jitter=0.1) -> Dict:
+def sandbox(n=10_000, seed=DEFAULT_SEED, box=(0.05, 0.05, 0.2), ntypes=11, jitter=0.1,
+            model="linear") -> Dict:
     """Closed box of 6 finite plane walls, diameters U[2,3] mm in `ntypes`
-    types, density 2500, LSD contacts (the reference's velocity-dependent
-    linear model) with nu=1/3 so that kt/kn=0.8, e=0.9, mu=0.3, dt=1e-5 s,
-    AB2.  Particles sit on a jittered cubic lattice filling the box (the
-    upper half cannot hold 10,000 non-overlapping spheres of up to 3 mm)."""
+    types, density 2500, e=0.9, mu=0.3, dt=1e-5 s, AB2.  model="linear": the
+    constant-stiffness spring-dashpot kn=1e5 N/m, kt=0.8 kn; model="lsd": the
+    reference's velocity-dependent linear model (LSD) with E=1e7 Pa and
+    nu=1/3, so that its kt/kn is also 0.8.  Particles sit on a jittered cubic
+    lattice filling the box (the upper half cannot hold 10,000
+    non-overlapping spheres of up to 3 mm)."""
     rng = np.random.default_rng(seed)
     edges = np.linspace(2.0e-3, 3.0e-3, ntypes + 1)
     centers = np.linspace(2.0e-3, 3.0e-3, ntypes)
@@ -260,13 +270,19 @@ def sandbox(n=10_000, seed=DEFAULT_SEED, box=(0.05, 0.05, 0.2), ntypes=11, jitte
     ids = np.arange(1, n + 1, dtype=np.int32)
     posr = np.concatenate([pos, dq[:, None] / 2], 1)
     st = zero_state(ids, t + 1, posr)
-    opts = base_options(dt=1.0e-5, domain_max=tuple(box), num_prtcl=n, cf_type=CFT_LSD)
-    return dict(options=opts,
-                ptypes=dict(diameter=list(centers), density=[2500.0] * ntypes, youngs=[1.0e7] * ntypes,
-                            poisson=[1.0 / 3.0] * ntypes),
-                pp=(0.3, 0.3, 0.0, 0.9, 0.9),
-                wtypes=dict(youngs=[1.0e7], poisson=[1.0 / 3.0]), pw=(0.3, 0.3, 0.0, 0.9, 0.9),
-                walls=box_walls((0, 0, 0), box), state=st)
+    if model not in ("linear", "lsd"):
+        raise ValueError("model must be 'linear' or 'lsd'")
+    opts = base_options(dt=1.0e-5, domain_max=tuple(box), num_prtcl=n,
+                        cf_type=CFT_LINEAR if model == "linear" else CFT_LSD)
+    sc = dict(options=opts,
+              ptypes=dict(diameter=list(centers), density=[2500.0] * ntypes, youngs=[1.0e7] * ntypes,
+                          poisson=[1.0 / 3.0] * ntypes),
+              pp=(0.3, 0.3, 0.0, 0.9, 0.9),
+              wtypes=dict(youngs=[1.0e7], poisson=[1.0 / 3.0]), pw=(0.3, 0.3, 0.0, 0.9, 0.9),
+              walls=box_walls((0, 0, 0), box), state=st)
+    if model == "linear":
+        sc["linear"] = (1.0e5, 0.8)
+    return sc
 
 
 def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:

@@ -11,6 +11,8 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "box_nlin_ab2": dict(),
     "box_lsd_ab2": dict(cf_type=native.CFT_LSD),
+    "box_linear_ab2": dict(cf_type=native.CFT_LINEAR),
+    "periodic_fixed_linear": dict(cf_type=native.CFT_LINEAR, periodic=(1, 1, 0), walls="zZ", nfixed=30),
     "periodic_x_walls_yz": dict(periodic=(1, 0, 0), walls="yYzZ"),
     "periodic_xy_bottom_infinite": dict(periodic=(1, 1, 0), walls="z", infinite_walls=True),
     "periodic_xyz": dict(periodic=(1, 1, 1), walls=""),
@@ -116,3 +118,24 @@ def test_tuning_does_not_change_bits(hip_lib):
                     ct["tan_delta"].tobytes()))
         g.close()
     assert out[0] == out[1] == out[2]
+
+
+# the benchmark configurations, shrunk so the oracle finishes in seconds
+CONFIGS = {
+    "sandbox_linear": lambda: scenarios.sandbox(n=1500, box=(0.04, 0.04, 0.08)),
+    "sandbox_lsd": lambda: scenarios.sandbox(n=1500, box=(0.04, 0.04, 0.08), model="lsd"),
+    "settling": lambda: scenarios.settling(n=20000),
+    "polydisperse_hierarchical": lambda: scenarios.polydisperse(n=3000),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_config_parity(name, hip_lib, oracle_lib):
+    print(f"config {name}: 30 steps", flush=True)
+    sc = CONFIGS[name]()
+    gpu, ref = engines(sc, hip_lib, oracle_lib)
+    for k in (1, 29):
+        gpu.iterate(k)
+        ref.iterate(k)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-11)
+        assert not errs, errs

@@ -73,6 +73,52 @@ def test_lsd_restitution_is_velocity_independent(oracle_lib):
     assert abs(np.mean(out) - 0.7) < 0.01, out
 
 
+@pytest.mark.parametrize("rest", [0.5, 0.9])
+def test_linear_model_restitution_and_duration(oracle_lib, rest):
+    """Constant-stiffness spring-dashpot (no reference counterpart): the
+    damping dn = -2 ln(e) sqrt(m* kn) / sqrt(pi^2 + ln^2 e) gives the damping
+    ratio zeta = -ln e / sqrt(pi^2 + ln^2 e), hence a rebound ratio of
+    exp(-zeta pi / sqrt(1 - zeta^2)) = e exactly and a contact time
+    pi / (omega_n sqrt(1 - zeta^2)), omega_n = sqrt(kn / m*)."""
+    v, d, rho, kn = 0.5, 2.0e-3, 2500.0, 1.0e5
+    dt = 2.0e-8
+    sc = scenarios.two_spheres(d=d, v=v, rest=rest, density=rho, dt=dt, cf_type=native.CFT_LINEAR, kn=kn)
+    # an AB2 history as if the spheres had been moving before: the default
+    # zero history makes the first step move 1.5 v dt, a spurious pre-load
+    sc["state"]["lin_vel"][:, 1, :] = sc["state"]["lin_vel"][:, 0, :]
+    e, nsteps = collide(oracle_lib, sc)
+    vel = e.get_state()["lin_vel"][:, 0, :]
+    m = rho * math.pi / 6 * d ** 3
+    mstar = m / 2
+    le = math.log(rest)
+    zeta = -le / math.sqrt(math.pi ** 2 + le * le)
+    wn = math.sqrt(kn / mstar)
+    tc = math.pi / (wn * math.sqrt(1 - zeta * zeta))
+    # the contact opens and closes between steps while the dashpot force
+    # c v is not zero there: an error of order zeta * wn * dt (first order)
+    tol = 4 * zeta * wn * dt + 1e-5
+    e_meas = (vel[1, 0] - vel[0, 0]) / (2 * v)
+    assert abs(e_meas - rest) < tol * rest, (e_meas, tol)
+    assert abs(nsteps * dt - tc) < 2 * dt, (nsteps * dt, tc)
+
+
+def test_linear_model_tangential_stiffness(oracle_lib):
+    """First step of a sheared contact: the history is the tangential
+    relative displacement u dt, so the tangential force is kt u dt with
+    kt = 0.8 kn, and the normal force is kn delta (no normal motion)."""
+    d, kn, u, dt, delta = 2.0e-3, 1.0e5, 0.01, 1.0e-7, 1.0e-6
+    sc = scenarios.two_spheres(d=d, v=0.0, gap=-delta, rest=0.9, dt=dt, cf_type=native.CFT_LINEAR, kn=kn,
+                               fric=0.5)
+    sc["state"]["lin_vel"][1, 0, :] = (0.0, u, 0.0)
+    e = make(oracle_lib, sc)
+    e.iterate(1)
+    _, f, _ = e.get_forces()
+    # particle 1 (lower id) is pushed away from 2 (-x) and dragged along +y
+    assert abs(f[0, 0] + kn * delta) < 1e-12 * kn * delta
+    assert abs(f[0, 1] - 0.8 * kn * u * dt) < 1e-9 * kn * u * dt
+    assert np.allclose(f[0], -f[1], rtol=0, atol=1e-18)
+
+
 def test_static_wall_overlap(oracle_lib):
     """A sphere resting on a wall settles at the Hertz equilibrium overlap
     delta = (m g / K_h)^(2/3), K_h = 4/3 E* sqrt(R) for a flat wall."""
