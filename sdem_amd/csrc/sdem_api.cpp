@@ -570,7 +570,8 @@ static int alloc_buf(sdem_ctx* c, Buf& b) {
 }
 
 // history slots per particle: a kissing-number bound for the largest type
-// touched by the smallest, with room for soft overlaps and walls
+// touched by the smallest, with room for soft overlaps and walls, at most 48
+// to start with (an overflow doubles it and replays the step)
 static int history_capacity(const sdem_ctx* c) {
   double rmax = max_type_radius(c), rmin = min_type_radius(c);
   double s = rmin / (rmax + rmin);
@@ -579,7 +580,7 @@ static int history_capacity(const sdem_ctx* c) {
   int h = (int)std::ceil(1.25 * bound) + 4 + (int)c->walls.size();
   h = std::max(h, 2 * c->o.cntct_list_size);
   h = std::max(h, 16);
-  return std::min(h, 256);
+  return std::min(h, 48);
 }
 
 int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* ptype,

@@ -2,12 +2,13 @@
 // C-ABI implementation of libsdem_hip.so.
 //
 // Particle state lives in two SoA buffers (A = state at the start of a step,
-// B = state at its end).  The fused step kernel reads A through the cell-sort
-// permutation and writes B in cell order, so every step leaves the particles
-// stored in cell order for the next one.  Contact history is held per owner
-// particle in CAP slots next to the particle (owner = the particle whose
-// bucket holds the record in the reference: the lower id for PP, the local
-// particle for PPG / PPFix / PW; Prtcl_CL_and_CF.f90:142-249).
+// B = state at its end).  The force kernel reads A (through the cell-sort
+// permutation on the steps that rebuild the candidate lists) and writes B,
+// so the particles stay stored in the cell order of the last rebuild.
+// Contact history is held per owner particle in hcap slots interleaved with
+// the other particles' (owner = the particle whose bucket holds the record
+// in the reference: the lower id for PP, the local particle for PPG / PPFix
+// / PW; Prtcl_CL_and_CF.f90:142-249).
 #pragma once
 
 #include <hip/hip_runtime.h>
