@@ -107,6 +107,19 @@ int sdem_options_default(sdem_options* o);
  * decomposition computed is replaced by the whole domain on one GPU. */
 int sdem_create(const sdem_options* o, sdem_ctx** out);
 
+/* Ensemble of independent boxes (no counterpart in the reference): nbox
+ * systems, each with its own options (domain, walls, materials, contact
+ * model, time step), stepped together by one sdem_step -- every kernel of a
+ * step covers all boxes at once.  All boxes share PI_Method, PRI_Method and
+ * ifirst; fixed particles need a single-box context.  sdem_create is the
+ * one-box case.  The setters and readers below address the box chosen with
+ * sdem_select_box (box 0 at creation); an ensemble's states and contacts are
+ * set before its first step.  Each box gives the same bits as a context of
+ * its own. */
+int sdem_create_ensemble(int32_t nbox, const sdem_options* opts /* nbox */, sdem_ctx** out);
+int sdem_get_num_boxes(sdem_ctx* c, int32_t* nbox);
+int sdem_select_box(sdem_ctx* c, int32_t box);
+
 /* InitPrtclProperty (Prtcl_Property.f90:62-143): per-type pure properties
  * and the particle-particle binary table, built with the reference's
  * row-type rule (pair (i,j) takes both partners' properties from type i,
@@ -227,6 +240,18 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
  * intact input after the storage has been enlarged.  Call before
  * sdem_set_state. */
 int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots);
+
+/* Block sizes of the contact and force kernels (64, 128 or 256; no
+ * counterpart in the reference, results do not depend on them). */
+int sdem_set_block_sizes(sdem_ctx* c, int32_t contact_block, int32_t force_block);
+int sdem_get_tuning(sdem_ctx* c, double cell_factor[3], double* skin_factor, int32_t* contact_block,
+                    int32_t* force_block);
+
+/* Autotuner (no counterpart in the reference): tries cell factors, skins
+ * and block sizes, each for steps_per_trial timed steps (plus two settling
+ * steps), and keeps the fastest.  The trials are real steps of the run
+ * (the results do not depend on the tuning); *steps_run says how many. */
+int sdem_autotune(sdem_ctx* c, int32_t steps_per_trial, int32_t* steps_run, double* best_ms_per_step);
 
 /* Memory the context holds on the device, in bytes. */
 int sdem_get_device_bytes(sdem_ctx* c, int64_t* bytes);

@@ -19,20 +19,12 @@
 #include "sdem_internal.h"
 
 namespace sdem {
-hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
-                      hipStream_t st);
-hipError_t launch_finish(Scalars* S, const Slot* slots, hipStream_t st);
-hipError_t launch_ctl(const Params& P, Scalars* S, Slot* slots, hipStream_t st);
-hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
-                       int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
-                       int* perm, double4* pk, hipStream_t st);
-hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
-                        const int* cstart, int* vcnt, int* vlist, hipStream_t st);
-hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
-                        int* ccnt, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
-                        hipEvent_t mid);
+hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st);
+hipError_t launch_ctl(const Dev& D, hipStream_t st);
+hipError_t launch_finish(const Dev& D, hipStream_t st);
+hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st);
+hipError_t launch_vlist(const Dev& D, hipStream_t st);
+hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
 }  // namespace sdem
 
@@ -188,59 +180,75 @@ int level_size(int m) { return m == SDEM_PIM_FE ? 1 : (m == SDEM_PIM_AB2 ? 2 : 3
 
 }  // namespace
 
-struct sdem_ctx {
-  sdem_options o{};
-  std::string err;
-  int tsize = 1, rsize = 1, ta = 1, tr = 1;
-  hipStream_t stream = nullptr;
 
+// ---------------------------------------------------------------------------
+// One box: an independent system (domain, materials, walls, particles).  A
+// plain context has one; an ensemble has many, stepped together.
+// ---------------------------------------------------------------------------
+struct Box {
+  sdem_options o{};
   int ntype = 0, nwtype = 0;
   std::vector<PureProp> pprop, wprop;
   std::vector<Bnry> pp_tab, pw_tab;
   bool have_pprop = false, have_wprop = false;
+  double lin_kn = 0.0, lin_kt_ratio = 0.0;  // constant-stiffness linear model
   std::vector<Wall> walls;
   int num_pwall_total = 0;
-
   std::vector<int> fix_id, fix_type;
   std::vector<double> fix_pos;
+  // the state as set (host layouts of sdem_set_state), and its contacts
+  bool have_state = false;
+  int n = 0;
+  std::vector<int> id, ptype, mark;
+  std::vector<double> posr, lin_vel, lin_acc, theta, rot_vel, rot_acc;
+  std::vector<int> ckind, cowner, cpartner;
+  std::vector<double> ctan, cvri;
+  // placement and search layout
+  int off = 0, bcap = 0;
+  std::vector<int> type_level;
+  std::vector<double> level_rmax, cells;
+  double host_max_a = 0.0;
+};
+
+struct sdem_ctx {
+  std::vector<Box> boxes;
+  int sel = 0;  // box that the setters and getters address
+  std::string err;
+  int tsize = 1, rsize = 1, ta = 1, tr = 1;
+  hipStream_t stream = nullptr;
+
+  // search tuning (results do not depend on it)
+  double cell_factor[3] = {1.0, 1.0, 1.0};
+  double skin_factor = 0.1;  // candidate-list skin as a fraction of a box's d_max
+  int keep_forces = 1;
+  int bs_contact = 256, bs_force = 256;
+  int hist_slots_hint = 0, list_slots_hint = 0;
 
   // device
-  int cap = 0, hcap = 0;
+  bool dev_ready = false;     // buffers allocated and the state uploaded
+  bool params_ready = false;  // box parameters, tables and grids uploaded
+  bool stepped = false, first_step_pending = false;
+  bool need_pre = false;      // the state in buffer A has no near-wall pass yet
+  int64_t nslots = 0;
+  int hcap = 0, lcap = 0;
   Buf buf[2]{};
   int cur = 0;
+  Dev D{};
   Scalars* S = nullptr;
-  int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr;
+  BoxState* BS = nullptr;
+  Slot* slots = nullptr;
+  BoxSlot* bslots = nullptr;
+  Params* d_params = nullptr;
+  int* d_blk_box = nullptr;
+  int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr, *tile_sum = nullptr;
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
   int *vcnt = nullptr, *vlist = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
-  Slot* slots = nullptr;
-  int lcap = 0;
-  double cell_factor[3] = {1.0, 1.0, 1.0};
-  double skin_factor = 0.1;  // candidate-list skin as a fraction of d_max
-  int keep_forces = 1;
-  bool first_step_pending = false;
-  double lin_kn = 0.0, lin_kt_ratio = 0.0;  // constant-stiffness linear model
-  bool need_pre = false;  // the state in buffer A has no near-wall pass yet
+  int64_t nkeys = 0;
   bool use_graph = true;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
-  int hist_slots_hint = 0;
-  int list_slots_hint = 0;
-  int* tile_sum = nullptr;
-  Bnry *d_pp = nullptr, *d_pw = nullptr;
-  double *d_mass = nullptr, *d_inertia = nullptr;
-  Wall* d_walls = nullptr;
-  int* d_type_level = nullptr;
-  double4* d_fpk = nullptr;
-  int *d_fid = nullptr, *d_ftype = nullptr, *d_fstart = nullptr;
-  Params P{};
-  int64_t dead_key = 0;
-  int n_upper = 0;
-  bool have_state = false, stepped = false, prepared = false;
-  std::vector<double> host_lacc1, host_racc1;  // level-1 accelerations before the first step
-  std::vector<double> level_rmax;              // per level, from the state
-  std::vector<int> type_level;
 
   bool prof = false;
   double prof_ms[SDEM_NUM_PHASES] = {};
@@ -248,6 +256,7 @@ struct sdem_ctx {
   std::vector<void*> allocs_state, allocs_grid, allocs_tab;
   int64_t dev_bytes = 0;
 
+  Box& box() { return boxes[sel]; }
   int fail(int code, const std::string& m) {
     err = m;
     return code;
@@ -285,32 +294,32 @@ void free_list(sdem_ctx* c, std::vector<void*>& list) {
   (void)c;
 }
 
-double max_type_radius(const sdem_ctx* c) {
-  double m = c->pprop[0].radius;
-  for (int i = 1; i < c->ntype; i++) m = fmx(m, c->pprop[i].radius);
+double max_type_radius(const Box& b) {
+  double m = b.pprop[0].radius;
+  for (int i = 1; i < b.ntype; i++) m = fmx(m, b.pprop[i].radius);
   return m;
 }
-double min_type_radius(const sdem_ctx* c) {
-  double m = c->pprop[0].radius;
-  for (int i = 1; i < c->ntype; i++) m = fmn(m, c->pprop[i].radius);
+double min_type_radius(const Box& b) {
+  double m = b.pprop[0].radius;
+  for (int i = 1; i < b.ntype; i++) m = fmn(m, b.pprop[i].radius);
   return m;
 }
 
 // Level of each type: one level for CS_Method=1, the reference's hierarchy
 // (Init_Munjiza_Hrchl, Prtcl_Hrchl_Munjiza.f90:84-237) for CS_Method=2.
 // Returns per-level cell lengths.
-std::vector<double> plan_levels(sdem_ctx* c, const std::vector<int>& count_per_type) {
+std::vector<double> plan_levels(Box& b, const std::vector<int>& count_per_type) {
   std::vector<double> cells;
-  c->type_level.assign(c->ntype, 0);
-  const double maxd = 2.0 * max_type_radius(c), mind = 2.0 * min_type_radius(c);
-  double cell_len = c->o.prtcl_cs_ratio * maxd;
-  if (c->o.cs_method != SDEM_CSM_NBS_MUNJIZA_HRCHL) {
+  b.type_level.assign(b.ntype, 0);
+  const double maxd = 2.0 * max_type_radius(b), mind = 2.0 * min_type_radius(b);
+  double cell_len = b.o.prtcl_cs_ratio * maxd;
+  if (b.o.cs_method != SDEM_CSM_NBS_MUNJIZA_HRCHL) {
     cells.push_back(cell_len);
     return cells;
   }
   int dmax_min = (int)(maxd / mind);
   int numlevels;
-  if (c->o.cs_numlvls <= 0) {
+  if (b.o.cs_numlvls <= 0) {
     if (dmax_min <= 1) numlevels = 1;
     else if (dmax_min <= 3) numlevels = 2;
     else if (dmax_min <= 7) numlevels = 3;
@@ -318,44 +327,44 @@ std::vector<double> plan_levels(sdem_ctx* c, const std::vector<int>& count_per_t
     else if (dmax_min <= 31) numlevels = 5;
     else numlevels = 6;
   } else {
-    numlevels = std::min(c->o.cs_numlvls, MAX_LEVELS);
+    numlevels = std::min(b.o.cs_numlvls, MAX_LEVELS);
   }
   double maxd_lvl = maxd;
   for (int idh = 1; idh <= numlevels; idh++) {
     double mind_lvl = maxd_lvl / 2.0;
     if (idh == numlevels) mind_lvl = 0.0;
     int num = 0;
-    for (int i = 0; i < c->ntype; i++) {
-      double diam = 2.0 * c->pprop[i].radius;
+    for (int i = 0; i < b.ntype; i++) {
+      double diam = 2.0 * b.pprop[i].radius;
       if (diam > mind_lvl && (idh == 1 || diam <= maxd_lvl)) num += count_per_type[i];
     }
     if (num > 0) {
       int lvl = (int)cells.size();
-      for (int i = 0; i < c->ntype; i++) {
-        double diam = 2.0 * c->pprop[i].radius;
-        if (diam > mind_lvl && (idh == 1 || diam <= maxd_lvl)) c->type_level[i] = lvl;
+      for (int i = 0; i < b.ntype; i++) {
+        double diam = 2.0 * b.pprop[i].radius;
+        if (diam > mind_lvl && (idh == 1 || diam <= maxd_lvl)) b.type_level[i] = lvl;
       }
       cells.push_back(cell_len);
     }
     maxd_lvl = mind_lvl;
     cell_len = cell_len / 2.0;
   }
-  if (cells.empty()) cells.push_back(c->o.prtcl_cs_ratio * maxd);
+  if (cells.empty()) cells.push_back(b.o.prtcl_cs_ratio * maxd);
   return cells;
 }
 
-int make_grid(sdem_ctx* c, double cell_len0, double rmax, int64_t key_base, Grid* G) {
+int make_grid(sdem_ctx* c, const Box& b, double cell_len0, double rmax, int64_t key_base, Grid* G) {
   G->key_base = key_base;
   G->rmax = rmax;
   G->ncell = 1;
   for (int d = 0; d < 3; d++) {
-    double L = c->o.domain_max[d] - c->o.domain_min[d];
+    double L = b.o.domain_max[d] - b.o.domain_min[d];
     double cell_len = cell_len0 * c->cell_factor[d];
-    G->origin[d] = c->o.domain_min[d];
-    if (c->o.is_periodic[d]) {
+    G->origin[d] = b.o.domain_min[d];
+    if (b.o.is_periodic[d]) {
       int n = (int)std::floor(L / cell_len);
       // the search range of one particle must not wrap onto itself
-      double reach = 2.0 * max_type_radius(c) + rmax;
+      double reach = 2.0 * max_type_radius(b) + rmax;
       int span = (int)std::floor(2.0 * reach * (1.0 + 1e-6) / (L / n)) + 2;
       if (n < 3 || span > n)
         return c->fail(SDEM_E_ARG, "periodic dimension too short for the search range");
@@ -375,7 +384,364 @@ int make_grid(sdem_ctx* c, double cell_len0, double rmax, int64_t key_base, Grid
   return SDEM_OK;
 }
 
+// history slots per particle: a kissing-number bound for the largest type
+// touched by the smallest, with room for soft overlaps and walls, at most 48
+// to start with (an overflow doubles it and replays the step)
+int history_capacity(const Box& b) {
+  double rmax = max_type_radius(b), rmin = min_type_radius(b);
+  double s = rmin / (rmax + rmin);
+  double cosv = std::sqrt(std::max(0.0, 1.0 - s * s));
+  double bound = 2.0 / (1.0 - cosv);
+  int h = (int)std::ceil(1.25 * bound) + 4 + (int)b.walls.size();
+  h = std::max(h, 2 * b.o.cntct_list_size);
+  h = std::max(h, 16);
+  return std::min(h, 48);
+}
+
+bool check_options(const sdem_options* o) {
+  if (o->pi_method < 1 || o->pi_method > 3 || o->pri_method < 1 || o->pri_method > 3 ||
+      o->cf_type < 1 || o->cf_type > 3 || o->ct_model < 1 || o->ct_model > 2 ||
+      o->cs_method < 1 || o->cs_method > 2 || o->num_prtcl < 1 || !(o->dt > 0.0) ||
+      !(o->prtcl_cs_ratio >= 1.0) || o->wall_max_update_iter < 1)
+    return false;
+  for (int d = 0; d < 3; d++)
+    if (!(o->domain_max[d] > o->domain_min[d])) return false;
+  return true;
+}
+
+int setup_device(sdem_ctx* c) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SDEM_E_NODEV;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SDEM_E_NODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return SDEM_E_NODEV;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SDEM_E_NODEV;
+  if (hipStreamCreate(&c->stream) != hipSuccess) return SDEM_E_HIP;
+  return SDEM_OK;
+}
+
+int round_up(int n, int a) { return (n + a - 1) / a * a; }
+
+template <class T>
+hipError_t dl(std::vector<T>& v, const T* src, size_t n) {
+  v.resize(n);
+  if (!n) return hipSuccess;
+  return hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
+}
+
+template <class T>
+hipError_t put_field(T* dst, const T& v) {
+  return hipMemcpy(dst, &v, sizeof(T), hipMemcpyHostToDevice);
+}
+
 }  // namespace
+
+// ===========================================================================
+// Device state: allocation and upload of every box's staged state
+// ===========================================================================
+static int alloc_buf(sdem_ctx* c, Buf& b) {
+  size_t cap = (size_t)c->nslots, hc = (size_t)c->hcap;
+  hipError_t e = hipSuccess;
+  auto A = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
+  A(&b.pos, cap);
+  A(&b.lv, cap * 3 * c->tsize); A(&b.la, cap * 3 * c->ta);
+  A(&b.th, cap * 3); A(&b.rv, cap * 3 * c->rsize); A(&b.ra, cap * 3 * c->tr);
+  A(&b.id, cap); A(&b.type, cap); A(&b.mark, cap); A(&b.wmask, cap);
+  A(&b.hcnt, cap); A(&b.hkey, cap * hc); A(&b.hval, cap * hc);
+  A(&b.force, cap * 6);
+  return e == hipSuccess ? SDEM_OK : hip_fail(c, e, "hipMalloc(state)");
+}
+
+static int upload_state(sdem_ctx* c) {
+  invalidate_graphs(c);
+  free_list(c, c->allocs_state);
+  // placement: every box owns a BOX_ALIGN-multiple of slots
+  int64_t off = 0;
+  int hc = 0;
+  for (auto& b : c->boxes) {
+    if (!b.have_state) return c->fail(SDEM_E_STATE, "every box needs a state before stepping");
+    b.off = (int)off;
+    b.bcap = round_up(std::max(b.n, 1), BOX_ALIGN);
+    off += b.bcap;
+    hc = std::max(hc, history_capacity(b));
+  }
+  if (off > INT32_MAX / 2) return c->fail(SDEM_E_CAPACITY, "too many particles for one context");
+  c->nslots = off;
+  const size_t cap = (size_t)off;
+  c->hcap = c->hist_slots_hint > 0 ? c->hist_slots_hint : hc;
+  c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
+  const int nbox = (int)c->boxes.size();
+  for (int k = 0; k < 2; k++)
+    if (int r = alloc_buf(c, c->buf[k])) return r;
+  hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
+  auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
+  M(&c->BS, nbox); M(&c->slots, NSLOT); M(&c->bslots, (size_t)nbox * NSLOT);
+  M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap);
+  M(&c->vcnt, cap); M(&c->ccnt, cap); M(&c->clist, cap * c->hcap); M(&c->vlist, cap * c->lcap);
+  M(&c->d_blk_box, cap / BOX_ALIGN);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+
+  const int ts = c->tsize, rs = c->rsize;
+  std::vector<double4> p4(cap, make_double4(0, 0, 0, 0));
+  std::vector<double> lv(cap * 3 * ts, 0.0), la(cap * 3 * c->ta, 0.0), th(cap * 3, 0.0);
+  std::vector<double> rv(cap * 3 * rs, 0.0), ra(cap * 3 * c->tr, 0.0);
+  std::vector<int> id(cap, 0), ty(cap, 0), mk(cap, 0), hcn(cap, 0), hk(cap * c->hcap, 0), blk(cap / BOX_ALIGN);
+  std::vector<double4> hv(cap * c->hcap, make_double4(0, 0, 0, 0));
+  std::vector<uint64_t> wm(cap, 0);
+  std::vector<BoxState> bs(nbox);
+  for (int bi = 0; bi < nbox; bi++) {
+    Box& b = c->boxes[bi];
+    for (int g = b.off / BOX_ALIGN; g < (b.off + b.bcap) / BOX_ALIGN; g++) blk[g] = bi;
+    double host_max_a = 0.0;
+    for (int i = 0; i < b.n; i++) {
+      const size_t s = (size_t)b.off + i;
+      const double* pr = &b.posr[4 * (size_t)i];
+      p4[s] = make_double4(pr[0], pr[1], pr[2], pr[3]);
+      id[s] = b.id[i]; ty[s] = b.ptype[i]; mk[s] = b.mark[i];
+      for (int l = 0; l < ts; l++)
+        for (int d = 0; d < 3; d++) lv[((size_t)l * 3 + d) * cap + s] = b.lin_vel[((size_t)i * ts + l) * 3 + d];
+      // stored acceleration levels: FE keeps level 1, AB2/AB3 keep levels 2..tsize
+      for (int l = 0; l < c->ta; l++) {
+        int src = ts == 1 ? 0 : l + 1;
+        for (int d = 0; d < 3; d++) la[((size_t)l * 3 + d) * cap + s] = b.lin_acc[((size_t)i * ts + src) * 3 + d];
+      }
+      for (int d = 0; d < 3; d++) th[(size_t)d * cap + s] = b.theta[3 * (size_t)i + d];
+      for (int l = 0; l < rs; l++)
+        for (int d = 0; d < 3; d++) rv[((size_t)l * 3 + d) * cap + s] = b.rot_vel[((size_t)i * rs + l) * 3 + d];
+      for (int l = 0; l < c->tr; l++) {
+        int src = rs == 1 ? 0 : l + 1;
+        for (int d = 0; d < 3; d++) ra[((size_t)l * 3 + d) * cap + s] = b.rot_acc[((size_t)i * rs + src) * 3 + d];
+      }
+      const double* a = &b.lin_acc[(size_t)i * ts * 3];
+      host_max_a = fmx(host_max_a, std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]));
+    }
+    b.host_max_a = host_max_a;
+    // history records (Add_RestartCntctlink-style install)
+    std::vector<std::pair<int, int>> idx;
+    idx.reserve(b.n);
+    for (int i = 0; i < b.n; i++) idx.push_back({b.id[i], i});
+    std::sort(idx.begin(), idx.end());
+    for (size_t k = 0; k < b.ckind.size(); k++) {
+      auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(b.cowner[k], -1));
+      if (it == idx.end() || it->first != b.cowner[k]) return c->fail(SDEM_E_ARG, "owner id not present");
+      const size_t s = (size_t)b.off + it->second;
+      if (hcn[s] >= c->hcap) return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
+      const size_t slot = (size_t)hcn[s]++ * cap + s;
+      hk[slot] = b.cpartner[k] | (b.ckind[k] << KEY_BITS);
+      hv[slot] = make_double4(b.ctan[3 * k], b.ctan[3 * k + 1], b.ctan[3 * k + 2], b.cvri[k]);
+    }
+    BoxState& B = bs[bi];
+    std::memset(&B, 0, sizeof(B));
+    B.n_stored = b.n;
+    B.n_alive = b.n;
+    B.next_wall_update = 0;
+    B.use_host_max_a = 1;
+    B.host_max_a = host_max_a;
+  }
+  Buf& A = c->buf[0];
+  auto H = [&](void* dst, const void* src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  H(A.pos, p4.data(), cap * sizeof(double4));
+  H(A.lv, lv.data(), lv.size() * 8); H(A.la, la.data(), la.size() * 8); H(A.th, th.data(), th.size() * 8);
+  H(A.rv, rv.data(), rv.size() * 8); H(A.ra, ra.data(), ra.size() * 8);
+  H(A.id, id.data(), cap * 4); H(A.type, ty.data(), cap * 4); H(A.mark, mk.data(), cap * 4);
+  H(A.wmask, wm.data(), cap * 8); H(A.hcnt, hcn.data(), cap * 4);
+  H(A.hkey, hk.data(), hk.size() * 4); H(A.hval, hv.data(), hv.size() * sizeof(double4));
+  H(c->d_blk_box, blk.data(), blk.size() * 4);
+  H(c->BS, bs.data(), bs.size() * sizeof(BoxState));
+  Scalars s;
+  std::memset(&s, 0, sizeof(s));
+  s.iter = c->boxes[0].o.ifirst - 1;
+  s.need_rebuild = 1;
+  H(c->S, &s, sizeof(s));
+  if (e == hipSuccess) e = hipMemset(c->slots, 0, sizeof(Slot) * NSLOT);
+  if (e == hipSuccess) e = hipMemset(c->bslots, 0, sizeof(BoxSlot) * NSLOT * nbox);
+  if (e != hipSuccess) return hip_fail(c, e, "upload state");
+  c->cur = 0;
+  c->dev_ready = true;
+  c->params_ready = false;
+  c->stepped = false;
+  c->first_step_pending = true;
+  c->need_pre = true;
+  return SDEM_OK;
+}
+
+// Box parameters, pair tables, walls and search grids for the current tuning.
+static int upload_params(sdem_ctx* c) {
+  invalidate_graphs(c);
+  free_list(c, c->allocs_grid);
+  free_list(c, c->allocs_tab);
+  const int nbox = (int)c->boxes.size();
+  std::vector<Params> P(nbox);
+  std::vector<Bnry> pp, pw;
+  std::vector<double> mass, inertia;
+  std::vector<Wall> walls;
+  std::vector<int> tlev;
+  std::vector<size_t> o_pp(nbox), o_pw(nbox), o_m(nbox), o_w(nbox), o_t(nbox);
+  int64_t key = 0;
+  double rebuild_disp = 1e300;
+  Grid fgrid{};
+  int nfix = 0;
+  for (int bi = 0; bi < nbox; bi++) {
+    Box& b = c->boxes[bi];
+    Params& p = P[bi];
+    std::memset(&p, 0, sizeof(p));
+    p.dt = b.o.dt;
+    for (int d = 0; d < 3; d++) {
+      p.g[d] = b.o.gravity[d];
+      p.dmin[d] = b.o.domain_min[d];
+      p.dmax[d] = b.o.domain_max[d];
+      p.simlen[d] = b.o.domain_max[d] - b.o.domain_min[d];
+      p.periodic[d] = b.o.is_periodic[d] ? 1 : 0;
+    }
+    const double maxd = 2.0 * max_type_radius(b);
+    for (int d = 0; d < 3; d++)
+      if (1.02 * b.o.prtcl_cs_ratio * maxd > p.simlen[d]) return c->fail(SDEM_E_ARG, "so big Diameter");
+    p.cf_type = b.o.cf_type;
+    p.ct_model = b.o.ct_model;
+    p.pi_method = b.o.pi_method;
+    p.pri_method = b.o.pri_method;
+    p.tsize = c->tsize; p.rsize = c->rsize; p.ta = c->ta; p.tr = c->tr;
+    p.cap = (int)c->nslots;
+    p.hcap = c->hcap;
+    p.lcap = c->lcap;
+    p.ntype = b.ntype;
+    p.nwtype = b.nwtype;
+    p.nwall = (int)b.walls.size();
+    p.search_fixed = b.o.cs_method == SDEM_CSM_NBS_MUNJIZA ? 1 : 0;
+    p.keep_forces = c->keep_forces;
+    const double rmax = max_type_radius(b);
+    p.dxmax1 = rmax * b.o.wall_neighbor_ratio;
+    p.dxmax2 = rmax * (b.o.wall_neighbor_ratio + 1.0);
+    double mwv = 0.0;
+    for (auto& w : b.walls) mwv = fmx(mwv, vnorm(v3(w.trans_vel)));
+    p.max_wall_vel = mwv;
+    p.wall_max_update_iter = b.o.wall_max_update_iter;
+    p.skin = c->skin_factor * maxd;
+    rebuild_disp = fmn(rebuild_disp, 0.5 * p.skin * (1.0 - 1.0e-6));
+    for (auto& w : b.walls)
+      if (w.wall_type > b.nwtype) return c->fail(SDEM_E_ARG, "wall_type exceeds numWall_type");
+    // search levels: radii never change, so the per-level largest radius is
+    // fixed by the state
+    std::vector<int> cnt(b.ntype, 0);
+    for (int i = 0; i < b.n; i++) cnt[b.ptype[i] - 1]++;
+    b.cells = plan_levels(b, cnt);
+    b.level_rmax.assign(b.cells.size(), 0.0);
+    for (int i = 0; i < b.n; i++) {
+      int L = b.type_level[b.ptype[i] - 1];
+      b.level_rmax[L] = fmx(b.level_rmax[L], b.posr[4 * (size_t)i + 3]);
+    }
+    p.nlvl = (int)b.cells.size();
+    for (int L = 0; L < p.nlvl; L++) {
+      if (int rc = make_grid(c, b, b.cells[L], b.level_rmax[L], key, &p.lvl[L])) return rc;
+      key += p.lvl[L].ncell;
+    }
+    p.dead_key = (int)key;
+    key += 1;
+    if (key > INT32_MAX / 2) return c->fail(SDEM_E_ARG, "search grids too large");
+    p.box = bi;
+    p.off = b.off;
+    p.bcap = b.bcap;
+    // tables, concatenated over the boxes
+    o_pp[bi] = pp.size(); pp.insert(pp.end(), b.pp_tab.begin(), b.pp_tab.end());
+    o_pw[bi] = pw.size(); pw.insert(pw.end(), b.pw_tab.begin(), b.pw_tab.end());
+    o_m[bi] = mass.size();
+    for (int i = 0; i < b.ntype; i++) { mass.push_back(b.pprop[i].mass); inertia.push_back(b.pprop[i].inertia); }
+    o_w[bi] = walls.size(); walls.insert(walls.end(), b.walls.begin(), b.walls.end());
+    o_t[bi] = tlev.size(); tlev.insert(tlev.end(), b.type_level.begin(), b.type_level.end());
+    // fixed particles: single-box contexts only (checked at set time)
+    if (!b.fix_id.empty()) {
+      nfix = (int)b.fix_id.size();
+      double fr = 0.0;
+      for (int i = 0; i < nfix; i++) fr = fmx(fr, b.fix_pos[4 * i + 3]);
+      if (int rc = make_grid(c, b, b.o.prtcl_cs_ratio * maxd, fr, 0, &fgrid)) return rc;
+    }
+    p.nfix = b.fix_id.empty() ? 0 : nfix;
+  }
+  c->nkeys = key;
+  hipError_t e = hipSuccess;
+  auto U = [&](auto** dst, const auto& vec) {
+    using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+    if (e == hipSuccess) e = dalloc(c, c->allocs_tab, dst, vec.size());
+    if (e == hipSuccess && !vec.empty()) e = hipMemcpy(*dst, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice);
+  };
+  Bnry *d_pp = nullptr, *d_pw = nullptr;
+  double *d_mass = nullptr, *d_inertia = nullptr;
+  Wall* d_walls = nullptr;
+  int* d_tlev = nullptr;
+  U(&d_pp, pp); U(&d_pw, pw); U(&d_mass, mass); U(&d_inertia, inertia); U(&d_walls, walls); U(&d_tlev, tlev);
+  // fixed particles sorted by the cells of their grid
+  double4* d_fpk = nullptr;
+  int *d_fid = nullptr, *d_ftype = nullptr, *d_fstart = nullptr;
+  {
+    const Box& b = c->boxes[0];
+    const Grid& G = fgrid;
+    std::vector<int64_t> fkey(nfix);
+    auto cl = [&](int d, double x) {
+      int cc = (int)std::floor((x - G.origin[d]) * G.inv_w[d]);
+      return cc < 0 ? 0 : (cc >= G.n[d] ? G.n[d] - 1 : cc);
+    };
+    for (int i = 0; i < nfix; i++) {
+      const double* p = &b.fix_pos[4 * i];
+      fkey[i] = ((int64_t)cl(2, p[2]) * G.n[1] + cl(1, p[1])) * G.n[0] + cl(0, p[0]);
+    }
+    std::vector<int> ord(nfix);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return fkey[x] < fkey[y]; });
+    std::vector<double4> fpk(nfix);
+    std::vector<int> fid(nfix), ftype(nfix), fstart(nfix > 0 ? G.ncell + 1 : 1, 0);
+    for (int k = 0; k < nfix; k++) {
+      int i = ord[k];
+      const double* p = &b.fix_pos[4 * i];
+      fpk[k] = make_double4(p[0], p[1], p[2], p[3]);
+      fid[k] = b.fix_id[i];
+      ftype[k] = b.fix_type[i];
+    }
+    if (nfix > 0) {
+      for (int i = 0; i < nfix; i++) fstart[fkey[i] + 1]++;
+      for (int64_t q = 0; q < G.ncell; q++) fstart[q + 1] += fstart[q];
+    }
+    U(&d_fpk, fpk); U(&d_fid, fid); U(&d_ftype, ftype); U(&d_fstart, fstart);
+  }
+  for (int bi = 0; bi < nbox; bi++) {
+    Params& p = P[bi];
+    p.pp = d_pp + o_pp[bi];
+    p.pw = d_pw + o_pw[bi];
+    p.mass = d_mass + o_m[bi];
+    p.inertia = d_inertia + o_m[bi];
+    p.walls = d_walls + o_w[bi];
+    p.type_level = d_tlev + o_t[bi];
+    p.fgrid = fgrid;
+  }
+  U(&c->d_params, P);
+  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->counts, (size_t)key + 1);
+  if (e == hipSuccess) e = hipMemset(c->counts, 0, ((size_t)key + 1) * sizeof(int));
+  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)key + 1);
+  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)scan_tiles(key + 1) + 1);
+  if (e == hipSuccess) e = put_field(&c->S->rebuild_disp, rebuild_disp);
+  if (e == hipSuccess) e = put_field(&c->S->need_rebuild, 1);  // grids or skin may have changed
+  if (e != hipSuccess) return hip_fail(c, e, "upload parameters");
+  Dev& D = c->D;
+  D.PB = c->d_params; D.blk_box = c->d_blk_box; D.nbox = nbox; D.nslots = c->nslots; D.nkeys = key;
+  D.S = c->S; D.BS = c->BS; D.slots = c->slots; D.bslots = c->bslots;
+  D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
+  D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.vcnt = c->vcnt; D.vlist = c->vlist;
+  D.clist = c->clist; D.ccnt = c->ccnt;
+  D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
+  D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
+  c->params_ready = true;
+  return SDEM_OK;
+}
+
+static int prepare(sdem_ctx* c) {
+  if (!c->dev_ready)
+    if (int rc = upload_state(c)) return rc;
+  if (!c->params_ready)
+    if (int rc = upload_params(c)) return rc;
+  return SDEM_OK;
+}
 
 // ===========================================================================
 // The entry points keep C linkage from their declarations in sdem.h.
@@ -401,44 +767,54 @@ int sdem_options_default(sdem_options* o) {
   return SDEM_OK;
 }
 
-int sdem_create(const sdem_options* o, sdem_ctx** out) {
-  if (!o || !out) return SDEM_E_ARG;
+int sdem_create_ensemble(int32_t nbox, const sdem_options* opts, sdem_ctx** out) {
+  if (!out) return SDEM_E_ARG;
   *out = nullptr;
-  if (o->pi_method < 1 || o->pi_method > 3 || o->pri_method < 1 || o->pri_method > 3 ||
-      o->cf_type < 1 || o->cf_type > 3 || o->ct_model < 1 || o->ct_model > 2 ||
-      o->cs_method < 1 || o->cs_method > 2 || o->num_prtcl < 1 || !(o->dt > 0.0) ||
-      !(o->prtcl_cs_ratio >= 1.0) || o->wall_max_update_iter < 1)
-    return SDEM_E_ARG;
-  for (int d = 0; d < 3; d++)
-    if (!(o->domain_max[d] > o->domain_min[d])) return SDEM_E_ARG;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SDEM_E_NODEV;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return SDEM_E_NODEV;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return SDEM_E_NODEV;
-  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SDEM_E_NODEV;
+  if (nbox < 1 || !opts) return SDEM_E_ARG;
+  for (int b = 0; b < nbox; b++) {
+    if (!check_options(&opts[b])) return SDEM_E_ARG;
+    if (opts[b].pi_method != opts[0].pi_method || opts[b].pri_method != opts[0].pri_method ||
+        opts[b].ifirst != opts[0].ifirst)
+      return SDEM_E_ARG;  // one state layout and one iterNumber for all boxes
+  }
   sdem_ctx* c = new sdem_ctx();
-  c->o = *o;
-  c->tsize = level_size(o->pi_method);
-  c->rsize = level_size(o->pri_method);
+  if (int rc = setup_device(c)) {
+    delete c;
+    return rc;
+  }
+  c->boxes.resize(nbox);
+  for (int b = 0; b < nbox; b++) c->boxes[b].o = opts[b];
+  c->tsize = level_size(opts[0].pi_method);
+  c->rsize = level_size(opts[0].pri_method);
   c->ta = c->tsize == 1 ? 1 : c->tsize - 1;
   c->tr = c->rsize == 1 ? 1 : c->rsize - 1;
-  if (hipStreamCreate(&c->stream) != hipSuccess) {
-    delete c;
-    return SDEM_E_HIP;
-  }
   *out = c;
+  return SDEM_OK;
+}
+
+int sdem_create(const sdem_options* o, sdem_ctx** out) { return sdem_create_ensemble(1, o, out); }
+
+int sdem_get_num_boxes(sdem_ctx* c, int32_t* nbox) {
+  if (!c || !nbox) return SDEM_E_ARG;
+  *nbox = (int32_t)c->boxes.size();
+  return SDEM_OK;
+}
+
+int sdem_select_box(sdem_ctx* c, int32_t box) {
+  if (!c) return SDEM_E_ARG;
+  if (box < 0 || box >= (int)c->boxes.size()) return c->fail(SDEM_E_ARG, "box index out of range");
+  c->sel = box;
   return SDEM_OK;
 }
 
 int sdem_set_linear_stiffness(sdem_ctx* c, double kn, double kt_over_kn) {
   if (!c) return SDEM_E_ARG;
-  if (c->o.cf_type != SDEM_CFT_LINEAR) return c->fail(SDEM_E_STATE, "cf_type is not the linear model");
+  Box& b = c->box();
+  if (b.o.cf_type != SDEM_CFT_LINEAR) return c->fail(SDEM_E_STATE, "cf_type is not the linear model");
   if (!(kn > 0.0) || !(kt_over_kn >= 0.0)) return c->fail(SDEM_E_ARG, "bad linear stiffness");
-  if (c->have_pprop) return c->fail(SDEM_E_STATE, "stiffness must be set before the properties");
-  c->lin_kn = kn;
-  c->lin_kt_ratio = kt_over_kn;
+  if (b.have_pprop) return c->fail(SDEM_E_STATE, "stiffness must be set before the properties");
+  b.lin_kn = kn;
+  b.lin_kt_ratio = kt_over_kn;
   return SDEM_OK;
 }
 
@@ -447,16 +823,17 @@ int sdem_set_particle_property(sdem_ctx* c, int32_t ntype, const double* diamete
                                double fs, double fk, double froll, double en, double et) {
   (void)et;
   if (!c) return SDEM_E_ARG;
+  Box& b = c->box();
   if (ntype < 1 || ntype > MAX_TYPES || !diameter || !density || !youngs || !poisson)
     return c->fail(SDEM_E_ARG, "bad particle property arguments");
-  if (c->have_state) return c->fail(SDEM_E_STATE, "properties must be set before the state");
-  if (c->o.cf_type == SDEM_CFT_LINEAR && !(c->lin_kn > 0.0))
+  if (b.have_state) return c->fail(SDEM_E_STATE, "properties must be set before the state");
+  if (b.o.cf_type == SDEM_CFT_LINEAR && !(b.lin_kn > 0.0))
     return c->fail(SDEM_E_STATE, "linear model: set the stiffness first");
   const double PI = 3.141592653589793238462643383279502884;
-  c->ntype = ntype;
-  c->pprop.assign(ntype, PureProp{});
+  b.ntype = ntype;
+  b.pprop.assign(ntype, PureProp{});
   for (int i = 0; i < ntype; i++) {
-    PureProp& p = c->pprop[i];
+    PureProp& p = b.pprop[i];
     p.density = density[i];
     p.radius = 0.5 * diameter[i];
     p.youngs = youngs[i];
@@ -466,16 +843,16 @@ int sdem_set_particle_property(sdem_ctx* c, int32_t ntype, const double* diamete
     p.inertia = 2.0 / 5.0 * p.mass * (p.radius * p.radius);
     if (!(p.radius > 0.0) || !(p.mass > 0.0)) return c->fail(SDEM_E_ARG, "non-positive diameter or density");
   }
-  c->pp_tab.assign((size_t)ntype * ntype, Bnry{});
+  b.pp_tab.assign((size_t)ntype * ntype, Bnry{});
   for (int i = 0; i < ntype; i++)
-    for (int j = 0; j < ntype; j++)  // pari = parj = type i (Prtcl_Property.f90:137-138)
+    for (int j = 0; j < ntype; j++)
       // the reference's table takes the row type only (Prtcl_Property.f90:137-138);
       // the linear model, which it does not have, takes the true pair
-      c->pp_tab[(size_t)i * ntype + j] =
-          bnry_prop(c->o.cf_type, c->pprop[i], c->o.cf_type == SDEM_CFT_LINEAR ? c->pprop[j] : c->pprop[i],
-                    fs, fk, froll, en, false, c->lin_kn, c->lin_kt_ratio);
-  c->have_pprop = true;
-  c->have_wprop = false;
+      b.pp_tab[(size_t)i * ntype + j] =
+          bnry_prop(b.o.cf_type, b.pprop[i], b.o.cf_type == SDEM_CFT_LINEAR ? b.pprop[j] : b.pprop[i],
+                    fs, fk, froll, en, false, b.lin_kn, b.lin_kt_ratio);
+  b.have_pprop = true;
+  b.have_wprop = false;
   return SDEM_OK;
 }
 
@@ -483,22 +860,23 @@ int sdem_set_wall_property(sdem_ctx* c, int32_t nwtype, const double* youngs, co
                            double fs, double fk, double froll, double en, double et) {
   (void)et;
   if (!c) return SDEM_E_ARG;
-  if (!c->have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
+  Box& b = c->box();
+  if (!b.have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
   if (nwtype < 1 || !youngs || !poisson) return c->fail(SDEM_E_ARG, "bad wall property arguments");
-  if (c->have_state) return c->fail(SDEM_E_STATE, "properties must be set before the state");
-  c->nwtype = nwtype;
-  c->wprop.assign(nwtype, PureProp{});
+  if (b.have_state) return c->fail(SDEM_E_STATE, "properties must be set before the state");
+  b.nwtype = nwtype;
+  b.wprop.assign(nwtype, PureProp{});
   for (int i = 0; i < nwtype; i++) {
-    PureProp& w = c->wprop[i];
+    PureProp& w = b.wprop[i];
     w.density = 1.0e50; w.radius = 1.0e50; w.youngs = youngs[i]; w.poisson = poisson[i];
     w.mass = 1.0e50; w.volume = 1.0e50; w.inertia = 1.0e50;
   }
-  c->pw_tab.assign((size_t)c->ntype * nwtype, Bnry{});
-  for (int i = 0; i < c->ntype; i++)
+  b.pw_tab.assign((size_t)b.ntype * nwtype, Bnry{});
+  for (int i = 0; i < b.ntype; i++)
     for (int j = 0; j < nwtype; j++)
-      c->pw_tab[(size_t)i * nwtype + j] =
-          bnry_prop(c->o.cf_type, c->pprop[i], c->wprop[j], fs, fk, froll, en, true, c->lin_kn, c->lin_kt_ratio);
-  c->have_wprop = true;
+      b.pw_tab[(size_t)i * nwtype + j] =
+          bnry_prop(b.o.cf_type, b.pprop[i], b.wprop[j], fs, fk, froll, en, true, b.lin_kn, b.lin_kt_ratio);
+  b.have_wprop = true;
   return SDEM_OK;
 }
 
@@ -506,8 +884,9 @@ int sdem_add_plane_wall(sdem_ctx* c, const double p1[3], const double p2[3], con
                         const double p4[3], int32_t user_id, int32_t wall_type, int32_t both_side,
                         int32_t is_infinite, const double trans_vel[3], int32_t* wall_id) {
   if (!c || !p1 || !p2 || !p3 || !p4) return SDEM_E_ARG;
-  if (!c->have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
-  if (c->have_state) return c->fail(SDEM_E_STATE, "walls must be added before the state");
+  Box& b = c->box();
+  if (!b.have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
+  if (b.have_state) return c->fail(SDEM_E_STATE, "walls must be added before the state");
   (void)user_id;
   Wall w;
   std::memset(&w, 0, sizeof(w));
@@ -527,16 +906,16 @@ int sdem_add_plane_wall(sdem_ctx* c, const double p1[3], const double p2[3], con
   if (wall_type < 1) return c->fail(SDEM_E_ARG, "wall_type must be >= 1");
   if (std::fabs(vdot(n, P4) + w.d) >= 0.00001) return c->fail(SDEM_E_ARG, "Cannot create a plane wall");
   if (wall_id) *wall_id = 0;
-  double lenexp = 1.2 * max_type_radius(c);
-  V3 dmin = v3(c->o.domain_min), dmax = v3(c->o.domain_max);
+  double lenexp = 1.2 * max_type_radius(b);
+  V3 dmin = v3(b.o.domain_min), dmax = v3(b.o.domain_max);
   V3 one = {1.0, 1.0, 1.0};
   V3 pmin = vsub(dmin, vmul(one, lenexp)), pmax = vadd(dmax, vmul(one, lenexp));
   if (!wall_in_domain(w, pmin, pmax)) return SDEM_OK;  // skipped, as G_add_PlaneWall does
-  if ((int)c->walls.size() >= MAX_WALLS) return c->fail(SDEM_E_CAPACITY, "more than 64 walls");
-  c->num_pwall_total++;
-  w.wall_id = c->num_pwall_total + c->o.base_wall_id;
+  if ((int)b.walls.size() >= MAX_WALLS) return c->fail(SDEM_E_CAPACITY, "more than 64 walls");
+  b.num_pwall_total++;
+  w.wall_id = b.num_pwall_total + b.o.base_wall_id;
   if (w.wall_id > KEY_MASK) return c->fail(SDEM_E_ARG, "wall id exceeds 2^29-1");
-  c->walls.push_back(w);
+  b.walls.push_back(w);
   if (wall_id) *wall_id = w.wall_id;
   return SDEM_OK;
 }
@@ -544,43 +923,29 @@ int sdem_add_plane_wall(sdem_ctx* c, const double p1[3], const double p2[3], con
 int sdem_set_fixed_particles(sdem_ctx* c, int32_t nfix, const int32_t* id, const int32_t* ptype,
                              const double* posr) {
   if (!c || nfix < 0 || (nfix > 0 && (!id || !ptype || !posr))) return SDEM_E_ARG;
-  if (!c->have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
-  if (c->have_state) return c->fail(SDEM_E_STATE, "fixed particles must be set before the state");
-  c->fix_id.assign(id, id + nfix);
-  c->fix_type.assign(ptype, ptype + nfix);
-  c->fix_pos.assign(posr, posr + 4 * (size_t)nfix);
+  Box& b = c->box();
+  if (!b.have_pprop) return c->fail(SDEM_E_STATE, "particle property first");
+  if (b.have_state) return c->fail(SDEM_E_STATE, "fixed particles must be set before the state");
+  if (nfix > 0 && c->boxes.size() > 1) return c->fail(SDEM_E_ARG, "fixed particles need a single-box context");
   for (int i = 0; i < nfix; i++) {
-    if (ptype[i] < 1 || ptype[i] > c->ntype) return c->fail(SDEM_E_ARG, "fixed ptype out of range");
+    if (ptype[i] < 1 || ptype[i] > b.ntype) return c->fail(SDEM_E_ARG, "fixed ptype out of range");
     if (id[i] < 0 || id[i] > KEY_MASK) return c->fail(SDEM_E_ARG, "fixed id out of range");
   }
+  b.fix_id.assign(id, id + nfix);
+  b.fix_type.assign(ptype, ptype + nfix);
+  b.fix_pos.assign(posr, posr + 4 * (size_t)nfix);
   return SDEM_OK;
 }
 
-static int alloc_buf(sdem_ctx* c, Buf& b) {
-  size_t cap = (size_t)c->cap, hc = (size_t)c->hcap;
-  hipError_t e = hipSuccess;
-  auto A = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
-  A(&b.pos, cap);
-  A(&b.lv, cap * 3 * c->tsize); A(&b.la, cap * 3 * c->ta);
-  A(&b.th, cap * 3); A(&b.rv, cap * 3 * c->rsize); A(&b.ra, cap * 3 * c->tr);
-  A(&b.id, cap); A(&b.type, cap); A(&b.mark, cap); A(&b.wmask, cap);
-  A(&b.hcnt, cap); A(&b.hkey, cap * hc); A(&b.hval, cap * hc);
-  A(&b.force, cap * 6);
-  return e == hipSuccess ? SDEM_OK : hip_fail(c, e, "hipMalloc(state)");
-}
-
-// history slots per particle: a kissing-number bound for the largest type
-// touched by the smallest, with room for soft overlaps and walls, at most 48
-// to start with (an overflow doubles it and replays the step)
-static int history_capacity(const sdem_ctx* c) {
-  double rmax = max_type_radius(c), rmin = min_type_radius(c);
-  double s = rmin / (rmax + rmin);
-  double cosv = std::sqrt(std::max(0.0, 1.0 - s * s));
-  double bound = 2.0 / (1.0 - cosv);
-  int h = (int)std::ceil(1.25 * bound) + 4 + (int)c->walls.size();
-  h = std::max(h, 2 * c->o.cntct_list_size);
-  h = std::max(h, 16);
-  return std::min(h, 48);
+// a new state or new contacts of one box
+static int restage(sdem_ctx* c) {
+  if (c->stepped && c->boxes.size() > 1)
+    return c->fail(SDEM_E_STATE, "an ensemble's states and contacts are set before its first step");
+  invalidate_graphs(c);
+  c->dev_ready = false;
+  c->params_ready = false;
+  c->stepped = false;
+  return SDEM_OK;
 }
 
 int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* ptype,
@@ -588,131 +953,36 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
                    const double* lin_acc, const double* theta, const double* rot_vel,
                    const double* rot_acc) {
   if (!c) return SDEM_E_ARG;
-  if (!c->have_pprop || !c->have_wprop) return c->fail(SDEM_E_STATE, "particle and wall properties first");
+  Box& b = c->box();
+  if (!b.have_pprop || !b.have_wprop) return c->fail(SDEM_E_STATE, "particle and wall properties first");
   if (n < 0 || (n > 0 && (!id || !ptype || !posr || !lin_vel || !lin_acc || !theta || !rot_vel || !rot_acc)))
     return c->fail(SDEM_E_ARG, "bad state arguments");
-  if (n > c->o.num_prtcl) return c->fail(SDEM_E_CAPACITY, "n exceeds numPrtcl");
+  if (n > b.o.num_prtcl) return c->fail(SDEM_E_CAPACITY, "n exceeds numPrtcl");
   for (int i = 0; i < n; i++) {
-    if (ptype[i] < 1 || ptype[i] > c->ntype) return c->fail(SDEM_E_ARG, "ptype out of range");
+    if (ptype[i] < 1 || ptype[i] > b.ntype) return c->fail(SDEM_E_ARG, "ptype out of range");
     if (id[i] < 0 || id[i] > KEY_MASK) return c->fail(SDEM_E_ARG, "particle id out of range");
     if (!(posr[4 * i + 3] > 0.0)) return c->fail(SDEM_E_ARG, "non-positive radius");
     for (int d = 0; d < 3; d++) {
       double p = posr[4 * i + d];
-      if (!(p >= c->o.domain_min[d] && p < c->o.domain_max[d]))
+      if (!(p >= b.o.domain_min[d] && p < b.o.domain_max[d]))
         return c->fail(SDEM_E_ARG, "particle outside the domain");
     }
   }
-  // (re)allocate state buffers on first use
-  if (c->allocs_state.empty()) {
-    c->cap = c->o.num_prtcl;
-    c->hcap = c->hist_slots_hint > 0 ? c->hist_slots_hint : history_capacity(c);
-    for (int b = 0; b < 2; b++)
-      if (int r = alloc_buf(c, c->buf[b])) return r;
-    hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->keys, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->rank, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm_tmp, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->perm, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->pk, c->cap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vcnt, (size_t)c->cap + 2);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->clist, (size_t)c->cap * c->hcap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->ccnt, c->cap);
-    c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->vlist, (size_t)c->cap * c->lcap);
-    if (e == hipSuccess) e = dalloc(c, c->allocs_state, &c->slots, NSLOT);
-    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(sort)");
-  }
-  const int cap = c->cap, ts = c->tsize, rs = c->rsize;
-  // SoA transposition on the host
-  std::vector<double> x(n), y(n), z(n), r(n), lv((size_t)3 * ts * cap, 0.0), la((size_t)3 * c->ta * cap, 0.0);
-  std::vector<double> th((size_t)3 * cap, 0.0), rv((size_t)3 * rs * cap, 0.0), ra((size_t)3 * c->tr * cap, 0.0);
-  std::vector<int> mark(n, 1), hc(n, 0);
-  std::vector<uint64_t> wm(n, 0);
-  c->host_lacc1.assign((size_t)3 * n, 0.0);
-  c->host_racc1.assign((size_t)3 * n, 0.0);
-  double host_max_a = 0.0;
-  for (int i = 0; i < n; i++) {
-    x[i] = posr[4 * i]; y[i] = posr[4 * i + 1]; z[i] = posr[4 * i + 2]; r[i] = posr[4 * i + 3];
-    if (usr_mark) mark[i] = usr_mark[i];
-    for (int l = 0; l < ts; l++)
-      for (int d = 0; d < 3; d++) lv[((size_t)l * 3 + d) * cap + i] = lin_vel[((size_t)i * ts + l) * 3 + d];
-    // stored acceleration levels: FE keeps level 1, AB2/AB3 keep levels 2..tsize
-    for (int l = 0; l < c->ta; l++) {
-      int src = ts == 1 ? 0 : l + 1;
-      for (int d = 0; d < 3; d++) la[((size_t)l * 3 + d) * cap + i] = lin_acc[((size_t)i * ts + src) * 3 + d];
-    }
-    for (int d = 0; d < 3; d++) th[(size_t)d * cap + i] = theta[3 * i + d];
-    for (int l = 0; l < rs; l++)
-      for (int d = 0; d < 3; d++) rv[((size_t)l * 3 + d) * cap + i] = rot_vel[((size_t)i * rs + l) * 3 + d];
-    for (int l = 0; l < c->tr; l++) {
-      int src = rs == 1 ? 0 : l + 1;
-      for (int d = 0; d < 3; d++) ra[((size_t)l * 3 + d) * cap + i] = rot_acc[((size_t)i * rs + src) * 3 + d];
-    }
-    for (int d = 0; d < 3; d++) {
-      c->host_lacc1[3 * i + d] = lin_acc[((size_t)i * ts) * 3 + d];
-      c->host_racc1[3 * i + d] = rot_acc[((size_t)i * rs) * 3 + d];
-    }
-    const double* a = lin_acc + (size_t)i * ts * 3;
-    host_max_a = fmx(host_max_a, std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]));
-  }
-  Buf& A = c->buf[0];
-  c->cur = 0;
-  hipError_t e = hipSuccess;
-  auto H = [&](void* dst, const void* src, size_t bytes) {
-    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
-  };
-  std::vector<double4> p4(n);
-  for (int i = 0; i < n; i++) p4[i] = make_double4(x[i], y[i], z[i], r[i]);
-  H(A.pos, p4.data(), (size_t)n * sizeof(double4));
-  H(A.lv, lv.data(), lv.size() * 8); H(A.la, la.data(), la.size() * 8); H(A.th, th.data(), th.size() * 8);
-  H(A.rv, rv.data(), rv.size() * 8); H(A.ra, ra.data(), ra.size() * 8);
-  H(A.id, id, n * 4); H(A.type, ptype, n * 4); H(A.mark, mark.data(), n * 4);
-  H(A.wmask, wm.data(), n * 8); H(A.hcnt, hc.data(), n * 4);
-  Scalars s;
-  std::memset(&s, 0, sizeof(s));
-  s.n_stored = n;
-  s.n_alive = n;
-  s.iter = c->o.ifirst - 1;
-  s.next_wall_update = 0;
-  s.use_host_max_a = 1;
-  s.host_max_a = host_max_a;
-  s.need_rebuild = 1;
-  H(c->S, &s, sizeof(s));
-  if (e == hipSuccess) e = hipMemset(c->slots, 0, sizeof(Slot) * NSLOT);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
-  // search levels: radii never change, so the per-level largest radius is
-  // fixed by the state
-  std::vector<int> cnt(c->ntype, 0);
-  for (int i = 0; i < n; i++) cnt[ptype[i] - 1]++;
-  std::vector<double> cells = plan_levels(c, cnt);
-  c->level_rmax.assign(cells.size(), 0.0);
-  for (int i = 0; i < n; i++) {
-    int L = c->type_level[ptype[i] - 1];
-    c->level_rmax[L] = fmx(c->level_rmax[L], r[i]);
-  }
-  c->n_upper = n;
-  c->have_state = true;
-  c->stepped = false;
-  c->prepared = false;
-  c->first_step_pending = true;
-  c->need_pre = true;
-  // plan the grids now so configuration errors surface here
-  free_list(c, c->allocs_grid);
-  int64_t base = 0;
-  c->P.nlvl = (int)cells.size();
-  for (int L = 0; L < c->P.nlvl; L++) {
-    double rm = c->level_rmax[L] > 0.0 ? c->level_rmax[L] : 0.0;
-    if (int rc = make_grid(c, cells[L], rm, base, &c->P.lvl[L])) return rc;
-    base += c->P.lvl[L].ncell;
-  }
-  if (base + 2 > INT32_MAX) return c->fail(SDEM_E_ARG, "search grid too large");
-  c->dead_key = base;
-  e = dalloc(c, c->allocs_grid, &c->counts, (size_t)base + 2);
-  if (e == hipSuccess) e = hipMemset(c->counts, 0, ((size_t)base + 2) * sizeof(int));
-  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)base + 2);
-  int64_t nt = scan_tiles(base + 2);
-  if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)nt + 1);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(grid)");
+  if (int rc = restage(c)) return rc;
+  const int ts = c->tsize, rs = c->rsize;
+  b.n = n;
+  b.id.assign(id, id + n);
+  b.ptype.assign(ptype, ptype + n);
+  if (usr_mark) b.mark.assign(usr_mark, usr_mark + n);
+  else b.mark.assign(n, 1);
+  b.posr.assign(posr, posr + 4 * (size_t)n);
+  b.lin_vel.assign(lin_vel, lin_vel + 3 * (size_t)ts * n);
+  b.lin_acc.assign(lin_acc, lin_acc + 3 * (size_t)ts * n);
+  b.theta.assign(theta, theta + 3 * (size_t)n);
+  b.rot_vel.assign(rot_vel, rot_vel + 3 * (size_t)rs * n);
+  b.rot_acc.assign(rot_acc, rot_acc + 3 * (size_t)rs * n);
+  b.ckind.clear(); b.cowner.clear(); b.cpartner.clear(); b.ctan.clear(); b.cvri.clear();
+  b.have_state = true;
   return SDEM_OK;
 }
 
@@ -720,156 +990,31 @@ int sdem_set_state(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* pty
 int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind, const int32_t* owner_id,
                       const int32_t* partner_id, const double* tan_delta, const double* vrel_init) {
   if (!c) return SDEM_E_ARG;
-  if (!c->have_state) return c->fail(SDEM_E_STATE, "state first");
+  Box& b = c->box();
+  if (!b.have_state) return c->fail(SDEM_E_STATE, "state first");
   if (m < 0 || (m > 0 && (!kind || !owner_id || !partner_id || !tan_delta || !vrel_init)))
     return c->fail(SDEM_E_ARG, "bad contact arguments");
-  hipError_t e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return hip_fail(c, e, "sync");
-  Buf& A = c->buf[c->cur];
-  int n = c->n_upper;
-  std::vector<int> ids(n), types(n);
-  e = hipMemcpy(ids.data(), A.id, (size_t)n * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(types.data(), A.type, (size_t)n * 4, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(ids)");
-  std::vector<std::pair<int, int>> idx;
-  idx.reserve(n);
-  for (int s = 0; s < n; s++)
-    if (types[s] >= 1) idx.push_back({ids[s], s});
-  std::sort(idx.begin(), idx.end());
-  std::vector<int> hcnt(n, 0), hkey((size_t)n * c->hcap, 0);
-  std::vector<double4> hval((size_t)n * c->hcap);  // record q of particle s at q*n+s here
   for (int64_t k = 0; k < m; k++) {
-    auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(owner_id[k], -1));
-    if (it == idx.end() || it->first != owner_id[k]) return c->fail(SDEM_E_ARG, "owner id not present");
     if (kind[k] < 1 || kind[k] > 4) return c->fail(SDEM_E_ARG, "bad contact kind");
     if (partner_id[k] < 0 || partner_id[k] > KEY_MASK) return c->fail(SDEM_E_ARG, "partner id out of range");
-    int s = it->second;
-    if (hcnt[s] >= c->hcap) return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
-    size_t slot = (size_t)hcnt[s]++ * n + s;
-    hkey[slot] = partner_id[k] | (kind[k] << KEY_BITS);
-    hval[slot] = make_double4(tan_delta[3 * k], tan_delta[3 * k + 1], tan_delta[3 * k + 2], vrel_init[k]);
   }
-  e = hipMemcpy(A.hcnt, hcnt.data(), (size_t)n * 4, hipMemcpyHostToDevice);
-  for (int q = 0; q < c->hcap && e == hipSuccess; q++) {
-    e = hipMemcpy(A.hkey + (size_t)q * c->cap, hkey.data() + (size_t)q * n, (size_t)n * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(A.hval + (size_t)q * c->cap, hval.data() + (size_t)q * n, (size_t)n * sizeof(double4),
-                    hipMemcpyHostToDevice);
-  }
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(contacts)");
+  if (c->stepped) return c->fail(SDEM_E_STATE, "contacts are set with the state, before stepping");
+  if (int rc = restage(c)) return rc;
+  std::vector<int> ids(b.id);
+  std::sort(ids.begin(), ids.end());
+  for (int64_t k = 0; k < m; k++)
+    if (!std::binary_search(ids.begin(), ids.end(), owner_id[k])) return c->fail(SDEM_E_ARG, "owner id not present");
+  b.ckind.assign(kind, kind + m);
+  b.cowner.assign(owner_id, owner_id + m);
+  b.cpartner.assign(partner_id, partner_id + m);
+  b.ctan.assign(tan_delta, tan_delta + 3 * (size_t)m);
+  b.cvri.assign(vrel_init, vrel_init + (size_t)m);
   return SDEM_OK;
 }
 
-static int prepare(sdem_ctx* c) {
-  if (c->prepared) return SDEM_OK;
-  invalidate_graphs(c);
-  free_list(c, c->allocs_tab);
-  Params& P = c->P;
-  P.dt = c->o.dt;
-  for (int d = 0; d < 3; d++) {
-    P.g[d] = c->o.gravity[d];
-    P.dmin[d] = c->o.domain_min[d];
-    P.dmax[d] = c->o.domain_max[d];
-    P.simlen[d] = c->o.domain_max[d] - c->o.domain_min[d];
-    P.periodic[d] = c->o.is_periodic[d] ? 1 : 0;
-  }
-  double maxd = 2.0 * max_type_radius(c);
-  double lenforcs = c->o.prtcl_cs_ratio * maxd;
-  for (int d = 0; d < 3; d++)
-    if (1.02 * lenforcs > P.simlen[d]) return c->fail(SDEM_E_ARG, "so big Diameter");
-  P.cf_type = c->o.cf_type;
-  P.ct_model = c->o.ct_model;
-  P.pi_method = c->o.pi_method;
-  P.pri_method = c->o.pri_method;
-  P.tsize = c->tsize; P.rsize = c->rsize; P.ta = c->ta; P.tr = c->tr;
-  P.cap = c->cap;
-  P.hcap = c->hcap;
-  P.ntype = c->ntype;
-  P.nwtype = c->nwtype;
-  P.nwall = (int)c->walls.size();
-  P.search_fixed = c->o.cs_method == SDEM_CSM_NBS_MUNJIZA ? 1 : 0;
-  P.keep_forces = c->keep_forces;
-  double rmax = max_type_radius(c);
-  P.dxmax1 = rmax * c->o.wall_neighbor_ratio;
-  P.dxmax2 = rmax * (c->o.wall_neighbor_ratio + 1.0);
-  double mwv = 0.0;
-  for (auto& w : c->walls) mwv = fmx(mwv, vnorm(v3(w.trans_vel)));
-  P.max_wall_vel = mwv;
-  P.wall_max_update_iter = c->o.wall_max_update_iter;
-  P.skin = c->skin_factor * maxd;
-  P.rebuild_disp = 0.5 * P.skin * (1.0 - 1.0e-6);
-  P.lcap = c->lcap;
-  for (auto& w : c->walls)
-    if (w.wall_type > c->nwtype) return c->fail(SDEM_E_ARG, "wall_type exceeds numWall_type");
-  std::vector<double> mass(c->ntype), inertia(c->ntype);
-  for (int i = 0; i < c->ntype; i++) { mass[i] = c->pprop[i].mass; inertia[i] = c->pprop[i].inertia; }
-  hipError_t e = hipSuccess;
-  auto U = [&](auto** dst, const auto& vec) {
-    using T = typename std::remove_reference<decltype(vec)>::type::value_type;
-    if (e == hipSuccess) e = dalloc(c, c->allocs_tab, dst, vec.size());
-    if (e == hipSuccess && !vec.empty()) e = hipMemcpy(*dst, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice);
-  };
-  U(&c->d_pp, c->pp_tab);
-  U(&c->d_pw, c->pw_tab);
-  U(&c->d_mass, mass);
-  U(&c->d_inertia, inertia);
-  U(&c->d_walls, c->walls);
-  U(&c->d_type_level, c->type_level);
-  // fixed particles, sorted by the cells of a single grid
-  int nfix = (int)c->fix_id.size();
-  P.nfix = nfix;
-  double fr = 0.0;
-  for (int i = 0; i < nfix; i++) fr = fmx(fr, c->fix_pos[4 * i + 3]);
-  if (int rc = make_grid(c, c->o.prtcl_cs_ratio * maxd, fr, 0, &P.fgrid)) return rc;
-  {
-    Grid& G = P.fgrid;
-    std::vector<int64_t> key(nfix);
-    auto cl = [&](int d, double x) {
-      int cc = (int)std::floor((x - G.origin[d]) * G.inv_w[d]);
-      return cc < 0 ? 0 : (cc >= G.n[d] ? G.n[d] - 1 : cc);
-    };
-    for (int i = 0; i < nfix; i++) {
-      const double* p = &c->fix_pos[4 * i];
-      key[i] = ((int64_t)cl(2, p[2]) * G.n[1] + cl(1, p[1])) * G.n[0] + cl(0, p[0]);
-    }
-    std::vector<int> ord(nfix);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
-    std::vector<double4> fpk(nfix);
-    std::vector<int> fid(nfix), ftype(nfix), fstart(nfix > 0 ? G.ncell + 1 : 1, 0);
-    for (int k = 0; k < nfix; k++) {
-      int i = ord[k];
-      const double* p = &c->fix_pos[4 * i];
-      fpk[k] = make_double4(p[0], p[1], p[2], p[3]);
-      fid[k] = c->fix_id[i];
-      ftype[k] = c->fix_type[i];
-    }
-    if (nfix > 0) {
-      for (int i = 0; i < nfix; i++) fstart[key[i] + 1]++;
-      for (int64_t q = 0; q < G.ncell; q++) fstart[q + 1] += fstart[q];
-    }
-    U(&c->d_fpk, fpk);
-    U(&c->d_fid, fid);
-    U(&c->d_ftype, ftype);
-    U(&c->d_fstart, fstart);
-  }
-  if (e != hipSuccess) return hip_fail(c, e, "upload tables");
-  P.pp = c->d_pp;
-  P.pw = c->d_pw;
-  P.mass = c->d_mass;
-  P.inertia = c->d_inertia;
-  P.walls = c->d_walls;
-  P.type_level = c->d_type_level;
-  c->prepared = true;
-  return SDEM_OK;
-}
-
-template <class T>
-static hipError_t dl(std::vector<T>& v, const T* src, size_t n) {
-  v.resize(n);
-  if (!n) return hipSuccess;
-  return hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
-}
+// ===========================================================================
+// Stepping
+// ===========================================================================
 
 // Replace a tracked device allocation by a larger one (contents not kept).
 template <class T>
@@ -887,7 +1032,7 @@ static hipError_t regrow(sdem_ctx* c, T** p, size_t n) {
 static int grow_history(sdem_ctx* c) {
   const int old = c->hcap, neu = std::min(old * 2, 4096);
   if (neu <= old) return c->fail(SDEM_E_CAPACITY, "contact history slots exhausted for a particle");
-  const size_t cap = (size_t)c->cap;
+  const size_t cap = (size_t)c->nslots;
   Buf& A = c->buf[c->cur];
   int* hk = nullptr;
   double4* hv = nullptr;
@@ -911,20 +1056,18 @@ static int grow_history(sdem_ctx* c) {
   if (e == hipSuccess) e = regrow(c, &c->clist, cap * neu);
   if (e != hipSuccess) return hip_fail(c, e, "grow history");
   c->hcap = neu;
-  c->P.hcap = neu;
-  invalidate_graphs(c);
+  c->params_ready = false;
   return SDEM_OK;
 }
 
 // Candidate-list slots per particle sized to what the failed rebuild counted.
 static int grow_vlist(sdem_ctx* c, int list_need) {
   int neu = std::max(list_need + list_need / 4 + 8, 2 * c->lcap);
-  if ((int64_t)neu * c->cap > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
-  hipError_t e = regrow(c, &c->vlist, (size_t)c->cap * neu);
+  if ((int64_t)neu * c->nslots > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
+  hipError_t e = regrow(c, &c->vlist, (size_t)c->nslots * neu);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
   c->lcap = neu;
-  c->P.lcap = neu;
-  invalidate_graphs(c);
+  c->params_ready = false;
   return SDEM_OK;
 }
 
@@ -934,18 +1077,13 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
   Buf& A = c->buf[c->cur];
   Buf& B = c->buf[1 - c->cur];
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipError_t e = launch_ctl(c->P, c->S, c->slots, c->stream);
+  hipError_t e = launch_ctl(c->D, c->stream);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
-  if (e == hipSuccess)
-    e = launch_sort(c->P, A, c->S, c->n_upper, c->dead_key, c->keys, c->rank, c->counts, c->start,
-                    c->tile_sum, c->perm_tmp, c->perm, c->pk, c->stream);
+  if (e == hipSuccess) e = launch_sort(c->D, A, c->stream);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
-  if (e == hipSuccess) e = launch_vlist(c->P, c->S, c->n_upper, c->pk, c->start, c->vcnt, c->vlist, c->stream);
+  if (e == hipSuccess) e = launch_vlist(c->D, c->stream);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
-  if (e == hipSuccess)
-    e = launch_force(c->P, A, B, c->S, c->n_upper, c->perm, c->vcnt, c->vlist, c->clist, c->ccnt,
-                     c->d_fpk, c->d_fid, c->d_ftype, c->d_fstart, c->slots, c->stream,
-                     ev ? ev[4] : nullptr);
+  if (e == hipSuccess) e = launch_force(c->D, A, B, c->stream, ev ? ev[4] : nullptr);
   if (ev) (void)hipEventRecord(ev[5], c->stream);
   c->cur = 1 - c->cur;
   return e;
@@ -973,12 +1111,13 @@ static hipError_t pair_graph(sdem_ctx* c, hipGraphExec_t* out) {
 
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
   if (!c) return SDEM_E_ARG;
-  if (!c->have_state) return c->fail(SDEM_E_STATE, "state must be set before stepping");
   if (nsteps < 0) return c->fail(SDEM_E_ARG, "nsteps < 0");
   if (int rc = prepare(c)) return rc;
+  const int nbox = (int)c->boxes.size();
   const int NE = SDEM_NUM_PHASES + 1;
   int remaining = nsteps, retries = 0;
   while (remaining > 0) {
+    if (int rc = prepare(c)) return rc;
     std::vector<hipEvent_t> evs;
     if (c->prof) evs.resize((size_t)remaining * NE);
     for (auto& x : evs)
@@ -987,7 +1126,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     hipError_t e = hipSuccess;
     if (c->need_pre) {
       // near-wall pass of a state that no step has produced
-      e = launch_pre(c->P, c->buf[c->cur], c->S, c->slots, c->n_upper, c->stream);
+      e = launch_pre(c->D, c->buf[c->cur], c->stream);
       c->need_pre = false;
     }
     int it = 0;
@@ -1000,7 +1139,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     for (; it < remaining && e == hipSuccess; it++)
       e = enqueue_step(c, c->prof ? &evs[(size_t)it * NE] : nullptr);
-    if (e == hipSuccess) e = launch_finish(c->S, c->slots, c->stream);
+    if (e == hipSuccess) e = launch_finish(c->D, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
       for (auto& x : evs) (void)hipEventDestroy(x);
@@ -1014,10 +1153,10 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     const int ran = remaining - s.halted_steps;  // steps that did work, the last one failed if overflow
     const int ok_steps = s.overflow ? ran - 1 : ran;
-    for (int it = 0; it < ok_steps && c->prof; it++)
+    for (int k = 0; k < ok_steps && c->prof; k++)
       for (int q = 0; q < SDEM_NUM_PHASES; q++) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, evs[(size_t)it * NE + q], evs[(size_t)it * NE + q + 1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, evs[(size_t)k * NE + q], evs[(size_t)k * NE + q + 1]) == hipSuccess) {
           c->prof_ms[q] += ms;
           c->prof_n[q]++;
         }
@@ -1034,23 +1173,28 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     c->cur = (c0 + ok_steps) % 2;
     const int kind = s.overflow, list_need = s.list_need;
     s.iter = s.iter_at_begin;
-    s.next_wall_update = s.next_wall_at_begin;
-    s.n_alive = s.n_stored;
     s.need_rebuild = 1;
     s.have_parts = 0;
-    s.n_deleted = 0;
     s.ticket_scan = 0;
     s.list_need = 0;
     s.overflow = 0;
     s.halt = 0;
     s.halted_steps = 0;
-    if (c->first_step_pending) s.use_host_max_a = 1;
-    c->need_pre = true;
-    e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
+    std::vector<BoxState> bs;
+    e = dl(bs, c->BS, (size_t)nbox);
+    for (auto& B : bs) {
+      B.n_alive = B.n_stored;
+      B.next_wall_update = B.next_wall_at_begin;
+      if (c->first_step_pending) B.use_host_max_a = 1;
+    }
+    if (e == hipSuccess) e = hipMemcpy(c->BS, bs.data(), bs.size() * sizeof(BoxState), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
     // the slots hold what the failing and the halted steps folded in; the
     // replayed step rebuilds its lists, so only its own maxima are needed
     if (e == hipSuccess) e = hipMemset(c->slots, 0, NSLOT * sizeof(Slot));
+    if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSLOT * sizeof(BoxSlot));
     if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
+    c->need_pre = true;
     int rc = kind == 2 ? grow_vlist(c, list_need) : grow_history(c);
     if (rc) return rc;
     remaining -= ok_steps;
@@ -1058,29 +1202,43 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
   return SDEM_OK;
 }
 
-// live particles of buffer A as (id, storage index), ascending id
+// ===========================================================================
+// Readers (the selected box; ascending particle id)
+// ===========================================================================
+
+// live particles of the selected box in buffer A as (id, storage slot)
 static int live_index(sdem_ctx* c, std::vector<std::pair<int, int>>& idx) {
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "sync");
-  Scalars s;
-  e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
-  int n = c->stepped ? s.n_alive : s.n_stored;
+  const Box& b = c->box();
+  BoxState B;
+  e = hipMemcpy(&B, c->BS + c->sel, sizeof(B), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(box state)");
+  const int n = B.n_alive;
   Buf& A = c->buf[c->cur];
   std::vector<int> ids(n), types(n);
-  e = hipMemcpy(ids.data(), A.id, (size_t)n * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(types.data(), A.type, (size_t)n * 4, hipMemcpyDeviceToHost);
+  e = hipMemcpy(ids.data(), A.id + b.off, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(types.data(), A.type + b.off, (size_t)n * 4, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(ids)");
   idx.clear();
   for (int k = 0; k < n; k++)
-    if (types[k] >= 1) idx.push_back({ids[k], k});
+    if (types[k] >= 1) idx.push_back({ids[k], b.off + k});
   std::sort(idx.begin(), idx.end());
   return SDEM_OK;
 }
 
+// the staged state of the selected box, ascending id
+static std::vector<std::pair<int, int>> staged_index(const Box& b) {
+  std::vector<std::pair<int, int>> idx;
+  for (int i = 0; i < b.n; i++) idx.push_back({b.id[i], i});
+  std::sort(idx.begin(), idx.end());
+  return idx;
+}
+
 int sdem_get_num_particles(sdem_ctx* c, int32_t* n) {
   if (!c || !n) return SDEM_E_ARG;
-  if (!c->have_state) { *n = 0; return SDEM_OK; }
+  const Box& b = c->box();
+  if (!c->stepped) { *n = b.have_state ? b.n : 0; return SDEM_OK; }
   std::vector<std::pair<int, int>> idx;
   if (int rc = live_index(c, idx)) return rc;
   *n = (int32_t)idx.size();
@@ -1091,18 +1249,38 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
                    int32_t* usr_mark, double* posr, double* lin_vel, double* lin_acc, double* theta,
                    double* rot_vel, double* rot_acc) {
   if (!c || !nout) return SDEM_E_ARG;
-  if (!c->have_state) return c->fail(SDEM_E_STATE, "no state");
+  const Box& b = c->box();
+  if (!b.have_state) return c->fail(SDEM_E_STATE, "no state");
+  const int ts = c->tsize, rs = c->rsize;
+  if (!c->stepped) {
+    auto idx = staged_index(b);
+    const int n = (int)idx.size();
+    *nout = n;
+    if (capn < n) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
+    for (int q = 0; q < n; q++) {
+      const size_t i = (size_t)idx[q].second;
+      if (id) id[q] = b.id[i];
+      if (ptype) ptype[q] = b.ptype[i];
+      if (usr_mark) usr_mark[q] = b.mark[i];
+      if (posr) std::copy(&b.posr[4 * i], &b.posr[4 * i] + 4, posr + 4 * (size_t)q);
+      if (lin_vel) std::copy(&b.lin_vel[3 * ts * i], &b.lin_vel[3 * ts * i] + 3 * ts, lin_vel + 3 * ts * (size_t)q);
+      if (lin_acc) std::copy(&b.lin_acc[3 * ts * i], &b.lin_acc[3 * ts * i] + 3 * ts, lin_acc + 3 * ts * (size_t)q);
+      if (theta) std::copy(&b.theta[3 * i], &b.theta[3 * i] + 3, theta + 3 * (size_t)q);
+      if (rot_vel) std::copy(&b.rot_vel[3 * rs * i], &b.rot_vel[3 * rs * i] + 3 * rs, rot_vel + 3 * rs * (size_t)q);
+      if (rot_acc) std::copy(&b.rot_acc[3 * rs * i], &b.rot_acc[3 * rs * i] + 3 * rs, rot_acc + 3 * rs * (size_t)q);
+    }
+    return SDEM_OK;
+  }
   std::vector<std::pair<int, int>> idx;
   if (int rc = live_index(c, idx)) return rc;
   int n = (int)idx.size();
   *nout = n;
   if (capn < n) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
   Buf& A = c->buf[c->cur];
-  const size_t cap = c->cap;
-  const int ts = c->tsize, rs = c->rsize;
+  const size_t cap = (size_t)c->nslots;
   std::vector<double> lv, la, th, rv, ra;
   std::vector<double4> p4;
-  std::vector<int> ty, mk, ids;
+  std::vector<int> ty, mk;
   hipError_t e = dl(p4, A.pos, cap);
   if (e == hipSuccess) e = dl(lv, A.lv, cap * 3 * ts);
   if (e == hipSuccess) e = dl(la, A.la, cap * 3 * c->ta);
@@ -1112,9 +1290,8 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
   if (e == hipSuccess) e = dl(ty, A.type, cap);
   if (e == hipSuccess) e = dl(mk, A.mark, cap);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
-  // before the first step the storage order is the caller's order
   for (int q = 0; q < n; q++) {
-    const int k = idx[q].second;
+    const size_t k = (size_t)idx[q].second;
     if (id) id[q] = idx[q].first;
     if (ptype) ptype[q] = ty[k];
     if (usr_mark) usr_mark[q] = mk[k];
@@ -1122,25 +1299,14 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
     for (int l = 0; l < ts; l++)
       for (int d = 0; d < 3; d++) {
         if (lin_vel) lin_vel[((size_t)q * ts + l) * 3 + d] = lv[((size_t)l * 3 + d) * cap + k];
-        if (lin_acc) {
-          double v;
-          if (ts == 1) v = la[(size_t)d * cap + k];
-          else if (l == 0) v = c->stepped ? la[(size_t)d * cap + k] : c->host_lacc1[3 * (size_t)k + d];
-          else v = la[((size_t)(l - 1) * 3 + d) * cap + k];
-          lin_acc[((size_t)q * ts + l) * 3 + d] = v;
-        }
+        // stored level l-1 holds linAcc(l); linAcc(1) is the stored level 0 (FE: the only one)
+        if (lin_acc) lin_acc[((size_t)q * ts + l) * 3 + d] = la[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * cap + k];
       }
     if (theta) for (int d = 0; d < 3; d++) theta[3 * q + d] = th[(size_t)d * cap + k];
     for (int l = 0; l < rs; l++)
       for (int d = 0; d < 3; d++) {
         if (rot_vel) rot_vel[((size_t)q * rs + l) * 3 + d] = rv[((size_t)l * 3 + d) * cap + k];
-        if (rot_acc) {
-          double v;
-          if (rs == 1) v = ra[(size_t)d * cap + k];
-          else if (l == 0) v = c->stepped ? ra[(size_t)d * cap + k] : c->host_racc1[3 * (size_t)k + d];
-          else v = ra[((size_t)(l - 1) * 3 + d) * cap + k];
-          rot_acc[((size_t)q * rs + l) * 3 + d] = v;
-        }
+        if (rot_acc) rot_acc[((size_t)q * rs + l) * 3 + d] = ra[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * cap + k];
       }
   }
   return SDEM_OK;
@@ -1148,23 +1314,34 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
 
 int sdem_get_forces(sdem_ctx* c, int32_t capn, int32_t* id, double* f, double* t) {
   if (!c) return SDEM_E_ARG;
-  if (!c->have_state) return c->fail(SDEM_E_STATE, "no state");
+  const Box& b = c->box();
+  if (!b.have_state) return c->fail(SDEM_E_STATE, "no state");
+  if (!c->stepped) {
+    auto idx = staged_index(b);
+    if (capn < (int)idx.size()) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
+    for (size_t q = 0; q < idx.size(); q++) {
+      if (id) id[q] = idx[q].first;
+      for (int d = 0; d < 3; d++) {
+        if (f) f[3 * q + d] = 0.0;
+        if (t) t[3 * q + d] = 0.0;
+      }
+    }
+    return SDEM_OK;
+  }
   std::vector<std::pair<int, int>> idx;
   if (int rc = live_index(c, idx)) return rc;
   int n = (int)idx.size();
   if (capn < n) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
   std::vector<double> fb;
-  hipError_t e = dl(fb, c->buf[c->cur].force, (size_t)c->cap * 6);
+  const size_t cap = (size_t)c->nslots;
+  hipError_t e = dl(fb, c->buf[c->cur].force, cap * 6);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(forces)");
-  const size_t cap = c->cap;
   for (int q = 0; q < n; q++) {
-    int k = idx[q].second;
+    const size_t k = (size_t)idx[q].second;
     if (id) id[q] = idx[q].first;
     for (int d = 0; d < 3; d++) {
-      double fv = c->stepped ? fb[(size_t)d * cap + k] : 0.0;
-      double tv = c->stepped ? fb[(size_t)(3 + d) * cap + k] : 0.0;
-      if (f) f[3 * q + d] = fv;
-      if (t) t[3 * q + d] = tv;
+      if (f) f[3 * q + d] = fb[(size_t)d * cap + k];
+      if (t) t[3 * q + d] = fb[(size_t)(3 + d) * cap + k];
     }
   }
   return SDEM_OK;
@@ -1172,39 +1349,45 @@ int sdem_get_forces(sdem_ctx* c, int32_t capn, int32_t* id, double* f, double* t
 
 struct Rec {
   int owner, partner, kind;
-  size_t slot;
+  double4 v;
 };
 
-static int collect(sdem_ctx* c, std::vector<Rec>& recs, std::vector<double4>& hv) {
-  std::vector<std::pair<int, int>> idx;
-  if (int rc = live_index(c, idx)) return rc;
-  Buf& A = c->buf[c->cur];
-  std::vector<int> hc, hk;
-  size_t n = (size_t)c->n_upper;
-  hipError_t e = dl(hc, A.hcnt, n);
-  if (e == hipSuccess) e = dl(hk, A.hkey, (size_t)c->cap * c->hcap);
-  if (e == hipSuccess) e = dl(hv, A.hval, (size_t)c->cap * c->hcap);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
+static int collect(sdem_ctx* c, std::vector<Rec>& recs) {
   recs.clear();
-  for (auto& pr : idx) {
-    int k = pr.second;
-    for (int q = 0; q < hc[k]; q++) {
-      size_t slot = (size_t)q * c->cap + k;
-      recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), slot});
+  const Box& b = c->box();
+  if (!c->stepped) {
+    for (size_t k = 0; k < b.ckind.size(); k++)
+      recs.push_back({b.cowner[k], b.cpartner[k], b.ckind[k],
+                      make_double4(b.ctan[3 * k], b.ctan[3 * k + 1], b.ctan[3 * k + 2], b.cvri[k])});
+  } else {
+    std::vector<std::pair<int, int>> idx;
+    if (int rc = live_index(c, idx)) return rc;
+    Buf& A = c->buf[c->cur];
+    const size_t cap = (size_t)c->nslots;
+    std::vector<int> hc, hk;
+    std::vector<double4> hv;
+    hipError_t e = dl(hc, A.hcnt, cap);
+    if (e == hipSuccess) e = dl(hk, A.hkey, cap * c->hcap);
+    if (e == hipSuccess) e = dl(hv, A.hval, cap * c->hcap);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
+    for (auto& pr : idx) {
+      const size_t k = (size_t)pr.second;
+      for (int q = 0; q < hc[k]; q++) {
+        const size_t slot = (size_t)q * cap + k;
+        recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), hv[slot]});
+      }
     }
   }
-  std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
-    return a.owner != b.owner ? a.owner < b.owner : a.partner < b.partner;
+  std::sort(recs.begin(), recs.end(), [](const Rec& x, const Rec& y) {
+    return x.owner != y.owner ? x.owner < y.owner : x.partner < y.partner;
   });
   return SDEM_OK;
 }
 
 int sdem_get_num_contacts(sdem_ctx* c, int64_t* m) {
   if (!c || !m) return SDEM_E_ARG;
-  if (!c->have_state) { *m = 0; return SDEM_OK; }
   std::vector<Rec> recs;
-  std::vector<double4> hv;
-  if (int rc = collect(c, recs, hv)) return rc;
+  if (int rc = collect(c, recs)) return rc;
   *m = (int64_t)recs.size();
   return SDEM_OK;
 }
@@ -1212,10 +1395,8 @@ int sdem_get_num_contacts(sdem_ctx* c, int64_t* m) {
 int sdem_get_contacts(sdem_ctx* c, int64_t capm, int64_t* m, int32_t* kind, int32_t* owner,
                       int32_t* partner, double* td, double* vr) {
   if (!c || !m) return SDEM_E_ARG;
-  if (!c->have_state) { *m = 0; return SDEM_OK; }
   std::vector<Rec> recs;
-  std::vector<double4> hv;
-  if (int rc = collect(c, recs, hv)) return rc;
+  if (int rc = collect(c, recs)) return rc;
   *m = (int64_t)recs.size();
   if (capm < *m) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
   for (size_t q = 0; q < recs.size(); q++) {
@@ -1223,37 +1404,41 @@ int sdem_get_contacts(sdem_ctx* c, int64_t capm, int64_t* m, int32_t* kind, int3
     if (kind) kind[q] = r.kind;
     if (owner) owner[q] = r.owner;
     if (partner) partner[q] = r.partner;
-    const double4 v = hv[r.slot];
-    if (td) { td[3 * q] = v.x; td[3 * q + 1] = v.y; td[3 * q + 2] = v.z; }
-    if (vr) vr[q] = v.w;
+    if (td) { td[3 * q] = r.v.x; td[3 * q + 1] = r.v.y; td[3 * q + 2] = r.v.z; }
+    if (vr) vr[q] = r.v.w;
   }
   return SDEM_OK;
 }
 
+static int read_box_state(sdem_ctx* c, BoxState* B, Scalars* s) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && B) e = hipMemcpy(B, c->BS + c->sel, sizeof(*B), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && s) e = hipMemcpy(s, c->S, sizeof(*s), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SDEM_OK : hip_fail(c, e, "hipMemcpy(scalars)");
+}
+
 int sdem_get_counts(sdem_ctx* c, int32_t num[4]) {
   if (!c || !num) return SDEM_E_ARG;
-  if (!c->have_state) { for (int k = 0; k < 4; k++) num[k] = 0; return SDEM_OK; }
-  hipError_t e = hipStreamSynchronize(c->stream);
-  Scalars s;
-  if (e == hipSuccess) e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
-  for (int k = 0; k < 4; k++) num[k] = c->stepped ? s.num_cntcts[k] : 0;
+  for (int k = 0; k < 4; k++) num[k] = 0;
+  if (!c->stepped) return SDEM_OK;
+  BoxState B;
+  if (int rc = read_box_state(c, &B, nullptr)) return rc;
+  for (int k = 0; k < 4; k++) num[k] = B.num_cntcts[k];
   return SDEM_OK;
 }
 
 int sdem_get_iter(sdem_ctx* c, int32_t* iter, int32_t* next_update) {
   if (!c) return SDEM_E_ARG;
-  if (!c->have_state) {
-    if (iter) *iter = c->o.ifirst - 1;
+  if (!c->stepped) {
+    if (iter) *iter = c->box().o.ifirst - 1;
     if (next_update) *next_update = 0;
     return SDEM_OK;
   }
-  hipError_t e = hipStreamSynchronize(c->stream);
+  BoxState B;
   Scalars s;
-  if (e == hipSuccess) e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
+  if (int rc = read_box_state(c, &B, &s)) return rc;
   if (iter) *iter = s.iter;
-  if (next_update) *next_update = s.next_wall_update;
+  if (next_update) *next_update = B.next_wall_update;
   return SDEM_OK;
 }
 
@@ -1275,28 +1460,91 @@ int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES], int64_t launches[S
 
 int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor, int32_t keep_forces) {
   if (!c) return SDEM_E_ARG;
-  if (skin_factor >= 0.0) {
-    if (skin_factor > 0.45) return c->fail(SDEM_E_ARG, "skin factor must be below 0.45");
-    c->skin_factor = skin_factor;
-    c->prepared = false;
-  }
-  if (cell_factor) {
+  if (skin_factor >= 0.0 && skin_factor > 0.45) return c->fail(SDEM_E_ARG, "skin factor must be below 0.45");
+  if (cell_factor)
     for (int d = 0; d < 3; d++)
       if (cell_factor[d] < 0.0 || cell_factor[d] > 64.0) return c->fail(SDEM_E_ARG, "cell factor out of range");
+  if (skin_factor >= 0.0) c->skin_factor = skin_factor;
+  if (cell_factor)
     for (int d = 0; d < 3; d++)
       if (cell_factor[d] > 0.0) c->cell_factor[d] = cell_factor[d];
+  if (keep_forces >= 0) c->keep_forces = keep_forces ? 1 : 0;
+  // new grids and parameters at the next step, which rebuilds the lists
+  c->params_ready = false;
+  return SDEM_OK;
+}
+
+int sdem_set_block_sizes(sdem_ctx* c, int32_t contact_block, int32_t force_block) {
+  if (!c) return SDEM_E_ARG;
+  auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+  if (!ok(contact_block) || !ok(force_block)) return c->fail(SDEM_E_ARG, "block sizes must be 64, 128 or 256");
+  c->bs_contact = contact_block;
+  c->bs_force = force_block;
+  c->params_ready = false;
+  return SDEM_OK;
+}
+
+int sdem_get_tuning(sdem_ctx* c, double cell_factor[3], double* skin_factor, int32_t* contact_block,
+                    int32_t* force_block) {
+  if (!c) return SDEM_E_ARG;
+  if (cell_factor) for (int d = 0; d < 3; d++) cell_factor[d] = c->cell_factor[d];
+  if (skin_factor) *skin_factor = c->skin_factor;
+  if (contact_block) *contact_block = c->bs_contact;
+  if (force_block) *force_block = c->bs_force;
+  return SDEM_OK;
+}
+
+// Autotuner: every candidate (cell shape, skin, block sizes) runs
+// `steps_per_trial` real steps after one settling step, timed with HIP
+// events on the library's stream; the fastest is kept.  The results do not
+// depend on the tuning, so the trial steps are ordinary steps of the run.
+int sdem_autotune(sdem_ctx* c, int32_t steps_per_trial, int32_t* steps_run, double* best_ms_per_step) {
+  if (!c) return SDEM_E_ARG;
+  if (steps_per_trial < 2) return c->fail(SDEM_E_ARG, "steps_per_trial must be at least 2");
+  struct Cand { double cf, skin; int bc, bf; };
+  std::vector<Cand> cands;
+  for (double cf : {0.5, 0.75, 1.0})
+    for (double sk : {0.03, 0.06, 0.12})
+      for (int bf : {128, 256}) cands.push_back({cf, sk, 256, bf});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return c->fail(SDEM_E_HIP, "hipEventCreate");
+  double best = 1e300;
+  Cand bestc = cands[0];
+  int total = 0, rc = SDEM_OK;
+  const bool prof = c->prof;
+  c->prof = false;
+  for (const Cand& k : cands) {
+    const double cf[3] = {k.cf, k.cf, k.cf};
+    if ((rc = sdem_set_tuning(c, cf, k.skin, -1))) break;
+    if ((rc = sdem_set_block_sizes(c, k.bc, k.bf))) break;
+    if ((rc = sdem_step(c, 2))) break;  // new grids: the first steps rebuild
+    total += 2;
+    (void)hipEventRecord(e0, c->stream);
+    if ((rc = sdem_step(c, steps_per_trial))) break;
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    total += steps_per_trial;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / steps_per_trial;
+    if (per < best) { best = per; bestc = k; }
   }
-  if (keep_forces >= 0) {
-    c->keep_forces = keep_forces ? 1 : 0;
-    c->P.keep_forces = c->keep_forces;
-    invalidate_graphs(c);
-  }
+  c->prof = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  const double cf[3] = {bestc.cf, bestc.cf, bestc.cf};
+  if ((rc = sdem_set_tuning(c, cf, bestc.skin, -1))) return rc;
+  if ((rc = sdem_set_block_sizes(c, bestc.bc, bestc.bf))) return rc;
+  if (steps_run) *steps_run = total;
+  if (best_ms_per_step) *best_ms_per_step = best;
   return SDEM_OK;
 }
 
 int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots) {
   if (!c) return SDEM_E_ARG;
-  if (!c->allocs_state.empty()) return c->fail(SDEM_E_STATE, "capacities must be set before the state");
+  if (c->dev_ready) return c->fail(SDEM_E_STATE, "capacities must be set before the first step");
   if (hist_slots < 0 || hist_slots > 4096 || list_slots < 0 || list_slots > 65536)
     return c->fail(SDEM_E_ARG, "capacity out of range");
   c->hist_slots_hint = hist_slots;

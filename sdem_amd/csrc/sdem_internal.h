@@ -86,7 +86,6 @@ struct Params {
   int nfix;
   double dxmax1, dxmax2, max_wall_vel;
   double skin;                // candidate-list skin (search tuning)
-  double rebuild_disp;        // rebuild when a particle moved further than this
   int lcap;                   // candidate-list entries per particle
   int wall_max_update_iter;
   const Bnry* pp;             // ntype*ntype, row = type of the record's lower-id particle
@@ -94,15 +93,23 @@ struct Params {
   const double* mass;         // ntype (1-based types -> index t-1)
   const double* inertia;
   const Wall* walls;          // nwall
+  // the box's place in the context
+  int box;                    // index of the box
+  int off, bcap;              // storage slots [off, off + bcap)
+  int dead_key;               // sort key of the box's removed / unused slots
 };
 
 // Partial results of the per-particle kernels, accumulated with atomics
 // spread over NSLOT addresses (block b uses slot b % NSLOT) so that no
 // single address takes thousands of atomics; reduced by the next step's
-// control block and by k_finish.
+// control kernels and by k_finish.
 constexpr int NSLOT = 64;
-struct Slot {
+struct Slot {                    // whole context
   unsigned long long step_bits;  // largest distance a particle moved in the step
+  int ndel;                      // particles removed
+  int pad;
+};
+struct BoxSlot {                 // one box (NSLOT of them per box)
   unsigned long long max_v_bits; // near-wall list: max |v|, max |a| (as bits of
   unsigned long long max_a_bits; // non-negative doubles)
   int cnt[4];                    // contacts by kind
@@ -110,29 +117,62 @@ struct Slot {
   int pad;
 };
 
-// device-side scalars of one context
-struct Scalars {
-  int n_stored;            // entries of buffer A at the start of the step
+// Every box (an independent system: one for a plain context, many for an
+// ensemble) owns the storage slots [off, off + bcap) of every per-particle
+// array; bcap is a multiple of BOX_ALIGN, so a block of up to BOX_ALIGN
+// threads never spans two boxes and reads its box once, as a scalar.
+constexpr int BOX_ALIGN = 256;
+
+// device-side state of one box
+struct BoxState {
+  int n_stored;            // entries of the box in buffer A at the start of the step
   int n_alive;             // live particles after this step's sort
   int n_deleted;           // particles removed at the end of the last step
-  int iter;                // iterNumber
   int next_wall_update;    // Next_iter_update
+  int next_wall_at_begin;  // ... at the start of the last step that ran
+  int use_host_max_a;      // first step after the state was set: max|a| from the host
+  double host_max_a;
+  int num_cntcts[4];       // numCntcts of the last step
+};
+
+// device-side scalars of one context
+struct Scalars {
+  int iter;                // iterNumber (all boxes step together)
+  int iter_at_begin;       // iterNumber at the start of the last step that ran
   int overflow;            // 1: history slots, 2: candidate-list entries
   int halt;                // set by the step after an overflow: no more work
   int halted_steps;        // steps of the call skipped after an overflow
-  int iter_at_begin;       // iterNumber / Next_iter_update at the start of
-  int next_wall_at_begin;  // the last step that ran
   int rebuild;             // this step rebuilds the cell sort and the lists
   int need_rebuild;        // forced rebuild at the next step
   int rebuilds;            // rebuilds so far (statistics)
   int have_parts;          // the force kernel's slots are current
   unsigned ticket_scan;    // last-block election in k_scan_tiles
   int list_need;           // largest candidate count seen when a list overflowed
-  int num_cntcts[4];
-  int use_host_max_a;      // first step after set_state: max|a| from host copy
-  double host_max_a;
+  int pad;
+  double rebuild_disp;     // smallest half skin over the boxes (with a margin)
   double disp_acc;         // sum over steps since the last rebuild of the largest
                            // move of a particle: bounds every displacement
+};
+
+// Device pointers and launch shapes the host hands to the launchers.
+struct Dev {
+  const Params* PB;      // one per box
+  const int* blk_box;    // box of each BOX_ALIGN-slot group
+  int nbox;
+  int64_t nslots;        // storage slots, sum of the boxes' bcap
+  int64_t nkeys;         // sort keys of all boxes (cells + one last key each)
+  Scalars* S;
+  BoxState* BS;
+  Slot* slots;
+  BoxSlot* bslots;       // NSLOT per box
+  int *keys, *rank, *counts, *start, *tile_sum, *perm_tmp, *perm;
+  double4* pk;
+  int *vcnt, *vlist;
+  unsigned* clist;
+  int* ccnt;
+  const double4* fpk;
+  const int *fid, *ftype, *fstart;
+  int bs_contact, bs_force;  // block sizes (divisors of BOX_ALIGN)
 };
 
 }  // namespace sdem

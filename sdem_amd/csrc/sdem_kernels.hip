@@ -278,12 +278,21 @@ __device__ __forceinline__ uint64_t near_wall_mask(const Params& P, double x, do
   return mask;
 }
 
-__global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* slots) {
+// The box of a block: boxes own whole BOX_ALIGN-slot groups, so the first
+// slot of a block (of at most BOX_ALIGN threads) names it.
+__device__ __forceinline__ int block_box(const int* __restrict__ blk_box) {
+  return blk_box[(blockIdx.x * blockDim.x) / BOX_ALIGN];
+}
+
+__global__ __launch_bounds__(256) void k_pre(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                             Buf A, Scalars* S, BoxState* BS, BoxSlot* bslots) {
   __shared__ double shd[4];
-  if (S->halt || S->iter < S->next_wall_update) return;  // max|v|, max|a| are taken even without walls
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  if (S->halt || S->iter < BS[box].next_wall_update) return;  // max|v|, max|a| are taken even without walls
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mv = 0.0, ma = 0.0;
-  if (s < S->n_alive && A.type[s] >= 1) {
+  if (s - P.off < BS[box].n_alive && A.type[s] >= 1) {
     const double4 p = A.pos[s];
     A.wmask[s] = near_wall_mask(P, p.x, p.y, p.z);
     mv = norm3(ld3(A.lv, P.cap, s));
@@ -292,61 +301,36 @@ __global__ __launch_bounds__(256) void k_pre(Params P, Buf A, Scalars* S, Slot* 
   mv = block_max(mv, shd);
   ma = block_max(ma, shd);
   if (threadIdx.x == 0) {
-    Slot& sl = slots[blockIdx.x % NSLOT];
+    BoxSlot& sl = bslots[box * NSLOT + blockIdx.x % NSLOT];
     atomicMax(&sl.max_v_bits, dbits(mv));
     atomicMax(&sl.max_a_bits, dbits(ma));
   }
 }
 
-// Step control, one wave: reduces the slots (the previous step's largest
-// move and deletions, this step's max|v| and max|a|), decides whether the
-// cell sort and the candidate lists are rebuilt, computes the near-wall
-// cadence (Prtcl_ContactSearchPW.f90:201-229) and advances iterNumber.
-// After a capacity overflow it only marks the call halted, so every later
-// kernel of the call does nothing and the failing step's input buffer stays
-// intact for the replay; every halted step is counted so that the host
-// knows how many steps completed.
-__global__ __launch_bounds__(64) void k_ctl(Params P, Scalars* S, Slot* slots) {
+// Per-box part of the step control, one wave: reduces the box's slots
+// (this step's max|v| and max|a|), clears them for the step, and runs the
+// near-wall cadence (Prtcl_ContactSearchPW.f90:201-229) at iterNumber `iter`.
+__device__ void box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl) {
   const int lane = threadIdx.x;
-  if (S->halt || S->overflow) {
-    if (lane == 0) {
-      S->halt = 1;
-      S->halted_steps++;
-    }
-    return;
-  }
-  const int iter = S->iter, next = S->next_wall_update;
-  Slot& sl = slots[lane];
-  double step = bitsd(sl.step_bits), mv = bitsd(sl.max_v_bits), ma = bitsd(sl.max_a_bits);
-  int ndel = sl.ndel;
+  BoxSlot& m = sl[lane];
+  double mv = bitsd(m.max_v_bits), ma = bitsd(m.max_a_bits);
   for (int off = 32; off > 0; off >>= 1) {
-    step = fmax(step, __shfl_down(step, off, 64));
     mv = fmax(mv, __shfl_down(mv, off, 64));
     ma = fmax(ma, __shfl_down(ma, off, 64));
-    ndel += __shfl_down(ndel, off, 64);
   }
-  sl.step_bits = 0ull;
-  sl.max_v_bits = 0ull;
-  sl.max_a_bits = 0ull;
-  sl.ndel = 0;
-  for (int q = 0; q < 4; q++) sl.cnt[q] = 0;
+  m.max_v_bits = 0ull;
+  m.max_a_bits = 0ull;
+  m.ndel = 0;
+  for (int q = 0; q < 4; q++) m.cnt[q] = 0;
   if (lane != 0) return;
-  const bool have = S->have_parts != 0;
-  // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
-  const double disp = S->disp_acc + step;
-  S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= P.rebuild_disp)) ? 1 : 0;
-  S->disp_acc = S->rebuild ? 0.0 : disp;
-  S->need_rebuild = 0;
-  S->have_parts = 0;
-  S->n_stored = S->n_alive;
-  S->iter_at_begin = iter;
-  S->next_wall_at_begin = next;
-  if (S->rebuild) S->rebuilds++;
+  const int next = B.next_wall_update;
+  B.n_stored = B.n_alive;
+  B.next_wall_at_begin = next;
   if (iter >= next) {
     double max_v = mv + P.max_wall_vel;
     max_v = max_v > 0.3 ? max_v : 0.3;
     max_v = max_v * 1.5;
-    double max_a = S->use_host_max_a ? S->host_max_a : ma;
+    double max_a = B.use_host_max_a ? B.host_max_a : ma;
     const double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
     max_a = max_a > gn ? max_a : gn;
     max_a = max_a * 1.5;
@@ -360,25 +344,77 @@ __global__ __launch_bounds__(64) void k_ctl(Params P, Scalars* S, Slot* slots) {
     }
     if (num_iter < 1) num_iter = 1;
     if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
-    S->next_wall_update = iter + num_iter;
+    B.next_wall_update = iter + num_iter;
   }
-  S->use_host_max_a = 0;
-  S->iter = iter + 1;  // iterNumber advances once per step (Prtcl_DEMSystem.f90:248)
+  B.use_host_max_a = 0;
 }
 
-// counters of the last step for the host (once per call)
-__global__ __launch_bounds__(64) void k_finish(Scalars* S, const Slot* slots) {
-  if (S->halt || !S->have_parts) return;
+// Step control, one wave: reduces the context's slots (the previous step's
+// largest move and deletions), decides whether the cell sort and the
+// candidate lists are rebuilt (for all boxes at once: the results do not
+// depend on it) and advances iterNumber (Prtcl_DEMSystem.f90:248); with a
+// single box it also runs that box's control.  After a capacity overflow it
+// only marks the call halted, so every later kernel of the call does nothing
+// and the failing step's input buffer stays intact for the replay; every
+// halted step is counted so that the host knows how many steps completed.
+__global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, Slot* slots,
+                                            BoxState* BS, BoxSlot* bslots, int nbox) {
   const int lane = threadIdx.x;
-  int c[4], nd = slots[lane].ndel;
-  for (int q = 0; q < 4; q++) c[q] = slots[lane].cnt[q];
+  if (S->halt || S->overflow) {
+    if (lane == 0) {
+      S->halt = 1;
+      S->halted_steps++;
+    }
+    return;
+  }
+  const int iter = S->iter;
+  Slot& sl = slots[lane];
+  double step = bitsd(sl.step_bits);
+  int ndel = sl.ndel;
+  for (int off = 32; off > 0; off >>= 1) {
+    step = fmax(step, __shfl_down(step, off, 64));
+    ndel += __shfl_down(ndel, off, 64);
+  }
+  sl.step_bits = 0ull;
+  sl.ndel = 0;
+  if (lane == 0) {
+    const bool have = S->have_parts != 0;
+    // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
+    const double disp = S->disp_acc + step;
+    S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= S->rebuild_disp)) ? 1 : 0;
+    S->disp_acc = S->rebuild ? 0.0 : disp;
+    S->need_rebuild = 0;
+    S->have_parts = 0;
+    S->iter_at_begin = iter;
+    if (S->rebuild) S->rebuilds++;
+    S->iter = iter + 1;  // iterNumber advances once per step
+  }
+  if (nbox == 1) box_control(PB[0], iter, BS[0], bslots);
+}
+
+// Per-box step control of an ensemble, one wave per box, after k_ctl.
+__global__ __launch_bounds__(64) void k_box(const Params* __restrict__ PB, const Scalars* S, BoxState* BS,
+                                            BoxSlot* bslots) {
+  if (S->halt) return;
+  const int box = blockIdx.x;
+  box_control(PB[box], S->iter_at_begin, BS[box], bslots + (size_t)box * NSLOT);
+}
+
+// counters of the last step for the host (once per call), one wave per box
+__global__ __launch_bounds__(64) void k_finish(const Scalars* S, BoxState* BS, const BoxSlot* bslots) {
+  if (S->halt || !S->have_parts) return;
+  const int box = blockIdx.x;
+  const int lane = threadIdx.x;
+  const BoxSlot& m = bslots[(size_t)box * NSLOT + lane];
+  int c[4], nd = m.ndel;
+  for (int q = 0; q < 4; q++) c[q] = m.cnt[q];
   for (int off = 32; off > 0; off >>= 1) {
     for (int q = 0; q < 4; q++) c[q] += __shfl_down(c[q], off, 64);
     nd += __shfl_down(nd, off, 64);
   }
   if (lane == 0) {
-    for (int q = 0; q < 4; q++) S->num_cntcts[q] = c[q];
-    S->n_deleted = nd;
+    for (int q = 0; q < 4; q++) BS[box].num_cntcts[q] = c[q];
+    BS[box].n_deleted = nd;
   }
 }
 
@@ -389,25 +425,27 @@ __global__ __launch_bounds__(64) void k_finish(Scalars* S, const Slot* slots) {
 // Lanes of a wave holding the same key (neighbours in storage order usually
 // share a cell) take their slots with one atomic per run.  The counts array
 // is all zero on entry (zeroed at allocation and by k_scan_add).
-__global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_t dead_key,
+__global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                             Buf A, const Scalars* S, const BoxState* BS,
                                              int* __restrict__ keys, int* __restrict__ rank,
                                              int* __restrict__ counts) {
   if (!S->rebuild || S->halt) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const bool valid = s < S->n_stored;
-  int key = -1 - lane;  // unique per lane when inactive
-  if (valid) {
-    int t = A.type[s];
-    if (t < 1) {
-      key = (int)dead_key;
-    } else {
+  // every slot of the box takes part: removed and unused ones go to the
+  // box's last key, so the box keeps its slot range
+  int key = P.dead_key;
+  if (s - P.off < BS[box].n_stored) {
+    const int t = A.type[s];
+    if (t >= 1) {
       const Grid& G = P.lvl[P.type_level[t - 1]];
       const double4 p = A.pos[s];
       key = (int)cell_key(G, p.x, p.y, p.z);
     }
-    keys[s] = key;
   }
+  keys[s] = key;
   const int prev = __shfl_up(key, 1, 64);
   const bool head = (lane == 0) || (prev != key);
   const unsigned long long heads = __ballot(head);
@@ -416,9 +454,9 @@ __global__ __launch_bounds__(256) void k_bin(Params P, Buf A, Scalars* S, int64_
   const unsigned long long after = heads & ~le;
   const int next_head = after ? (__ffsll((long long)after) - 1) : 64;
   int base = 0;
-  if (head && valid) base = atomicAdd(&counts[key], next_head - lane);
+  if (head) base = atomicAdd(&counts[key], next_head - lane);
   base = __shfl(base, my_head, 64);
-  if (valid) rank[s] = base + (lane - my_head);
+  rank[s] = base + (lane - my_head);
 }
 
 // Exclusive scan of the counts in two kernels: tile scans (the last tile
@@ -488,25 +526,32 @@ __global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __
                                                  const int* __restrict__ start,
                                                  int* __restrict__ perm_tmp) {
   if (!S->rebuild || S->halt) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S->n_stored) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   perm_tmp[start[keys[s]] + rank[s]] = s;
 }
 
 // Make the in-cell order the storage order (deterministic) and pack the
-// positions in cell order.  The number of live particles is start[dead_key].
-__global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_key,
+// positions in cell order.  A box's live particles are the slots before its
+// last key: n_alive = start[dead_key] - off.  Removed and unused slots keep
+// the order the counting sort gave them; nothing reads them.
+__global__ __launch_bounds__(256) void k_rank(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                              Buf A, const Scalars* S, BoxState* BS,
                                               const int* __restrict__ keys,
                                               const int* __restrict__ start,
                                               const int* __restrict__ perm_tmp,
                                               int* __restrict__ perm, double4* __restrict__ pk) {
   if (!S->rebuild || S->halt) return;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0) S->n_alive = start[dead_key];
-  if (k >= S->n_stored) return;
-  int s = perm_tmp[k];
-  int key = keys[s];
-  int a = start[key], b = start[key + 1];
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == P.off) BS[box].n_alive = start[P.dead_key] - P.off;
+  const int s = perm_tmp[k];
+  const int key = keys[s];
+  if (key == P.dead_key) {
+    perm[k] = s;
+    return;
+  }
+  const int a = start[key], b = start[key + 1];
   int r = 0;
   for (int j = a; j < b; j++) r += (perm_tmp[j] < s);
   perm[a + r] = s;
@@ -521,12 +566,15 @@ __global__ __launch_bounds__(256) void k_rank(Buf A, Scalars* S, int64_t dead_ke
 // over the cells of each level overlapping [x - reach, x + reach], reach =
 // ri + (largest radius of the level) + skin.  Entry q of particle k sits at
 // vlist[q*cap + k] (lanes of a wave read consecutive words).
-__global__ __launch_bounds__(256) void k_vlist(Params P, Scalars* S, const double4* __restrict__ pk,
+__global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                               Scalars* S, const BoxState* BS, const double4* __restrict__ pk,
                                                const int* __restrict__ cstart, int* __restrict__ vcnt,
                                                int* __restrict__ vlist) {
   if (!S->rebuild || S->halt) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S->n_alive) return;
+  if (k - P.off >= BS[box].n_alive) return;
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
@@ -665,7 +713,9 @@ __device__ __forceinline__ int cl_id(const Buf& A, const int* __restrict__ fid, 
   return (e & CL_FIXED) ? fid[e & CL_INDEX] : A.id[e & CL_INDEX];
 }
 
-__global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
+__global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
+                                                 const int* __restrict__ blk_box, Buf A, Scalars* S,
+                                                 const BoxState* BS,
                                                  const int* __restrict__ perm,
                                                  const int* __restrict__ vcnt,
                                                  const int* __restrict__ vlist,
@@ -675,8 +725,10 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
                                                  const int* __restrict__ fid,
                                                  const int* __restrict__ fstart) {
   if (S->halt || S->overflow == 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= S->n_alive) return;
+  if (k - P.off >= BS[box].n_alive) return;
   const int nq = min(vcnt[k], P.lcap);
   const bool fixed = P.search_fixed && P.nfix > 0;
   if (nq == 0 && !fixed) {  // loose beds: most particles have no candidate at all
@@ -779,7 +831,9 @@ __global__ __launch_bounds__(256) void k_contact(Params P, Buf A, Scalars* S,
   }
 }
 
-__global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* S,
+__global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
+                                               const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
+                                               BoxState* BS,
                                                const int* __restrict__ perm,
                                                const unsigned* __restrict__ clist,
                                                const int* __restrict__ ccnt,
@@ -787,15 +841,16 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
                                                const int* __restrict__ fstart,
-                                               Slot* __restrict__ slots) {
+                                               Slot* __restrict__ slots, BoxSlot* __restrict__ bslots) {
   // a candidate-list overflow in this step's rebuild leaves the lists
   // incomplete: do nothing, the host regrows them and replays the step
   if (S->halt || S->overflow == 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = S->n_alive;
   const bool rb = S->rebuild != 0;
   // k_ctl has advanced iterNumber: is the next step a near-wall update?
-  const bool upd_next = S->iter >= S->next_wall_update;
+  const bool upd_next = S->iter >= BS[box].next_wall_update;
   Acc acc;
   acc.F = D3(0.0, 0.0, 0.0);
   acc.T = D3(0.0, 0.0, 0.0);
@@ -804,7 +859,7 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
   acc.overflow = 0;
   double move2 = 0.0, vnext = 0.0, anext = 0.0;
   bool dead = false;
-  if (k < n) {
+  if (k - P.off < BS[box].n_alive) {
     const int cap = P.cap;
     const int s = rb ? perm[k] : k;
     const double4 pi = A.pos[s];
@@ -1034,80 +1089,75 @@ __global__ __launch_bounds__(256) void k_force(Params P, Buf A, Buf B, Scalars* 
   }
   if (threadIdx.x == 0) {
     Slot& sl = slots[blockIdx.x % NSLOT];
+    BoxSlot& bl = bslots[box * NSLOT + blockIdx.x % NSLOT];
     if (bmove > 0.0) atomicMax(&sl.step_bits, dbits(sqrt(bmove)));
+    if (bdel) atomicAdd(&sl.ndel, bdel);
     if (upd_next) {
-      atomicMax(&sl.max_v_bits, dbits(bv));
-      atomicMax(&sl.max_a_bits, dbits(ba));
+      atomicMax(&bl.max_v_bits, dbits(bv));
+      atomicMax(&bl.max_a_bits, dbits(ba));
     }
     for (int q = 0; q < 4; q++)
-      if (bc[q]) atomicAdd(&sl.cnt[q], bc[q]);
-    if (bdel) atomicAdd(&sl.ndel, bdel);
+      if (bc[q]) atomicAdd(&bl.cnt[q], bc[q]);
+    if (bdel) atomicAdd(&bl.ndel, bdel);
     if (blockIdx.x == 0) S->have_parts = 1;
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
 }
 
 // ---------------------------------------------------------------------------
-// host-side launchers (called from sdem_api.cpp)
+// host-side launchers (called from sdem_api.cpp).  Particle kernels cover all
+// `nslots` storage slots (a multiple of BOX_ALIGN) with blocks of `bs`
+// threads (a divisor of BOX_ALIGN).
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t launch_pre(const Params& P, const Buf& A, Scalars* S, Slot* slots, int n_upper,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_pre, dim3(nblk(n_upper > 0 ? n_upper : 1, 256)), dim3(256), 0, st, P, A, S, slots);
+hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_pre, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS,
+                     D.bslots);
   return hipGetLastError();
 }
 
-hipError_t launch_ctl(const Params& P, Scalars* S, Slot* slots, hipStream_t st) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, P, S, slots);
+hipError_t launch_ctl(const Dev& D, hipStream_t st) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.slots, D.BS, D.bslots, D.nbox);
+  if (D.nbox > 1) hipLaunchKernelGGL(k_box, dim3(D.nbox), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots);
   return hipGetLastError();
 }
 
-hipError_t launch_finish(Scalars* S, const Slot* slots, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, S, slots);
+hipError_t launch_finish(const Dev& D, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish, dim3(D.nbox), dim3(64), 0, st, D.S, D.BS, D.bslots);
   return hipGetLastError();
 }
 
-hipError_t launch_sort(const Params& P, const Buf& A, Scalars* S, int n_upper, int64_t dead_key,
-                       int* keys, int* rank, int* counts, int* start, int* tile_sum, int* perm_tmp,
-                       int* perm, double4* pk, hipStream_t st) {
-  const int64_t n = dead_key + 2;
+hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
+  const int64_t n = D.nkeys + 1;
   const unsigned nt = nblk(n, SCAN_TILE);
-  if (n_upper > 0)
-    hipLaunchKernelGGL(k_bin, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, dead_key, keys,
-                       rank, counts);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, counts, start, tile_sum, n, S);
-  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, counts, start, tile_sum, n, S);
-  if (n_upper > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n_upper, 256)), dim3(256), 0, st, S, keys, rank, start,
-                       perm_tmp);
-    hipLaunchKernelGGL(k_rank, dim3(nblk(n_upper, 256)), dim3(256), 0, st, A, S, dead_key, keys,
-                       start, perm_tmp, perm, pk);
-  }
+  const unsigned nb = nblk(D.nslots, 256);
+  hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.rank,
+                     D.counts);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
+  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.S, D.keys, D.rank, D.start, D.perm_tmp);
+  hipLaunchKernelGGL(k_rank, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.start,
+                     D.perm_tmp, D.perm, D.pk);
   return hipGetLastError();
 }
 
-hipError_t launch_vlist(const Params& P, Scalars* S, int n_upper, const double4* pk,
-                        const int* cstart, int* vcnt, int* vlist, hipStream_t st) {
-  if (n_upper > 0)
-    hipLaunchKernelGGL(k_vlist, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, S, pk, cstart, vcnt, vlist);
+hipError_t launch_vlist(const Dev& D, hipStream_t st) {
+  hipLaunchKernelGGL(k_vlist, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box, D.S, D.BS, D.pk,
+                     D.start, D.vcnt, D.vlist);
   return hipGetLastError();
 }
 
-hipError_t launch_force(const Params& P, const Buf& A, const Buf& B, Scalars* S, int n_upper,
-                        const int* perm, const int* vcnt, const int* vlist, unsigned* clist,
-                        int* ccnt, const double4* fpk, const int* fid,
-                        const int* ftype, const int* fstart, Slot* slots, hipStream_t st,
-                        hipEvent_t mid) {
-  if (n_upper > 0) {
-    hipLaunchKernelGGL(k_contact, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, S, perm, vcnt,
-                       vlist, clist, ccnt, fpk, fid, fstart);
-    if (mid) (void)hipEventRecord(mid, st);
-    hipLaunchKernelGGL(k_force, dim3(nblk(n_upper, 256)), dim3(256), 0, st, P, A, B, S, perm, clist,
-                       ccnt, fpk, fid, ftype, fstart, slots);
-  }
+hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid) {
+  hipLaunchKernelGGL(k_contact, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
+                     D.blk_box, A, D.S, D.BS, D.perm, D.vcnt, D.vlist, D.clist, D.ccnt, D.fpk, D.fid,
+                     D.fstart);
+  if (mid) (void)hipEventRecord(mid, st);
+  hipLaunchKernelGGL(k_force, dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A,
+                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.fstart, D.slots,
+                     D.bslots);
   return hipGetLastError();
 }
 

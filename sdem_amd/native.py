@@ -113,6 +113,12 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "get_device_bytes": ([V, _LP], C.c_int),
         "set_tuning": ([V, _DP, C.c_double, C.c_int32], C.c_int),
         "set_capacity": ([V, C.c_int32, C.c_int32], C.c_int),
+        "create_ensemble": ([C.c_int32, C.POINTER(Options), C.POINTER(V)], C.c_int),
+        "get_num_boxes": ([V, _IP], C.c_int),
+        "select_box": ([V, C.c_int32], C.c_int),
+        "set_block_sizes": ([V, C.c_int32, C.c_int32], C.c_int),
+        "get_tuning": ([V, _DP, _DP, _IP, _IP], C.c_int),
+        "autotune": ([V, C.c_int32, _IP, _DP], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = f(name)
@@ -142,19 +148,28 @@ def _i32(a) -> np.ndarray:
 class Engine:
     """One context of a library exporting the include/sdem.h entry points."""
 
-    def __init__(self, lib: C.CDLL, options: Options, prefix: str = "sdem_"):
+    def __init__(self, lib: C.CDLL, options, prefix: str = "sdem_"):
+        """options: one Options (a plain context) or a sequence of them (an
+        ensemble of independent boxes, sdem_create_ensemble)."""
         self.lib, self.prefix = lib, prefix
         if not getattr(lib, "_sdem_declared_" + prefix, False):
             _declare(lib, prefix)
             setattr(lib, "_sdem_declared_" + prefix, True)
-        self.opts = options
-        self.tsize = level_size(options.pi_method)
-        self.rsize = level_size(options.pri_method)
+        many = not isinstance(options, Options)
+        first = options[0] if many else options
+        self.opts = first
+        self.tsize = level_size(first.pi_method)
+        self.rsize = level_size(first.pri_method)
         h = C.c_void_p()
-        rc = self._fn("create")(C.byref(options), C.byref(h))
+        if many:
+            arr = (Options * len(options))(*options)
+            rc = self._fn("create_ensemble")(len(options), arr, C.byref(h))
+        else:
+            rc = self._fn("create")(C.byref(options), C.byref(h))
         if rc != SDEM_OK:
             raise SdemError(rc, prefix + "create", "context creation failed")
         self.h = h
+        self.nbox = len(options) if many else 1
 
     # ------------------------------------------------------------------ util
     def _fn(self, name):
@@ -292,6 +307,29 @@ class Engine:
         self._check(self._fn("set_tuning")(self.h, _dptr(cf), float(skin), int(keep_forces)),
                     "set_tuning")
 
+    def select(self, box: int) -> None:
+        """Address box `box` of an ensemble with the setters and readers."""
+        self._check(self._fn("select_box")(self.h, int(box)), "select_box")
+
+    def set_block_sizes(self, contact_block=256, force_block=256) -> None:
+        self._check(self._fn("set_block_sizes")(self.h, int(contact_block), int(force_block)),
+                    "set_block_sizes")
+
+    def get_tuning(self) -> dict:
+        cf = np.zeros(3)
+        sk = C.c_double(0)
+        bc, bf = C.c_int32(0), C.c_int32(0)
+        self._check(self._fn("get_tuning")(self.h, _dptr(cf), C.byref(sk), C.byref(bc), C.byref(bf)),
+                    "get_tuning")
+        return dict(cell_factor=tuple(cf), skin=sk.value, contact_block=bc.value, force_block=bf.value)
+
+    def autotune(self, steps_per_trial: int = 10):
+        """Try the candidate tunings on real steps, keep the fastest; returns
+        (steps run, best ms per step)."""
+        n, ms = C.c_int32(0), C.c_double(0)
+        self._check(self._fn("autotune")(self.h, int(steps_per_trial), C.byref(n), C.byref(ms)), "autotune")
+        return n.value, ms.value
+
     def set_capacity(self, hist_slots=0, list_slots=0) -> None:
         self._check(self._fn("set_capacity")(self.h, int(hist_slots), int(list_slots)), "set_capacity")
 
@@ -331,16 +369,9 @@ def make_options(lib: C.CDLL, prefix: str = "sdem_", **kw) -> Options:
     return o
 
 
-def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None,
-                 capacity: Optional[dict] = None) -> Engine:
-    """Create an engine and push a scenario (see sdem_amd.scenarios) into it,
-    in the reference's initialisation order (DEMS_Initialize :58-195)."""
-    o = make_options(lib, prefix, **scenario["options"])
-    eng = Engine(lib, o, prefix)
-    if tuning and hasattr(lib, prefix + "set_tuning"):
-        eng.set_tuning(**tuning)
-    if capacity and hasattr(lib, prefix + "set_capacity"):
-        eng.set_capacity(**capacity)
+def _push(eng: Engine, scenario: dict) -> None:
+    """Properties, walls, fixed particles, state and contacts of one box, in
+    the reference's initialisation order (DEMS_Initialize :58-195)."""
     if "linear" in scenario:
         eng.set_linear_stiffness(*scenario["linear"])
     pt = scenario["ptypes"]
@@ -356,4 +387,29 @@ def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Opt
     ct = scenario.get("contacts")
     if ct is not None and len(ct["kind"]):
         eng.set_contacts(ct["kind"], ct["owner"], ct["partner"], ct["tan_delta"], ct["vrel_init"])
+
+
+def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None,
+                 capacity: Optional[dict] = None) -> Engine:
+    """Create an engine and push a scenario (see sdem_amd.scenarios) into it."""
+    o = make_options(lib, prefix, **scenario["options"])
+    eng = Engine(lib, o, prefix)
+    if tuning and hasattr(lib, prefix + "set_tuning"):
+        eng.set_tuning(**tuning)
+    if capacity and hasattr(lib, prefix + "set_capacity"):
+        eng.set_capacity(**capacity)
+    _push(eng, scenario)
+    return eng
+
+
+def build_ensemble(lib: C.CDLL, scenarios_: list, tuning: Optional[dict] = None) -> Engine:
+    """An ensemble context (sdem_create_ensemble) holding one box per scenario."""
+    opts = [make_options(lib, "sdem_", **sc["options"]) for sc in scenarios_]
+    eng = Engine(lib, opts)
+    if tuning:
+        eng.set_tuning(**tuning)
+    for b, sc in enumerate(scenarios_):
+        eng.select(b)
+        _push(eng, sc)
+    eng.select(0)
     return eng

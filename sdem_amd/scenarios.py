@@ -318,3 +318,58 @@ def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
                             poisson=[0.3] * 3),
                 pp=(0.5, 0.5, 0.0, 0.8, 0.8), wtypes=dict(youngs=[5.0e7], poisson=[0.3]),
                 pw=(0.5, 0.5, 0.0, 0.8, 0.8), walls=walls, state=st)
+
+
+# --------------------------------------------------------------------------
+# batched ensemble of independent sand boxes
+# --------------------------------------------------------------------------
+def ensemble_box(n: int, seed: int, model: str = "hm", fric: float = 0.4, rest: float = 0.8,
+                 kn: float = 1.0e5, phi: float = 0.3, ntypes: int = 8, dt: float = 1.0e-5) -> Dict:
+    """One closed sand box of an ensemble: n spheres with diameters U[1.5,3]
+    mm in `ntypes` types, density 2500, a random loose pack at solid fraction
+    `phi` filling a 1:1:2 box closed by 6 planes, gravity -z, AB2, dt 1e-5 s.
+    model "hm": Hertz-Mindlin, E=5e7 Pa, nu=0.3; model "linear": constant kn,
+    kt = 0.8 kn.  Friction and restitution as given (particles and walls)."""
+    rng = np.random.default_rng(seed)
+    edges = np.linspace(1.5e-3, 3.0e-3, ntypes + 1)
+    centers = 0.5 * (edges[:-1] + edges[1:])
+    d = rng.uniform(1.5e-3, 3.0e-3, n)
+    t, dq = discretise(d, edges, centers)
+    vol_solid = float(np.sum(np.pi / 6 * dq ** 3))
+    s = (vol_solid / phi / 2.0) ** (1 / 3)
+    s = max(s, 6 * 3.0e-3)
+    box = (s, s, 2 * s)
+    height = min(vol_solid / phi / (s * s), box[2] * 0.999)
+    pos = rsa_pack(dq / 2, (0.0, 0.0, 0.0), (s, s, height), seed)
+    order = np.lexsort((pos[:, 0], pos[:, 1], pos[:, 2]))
+    pos, t, dq = pos[order], t[order], dq[order]
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    st = zero_state(ids, t, np.concatenate([pos, dq[:, None] / 2], 1))
+    cf = CFT_LINEAR if model == "linear" else CFT_NLIN
+    opts = base_options(dt=dt, domain_max=box, num_prtcl=n, cf_type=cf)
+    sc = dict(options=opts,
+              ptypes=dict(diameter=list(centers), density=[2500.0] * ntypes, youngs=[5.0e7] * ntypes,
+                          poisson=[0.3] * ntypes),
+              pp=(fric, fric, 0.0, rest, rest),
+              wtypes=dict(youngs=[5.0e7], poisson=[0.3]), pw=(fric, fric, 0.0, rest, rest),
+              walls=box_walls((0, 0, 0), box), state=st, meta=dict(model=model, box=box))
+    if cf == CFT_LINEAR:
+        sc["linear"] = (kn, 0.8)
+    return sc
+
+
+def ensemble(nbox: int = 256, seed: int = DEFAULT_SEED, nmin: int = 2000, nmax: int = 50000) -> List[Dict]:
+    """The batched-ensemble workload: `nbox` boxes with log-uniform particle
+    counts in [nmin, nmax], friction U[0.2,0.6], restitution U[0.5,0.95];
+    even boxes use the linear model with kn U[5e4,2e5] N/m, odd boxes
+    Hertz-Mindlin with E=5e7 Pa."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(nbox):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        fric = float(rng.uniform(0.2, 0.6))
+        rest = float(rng.uniform(0.5, 0.95))
+        kn = float(rng.uniform(5.0e4, 2.0e5))
+        out.append(ensemble_box(n, seed + 7919 * (b + 1), model="linear" if b % 2 == 0 else "hm",
+                                fric=fric, rest=rest, kn=kn))
+    return out
