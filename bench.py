@@ -95,11 +95,13 @@ def ensemble_bench(args):
           file=sys.stderr, flush=True)
     lib = native.load_hip_library()
     eng = native.build_ensemble(lib, scs, tuning=dict(keep_forces=0))
+    # warm up, then let the autotuner pick the shapes on the moving packs
+    # (its trials are ordinary steps of the run)
+    eng.iterate(args.warmup)
     t0 = time.perf_counter()
-    tuned_steps, tuned_ms = eng.autotune(steps_per_trial=10)
+    tuned_steps, tuned_ms = eng.autotune(steps_per_trial=20)
     print(f"# autotune: {tuned_steps} steps in {time.perf_counter() - t0:.1f}s, best {tuned_ms:.3f} ms/step, "
           f"{eng.get_tuning()}", file=sys.stderr, flush=True)
-    eng.iterate(args.warmup)
     t0 = time.perf_counter()
     done = 0
     chunk = max(2, (args.steps // 10) // 2 * 2)

@@ -221,7 +221,7 @@ struct sdem_ctx {
   double cell_factor[3] = {1.0, 1.0, 1.0};
   double skin_factor = 0.1;  // candidate-list skin as a fraction of a box's d_max
   int keep_forces = 1;
-  int bs_contact = 256, bs_force = 256;
+  int bs_contact = 256, bs_force = 128;
   int hist_slots_hint = 0, list_slots_hint = 0;
 
   // device
@@ -1175,7 +1175,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.iter = s.iter_at_begin;
     s.need_rebuild = 1;
     s.have_parts = 0;
-    s.ticket_scan = 0;
+
     s.list_need = 0;
     s.overflow = 0;
     s.halt = 0;
@@ -1504,7 +1504,7 @@ int sdem_autotune(sdem_ctx* c, int32_t steps_per_trial, int32_t* steps_run, doub
   struct Cand { double cf, skin; int bc, bf; };
   std::vector<Cand> cands;
   for (double cf : {0.5, 0.75, 1.0})
-    for (double sk : {0.03, 0.06, 0.12})
+    for (double sk : {0.03, 0.06, 0.12, 0.2})
       for (int bf : {128, 256}) cands.push_back({cf, sk, 256, bf});
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)

@@ -146,7 +146,7 @@ struct Scalars {
   int need_rebuild;        // forced rebuild at the next step
   int rebuilds;            // rebuilds so far (statistics)
   int have_parts;          // the force kernel's slots are current
-  unsigned ticket_scan;    // last-block election in k_scan_tiles
+  unsigned spare;
   int list_need;           // largest candidate count seen when a list overflowed
   int pad;
   double rebuild_disp;     // smallest half skin over the boxes (with a margin)

@@ -14,7 +14,7 @@
 //                             near-wall cadence and advances iterNumber
 //   -- rebuild steps only (the kernels return at once otherwise) --
 //   k_bin                   : (level, cell) key per particle, per-key counts
-//   k_scan_tiles/k_scan_add : exclusive scan of the counts
+//   k_scan_tiles/_sums/_add : exclusive scan of the counts
 //   k_scatter / k_rank      : deterministic counting sort (in-cell order =
 //                             storage order) + packed positions in cell order
 //   k_vlist                 : candidate lists, pairs within ri+rj+skin
@@ -459,18 +459,17 @@ __global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ PB, cons
   rank[s] = base + (lane - my_head);
 }
 
-// Exclusive scan of the counts in two kernels: tile scans (the last tile
-// block then scans the tile sums), and the tile offsets added (while the
-// counts are zeroed for the next rebuild).
+// Exclusive scan of the counts in three kernels: tile scans, the scan of the
+// tile sums, and the tile offsets added (while the counts are zeroed for the
+// next rebuild).
 constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out,
                                                         int* __restrict__ tile_sum, int64_t n,
-                                                        Scalars* S) {
+                                                        const Scalars* S) {
   if (!S->rebuild || S->halt) return;
   using BS = hipcub::BlockScan<int, SCAN_T>;
   __shared__ typename BS::TempStorage tmp;
-  __shared__ bool last;
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0;
@@ -478,25 +477,23 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ i
   BS(tmp).ExclusiveSum(v, v, total);
   for (int i = 0; i < SCAN_ITEMS; i++)
     if (base + i < n) out[base + i] = v[i];
-  if (threadIdx.x == 0) {
-    tile_sum[blockIdx.x] = total;
-    __threadfence();
-    last = atomicAdd(&S->ticket_scan, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // the last tile block scans the tile sums
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums, one block (a kernel of its own: a
+// last-block election would need a device-wide fence in every tile block)
+__global__ __launch_bounds__(SCAN_T) void k_scan_sums(int* __restrict__ tile_sum, int ntiles, const Scalars* S) {
+  if (!S->rebuild || S->halt) return;
+  using BS = hipcub::BlockScan<int, SCAN_T>;
+  __shared__ typename BS::TempStorage tmp;
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  const int ntiles = gridDim.x;
   for (int start = 0; start < ntiles; start += SCAN_TILE) {
     const int b0 = start + threadIdx.x * SCAN_ITEMS;
     int w[SCAN_ITEMS];
     for (int i = 0; i < SCAN_ITEMS; i++) w[i] = (b0 + i < ntiles) ? tile_sum[b0 + i] : 0;
     int tot;
-    __syncthreads();
     BS(tmp).ExclusiveSum(w, w, tot);
     const int c0 = carry;
     for (int i = 0; i < SCAN_ITEMS; i++)
@@ -505,7 +502,6 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ i
     if (threadIdx.x == 0) carry = c0 + tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) S->ticket_scan = 0;
 }
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ in, int* __restrict__ out,
@@ -1137,6 +1133,7 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
   hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.rank,
                      D.counts);
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, st, D.tile_sum, (int)nt, D.S);
   hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.S, D.keys, D.rank, D.start, D.perm_tmp);
   hipLaunchKernelGGL(k_rank, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.start,

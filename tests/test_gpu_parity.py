@@ -105,13 +105,16 @@ def test_two_runs_are_bitwise_identical(hip_lib):
 
 
 def test_tuning_does_not_change_bits(hip_lib):
-    """Contact forces are summed in ascending partner id, so the cell shape and
-    the candidate-list skin leave every bit of the result unchanged."""
+    """Contact forces are summed in ascending partner id, so the cell shape,
+    the candidate-list skin and the block sizes leave every bit of the result
+    unchanged."""
     print("tuning independence: 3 settings x 20 steps", flush=True)
     sc = scenarios.random_cloud(n=1500, seed=13, periodic=(1, 1, 0), walls="z", infinite_walls=True)
     out = []
-    for cf, skin in [((1, 1, 1), 0.1), ((0.5, 0.75, 0.6), 0.0), ((0.3, 0.5, 1.0), 0.3)]:
+    for (cf, skin), bs in zip([((1, 1, 1), 0.1), ((0.5, 0.75, 0.6), 0.0), ((0.3, 0.5, 1.0), 0.3)],
+                              [(256, 256), (128, 64), (64, 128)]):
         g = native.build_engine(hip_lib, sc, tuning=dict(cell_factor=cf, skin=skin))
+        g.set_block_sizes(*bs)
         g.iterate(20)
         st, ct = g.get_state(), g.get_contacts()
         out.append((st["posr"].tobytes(), st["lin_vel"].tobytes(), st["rot_vel"].tobytes(),
