@@ -1570,3 +1570,77 @@ void sdem_destroy(sdem_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+// ===========================================================================
+// Access for the learned surrogate (gnn_api.cpp), single-box contexts only.
+// ===========================================================================
+struct sdem_dem_view {
+  double4* pos;
+  double* lv;
+  double* la;
+  int cap, n;
+  double dt, g[3], lo[3], hi[3];
+  int per[3];
+  hipStream_t stream;
+};
+
+extern "C" int sdem_internal_view(sdem_ctx* c, sdem_dem_view* v) {
+  if (!c || !v) return SDEM_E_ARG;
+  if (c->boxes.size() != 1) return c->fail(SDEM_E_STATE, "the surrogate drives single-box contexts");
+  if (int rc = prepare(c)) return rc;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  BoxState B;
+  if (e == hipSuccess) e = hipMemcpy(&B, c->BS, sizeof(B), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "surrogate view");
+  const Box& b = c->boxes[0];
+  Buf& A = c->buf[c->cur];
+  v->pos = A.pos;
+  v->lv = A.lv;
+  v->la = A.la;
+  v->cap = (int)c->nslots;
+  v->n = B.n_alive;
+  v->dt = b.o.dt;
+  for (int d = 0; d < 3; d++) {
+    v->g[d] = b.o.gravity[d];
+    v->lo[d] = b.o.domain_min[d];
+    v->hi[d] = b.o.domain_max[d];
+    v->per[d] = b.o.is_periodic[d] ? 1 : 0;
+  }
+  v->stream = c->stream;
+  return SDEM_OK;
+}
+
+// The state in buffer A was advanced by `steps` steps outside the DEM
+// kernels: the velocity history restarts from the new velocity, contact
+// history is cleared, the near-wall list and the candidate lists are
+// rebuilt at the next step, and iterNumber moves on.
+extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps) {
+  if (!c || steps < 0) return SDEM_E_ARG;
+  Buf& A = c->buf[c->cur];
+  const size_t cap = (size_t)c->nslots;
+  hipError_t e = hipSuccess;
+  for (int l = 1; l < c->tsize && e == hipSuccess; l++)
+    e = hipMemcpy(A.lv + (size_t)l * 3 * cap, A.lv, 3 * cap * sizeof(double), hipMemcpyDeviceToDevice);
+  if (e == hipSuccess) e = hipMemset(A.hcnt, 0, cap * sizeof(int));
+  Scalars s;
+  BoxState B;
+  if (e == hipSuccess) e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&B, c->BS, sizeof(B), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(c, e, "surrogate update");
+  s.iter += steps;
+  s.need_rebuild = 1;
+  s.have_parts = 0;
+  s.disp_acc = 0.0;
+  B.next_wall_update = s.iter;
+  B.n_stored = B.n_alive;
+  for (int k = 0; k < 4; k++) B.num_cntcts[k] = 0;
+  e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->BS, &B, sizeof(B), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(c->slots, 0, NSLOT * sizeof(Slot));
+  if (e == hipSuccess) e = hipMemset(c->bslots, 0, NSLOT * sizeof(BoxSlot));
+  if (e != hipSuccess) return hip_fail(c, e, "surrogate update");
+  c->need_pre = true;
+  c->stepped = true;
+  c->first_step_pending = false;
+  return SDEM_OK;
+}

@@ -1,0 +1,157 @@
+"""The surrogate's kernels on the GPU against numpy and the CPU restatement
+of the network (oracle/gnn_oracle.py, pinned by finite differences)."""
+import numpy as np
+import pytest
+
+from oracle import gnn_oracle as G
+from sdem_amd import gnn, native, scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(x):
+    """bf16 bit patterns of float values that bf16 holds exactly"""
+    return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def test_mfma_lane_layout():
+    """v_mfma_f32_32x32x16_bf16: A (32x16), B (16x32) given lane by lane as the
+    kernels load them; D read back with the layout the GEMM epilogue uses."""
+    print("mfma probe", flush=True)
+    rng = np.random.default_rng(0)
+    A = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    B = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    a = np.zeros((64, 8), np.float32)
+    b = np.zeros((64, 8), np.float32)
+    for l in range(64):
+        for j in range(8):
+            a[l, j] = A[l % 32, 8 * (l // 32) + j]
+            b[l, j] = B[8 * (l // 32) + j, l % 32]
+    d = gnn.mfma_probe(bits(a), bits(b))
+    D = A @ B
+    for l in range(64):
+        for r in range(16):
+            assert d[l, r] == D[8 * (r // 4) + 4 * (l // 32) + r % 4, l % 32], (l, r)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("shape", [(70, 45, 37), (128, 128, 384), (5, 200, 16)])
+def test_gemm_against_numpy(ta, tb, shape):
+    M, N, K = shape
+    print(f"gemm ta={ta} tb={tb} {shape}", flush=True)
+    rng = np.random.default_rng(M + N + K)
+    A = G.bf16(rng.normal(size=(M, K)))
+    B = G.bf16(rng.normal(size=(K, N)))
+    bias = rng.normal(size=N).astype(np.float32)
+    a_st = A.T.copy() if ta else A
+    b_st = B.T.copy() if tb else B
+    c, pre = gnn.gemm(a_st, b_st, ta=ta, tb=tb, bias=bias, flags=1 | 2 | 4)
+    ref_pre = A @ B + bias
+    # bf16 inputs are exact; only the fp32 accumulation order differs
+    tol = 1e-5 * np.sqrt(K) * (np.abs(A) @ np.abs(B) + 1)
+    assert np.all(np.abs(pre - ref_pre) <= tol)
+    assert np.allclose(c, G.silu(ref_pre), rtol=1e-5, atol=1e-4 * np.sqrt(K))
+
+
+def test_gemm_split_k():
+    print("gemm split-K", flush=True)
+    rng = np.random.default_rng(4)
+    A = G.bf16(rng.normal(size=(300, 50000)))  # stored as A^T: 50000 x 300 below
+    B = G.bf16(rng.normal(size=(50000, 64)))
+    c, _ = gnn.gemm(A.T.copy(), B, ta=True, splits=12)
+    ref = A @ B
+    assert np.allclose(c, ref, rtol=1e-4, atol=2e-3 * np.sqrt(50000))
+
+
+def random_graph(n=60, seed=2):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 3))
+    src, dst = [], []
+    for i in range(n):
+        d = np.linalg.norm(x - x[i], axis=1)
+        for j in np.nonzero((d < 0.3) & (d > 0))[0]:
+            src.append(j)
+            dst.append(i)
+    return (rng.normal(size=(n, 8)).astype(np.float32), np.array(src, np.int32), np.array(dst, np.int32),
+            rng.normal(size=(len(src), 7)).astype(np.float32))
+
+
+def model_and_oracle(hidden=32, layers=2, seed=3):
+    cfg = gnn.default_config(hidden=hidden, layers=layers)
+    m = gnn.Model(cfg, seed=seed)
+    offs, dims = m.layout()
+    ref = G.Net(m.params().astype(np.float64), offs, dims, layers, 6, bf16_inputs=True)
+    return m, ref
+
+
+def test_forward_matches_oracle():
+    print("gnn forward vs oracle", flush=True)
+    m, ref = model_and_oracle()
+    fn, src, dst, fe = random_graph()
+    m.set_graph(fn, src, dst, fe)
+    out = m.forward()
+    r = ref.forward(G.bf16(fn), src, dst, G.bf16(fe))
+    scale = np.abs(r).max()
+    assert np.abs(out - r).max() <= 1e-2 * scale, (np.abs(out - r).max(), scale)
+
+
+def test_gradients_match_oracle():
+    """Gradients of sum(out * dout) from the hand-written backward kernels
+    against the CPU restatement with the same bf16 rounding points, MLP by
+    MLP (relative L2 error and direction)."""
+    print("gnn gradients vs oracle", flush=True)
+    m, ref = model_and_oracle()
+    fn, src, dst, fe = random_graph()
+    m.set_graph(fn, src, dst, fe)
+    m.forward()
+    rng = np.random.default_rng(7)
+    dout = rng.normal(size=(fn.shape[0], 6)).astype(np.float32)
+    m.backward(dout)
+    g = m.grads().astype(np.float64)
+    ref.forward(G.bf16(fn), src, dst, G.bf16(fe))
+    gr = ref.backward(dout.astype(np.float64))
+    offs, dims = m.layout()
+    for k in range(len(offs)):
+        lo, hi = offs[k][0], offs[k][5] + dims[k][2]
+        a, b = g[lo:hi], gr[lo:hi]
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+        assert rel < 2e-2 and cos > 0.9998, (k, rel, cos)
+
+
+def test_graph_from_dem_matches_brute_force(hip_lib):
+    """Neighbour graph and features built on the GPU from a DEM state."""
+    print("gnn graph from a DEM state", flush=True)
+    sc = scenarios.settling(n=20000, seed=5)
+    eng = native.build_engine(hip_lib, sc)
+    eng.iterate(3)
+    st = eng.get_state()
+    cfg = gnn.default_config(hidden=32, layers=1, radius=2.6e-3, len_scale=2e-3, vel_scale=0.1,
+                             floor_z=0.0)
+    m = gnn.Model(cfg)
+    m.graph_from_dem(eng)
+    fn, src, dst, fe = m.get_graph()
+    # every pair within the radius (nearest periodic image), both directions
+    from scipy.spatial import cKDTree
+    p = st["posr"][:, :3] - np.array(sc["options"]["domain_min"])
+    L = np.array(sc["options"]["domain_max"]) - np.array(sc["options"]["domain_min"])
+    per = np.array(sc["options"]["is_periodic"], bool)
+    box = np.where(per, L, 1e6)
+    pairs = cKDTree(np.mod(p, box), boxsize=box).query_pairs(2.6e-3)
+    assert len(src) == 2 * len(pairs)
+    assert np.all(np.diff(dst) >= 0)
+    # edge features: |dx| / len_scale within the radius, consistent with dx
+    assert np.all(fe[:, 3] <= 2.6e-3 / 2e-3 * 1.01)
+    assert np.allclose(np.sqrt((fe[:, :3].astype(np.float64) ** 2).sum(1)), fe[:, 3], rtol=2e-2, atol=1e-3)
+
+
+def test_training_reduces_the_loss():
+    """Hand-written training on the GPU fits a smooth per-node target."""
+    print("gnn training", flush=True)
+    cfg = gnn.default_config(hidden=32, layers=2)
+    m = gnn.Model(cfg, seed=11)
+    fn, src, dst, fe = random_graph(n=200, seed=9)
+    target = np.tanh(fn[:, :6] * 0.5).astype(np.float32)
+    m.set_graph(fn, src, dst, fe)
+    losses = [m.train_step(target, lr=3e-3) for _ in range(150)]
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
