@@ -3,8 +3,9 @@
  *
  * No counterpart in the reference: a graph network that advances a DEM
  * state by `nsub` time steps per call, trained on trajectories of the plain
- * DEM path.  Graph: every pair of particles closer than `radius` (both
- * directions, nearest periodic image).  Network: encoders (node, edge), then
+ * DEM path.  Graph: every pair of particles i, j with |x_i - x_j| <=
+ * cutoff_factor * 2 max(r_i, r_j) (both directions, nearest periodic image;
+ * `radius` must bound that distance).  Network: encoders (node, edge), then
  * `layers` message-passing layers (edge update e += MLP([e, h_src, h_dst]),
  * node update h += MLP([h, sum of incoming e])), then a decoder; every MLP is
  * Linear-SiLU-Linear-SiLU-Linear with `hidden` units.  bf16 weights on the
@@ -32,7 +33,10 @@ extern "C" {
 #endif
 
 typedef struct sdem_gnn_config {
-  double vel_scale, acc_scale, len_scale, radius, floor_z, dx_scale, dv_scale;
+  double vel_scale, acc_scale, len_scale;
+  double radius;         /* largest cut-off: the graph search's cell edge */
+  double cutoff_factor;  /* edge i-j when |x_i - x_j| <= cutoff_factor * 2 max(r_i, r_j) */
+  double floor_z, dx_scale, dv_scale;
   int32_t hidden, layers, node_in, edge_in, out, nsub;
 } sdem_gnn_config;
 
@@ -58,8 +62,13 @@ int sdem_gnn_get_layout(sdem_gnn* m, int32_t nmlp_cap, int32_t* nmlp, int64_t* o
  * by dst (receiver), edge features ne x edge_in */
 int sdem_gnn_set_graph(sdem_gnn* m, int32_t n, const float* node_feat, int64_t ne, const int32_t* src,
                        const int32_t* dst, const float* edge_feat);
-/* graph and features from the state of a DEM context (its first box) */
-int sdem_gnn_graph_from_dem(sdem_gnn* m, sdem_ctx* dem, int64_t* nedges);
+/* graph and features from the state of a DEM context (its first box);
+ * nodes in the context's storage order, whose particle ids
+ * sdem_gnn_get_node_ids gives.  for_training = 0 keeps only one layer's
+ * activations (inference: no backward pass on this graph). */
+int sdem_gnn_graph_from_dem(sdem_gnn* m, sdem_ctx* dem, int32_t for_training, int64_t* nedges);
+int sdem_gnn_get_node_ids(sdem_gnn* m, int32_t cap, int32_t* ids);
+int sdem_gnn_get_num_nodes(sdem_gnn* m, int32_t* n, int64_t* ne);
 int sdem_gnn_get_graph(sdem_gnn* m, int64_t cap, int32_t* src, int32_t* dst, float* node_feat, float* edge_feat);
 
 int sdem_gnn_forward(sdem_gnn* m, float* out /* n x out, may be null */);

@@ -26,6 +26,7 @@ struct sdem_dem_view {
   double4* pos;
   double* lv;
   double* la;
+  int* id;
   int cap, n;
   double dt, g[3], lo[3], hi[3];
   int per[3];
@@ -90,6 +91,9 @@ struct sdem_gnn {
   int search_n = 0;
   int gcap_n = 0;
   int64_t gcap_e = 0;
+  bool train_alloc = false;  // activations of every layer kept (training)
+  std::vector<int> ids;      // particle ids of the nodes of a graph taken from a DEM state
+  int ids_cap = 0;
   std::vector<void*> graph_allocs;
   std::vector<void*> allocs;
   int fail(int code, const std::string& s) {
@@ -202,10 +206,14 @@ hipError_t mlp_bwd(sdem_gnn* M, const MLP& p, const Act& a, const float* dy, flo
 
 int L_of(const sdem_gnn* M) { return M->cfg.layers; }
 
-// activation buffers for n nodes and ne edges
-int alloc_graph(sdem_gnn* M, int n, int64_t ne) {
-  if (n <= M->gcap_n && ne <= M->gcap_e) return SDEM_OK;
+// Activation buffers for n nodes and ne edges.  Training keeps every
+// layer's activations for the backward pass; inference keeps one layer's
+// (residual streams ping-pong, the MLP buffers are shared by all layers).
+int alloc_graph(sdem_gnn* M, int n, int64_t ne, bool train) {
+  if (n <= M->gcap_n && ne <= M->gcap_e && train == M->train_alloc) return SDEM_OK;
   free_all(M->graph_allocs);
+  M->gcap_n = 0;
+  M->gcap_e = 0;
   const int H = M->cfg.hidden, L = L_of(M);
   const size_t N = (size_t)std::max(n, 1), E = (size_t)std::max<int64_t>(ne, 1);
   auto& A = M->graph_allocs;
@@ -213,34 +221,46 @@ int alloc_graph(sdem_gnn* M, int n, int64_t ne) {
   auto G = [&](auto** p, size_t k) { if (e == hipSuccess) e = galloc(A, p, k); };
   G(&M->esrc, E); G(&M->edst, E); G(&M->rstart, N + 1);
   G(&M->fn, N * PAD_IN); G(&M->fe, E * PAD_IN);
-  M->act.assign(M->mlp.size() + 0, Act{});
+  M->act.assign(M->mlp.size(), Act{});
   // MLP applications: 0 enc_node (N), 1 enc_edge (E), per layer edge (E), node (N), dec (N)
   for (size_t q = 0; q < M->mlp.size(); q++) {
     const MLP& p = M->mlp[q];
-    const bool on_edges = q == 1 || (q >= 2 && q < 2 + 2 * (size_t)L && (q - 2) % 2 == 0);
-    const size_t R = on_edges ? E : N;
+    const bool layer = q >= 2 && q < 2 + 2 * (size_t)L;
+    const bool on_edges = q == 1 || (layer && (q - 2) % 2 == 0);
     Act& a = M->act[q];
+    if (!train && layer && q >= 4) {  // share the first layer's buffers
+      a = M->act[q - 2];
+      continue;
+    }
+    const size_t R = on_edges ? E : N;
     a.R = (int)R;
     G(&a.pre1, R * p.hid); G(&a.pre2, R * p.hid); G(&a.a1, R * p.hid); G(&a.a2, R * p.hid);
     G(&a.out, R * p.out);
   }
-  M->h.assign(L + 1, nullptr);
-  M->e.assign(L + 1, nullptr);
-  M->xe.assign(L, nullptr);
-  M->xn.assign(L, nullptr);
-  for (int l = 0; l <= L; l++) { G(&M->h[l], N * H); G(&M->e[l], E * H); }
-  for (int l = 0; l < L; l++) { G(&M->xe[l], E * 3 * H); G(&M->xn[l], N * 2 * H); }
+  const int ns = train ? L + 1 : 2, nx = train ? L : 1;
+  M->h.assign(ns, nullptr);
+  M->e.assign(ns, nullptr);
+  M->xe.assign(nx, nullptr);
+  M->xn.assign(nx, nullptr);
+  for (int l = 0; l < ns; l++) { G(&M->h[l], N * H); G(&M->e[l], E * H); }
+  for (int l = 0; l < nx; l++) { G(&M->xe[l], E * 3 * H); G(&M->xn[l], N * 2 * H); }
   G(&M->agg, N * H);
-  const size_t R = std::max(N, E);
-  G(&M->dout, N * PAD_IN); G(&M->tmp_a, R * 3 * H); G(&M->tmp_b, R * 3 * H);
-  G(&M->dx_e, E * 3 * H); G(&M->dx_n, N * 2 * H);
-  G(&M->dh, N * H); G(&M->dh2, N * H); G(&M->de, E * H); G(&M->de2, E * H); G(&M->dagg, N * H);
-  G(&M->db16, R * 3 * H); G(&M->db16b, R * 3 * H);
-  M->part_len = (size_t)64 * 3 * H * H;
-  G(&M->part, M->part_len);
+  G(&M->dout, N * PAD_IN);
+  if (train) {
+    const size_t R = std::max(N, E);
+    G(&M->tmp_a, R * 3 * H); G(&M->tmp_b, R * 3 * H);
+    G(&M->dx_e, E * 3 * H); G(&M->dx_n, N * 2 * H);
+    G(&M->dh, N * H); G(&M->dh2, N * H); G(&M->de, E * H); G(&M->de2, E * H); G(&M->dagg, N * H);
+    G(&M->db16, R * 3 * H); G(&M->db16b, R * 3 * H);
+    M->part_len = (size_t)64 * 3 * H * H;
+    G(&M->part, M->part_len);
+  } else {
+    G(&M->db16, N * H);
+  }
   if (e != hipSuccess) return hipf(M, e, "hipMalloc(graph activations)");
   M->gcap_n = (int)N;
   M->gcap_e = (int64_t)E;
+  M->train_alloc = train;
   return SDEM_OK;
 }
 
@@ -249,39 +269,44 @@ int forward(sdem_gnn* M) {
   const int H = M->cfg.hidden, L = L_of(M), n = M->n;
   const int ne = (int)M->ne;
   hipStream_t st = M->stream;
+  const bool tr = M->train_alloc;
+  auto Hs = [&](int l) { return M->h[tr ? l : l % 2]; };
+  auto Es = [&](int l) { return M->e[tr ? l : l % 2]; };
+  auto XE = [&](int l) { return M->xe[tr ? l : 0]; };
+  auto XN = [&](int l) { return M->xn[tr ? l : 0]; };
   Act& an = M->act[0];
   an.R = n; an.x = M->fn;
   Act& ae = M->act[1];
   ae.R = ne; ae.x = M->fe;
   hipError_t e = mlp_fwd(M, M->mlp[0], an, nullptr);
-  if (e == hipSuccess) e = hipMemcpyAsync(M->h[0], an.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(Hs(0), an.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
   if (e == hipSuccess && ne > 0) e = mlp_fwd(M, M->mlp[1], ae, nullptr);
-  if (e == hipSuccess && ne > 0) e = hipMemcpyAsync(M->e[0], ae.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess && ne > 0) e = hipMemcpyAsync(Es(0), ae.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
   for (int l = 0; l < L && e == hipSuccess; l++) {
     Act& a_e = M->act[2 + 2 * l];
     Act& a_n = M->act[3 + 2 * l];
     if (ne > 0) {
       // edge update e' = e + MLP([e, h_src, h_dst])
-      e = pack_cols(M->e[l], nullptr, ne, H, M->xe[l], 3 * H, 0, st);
-      if (e == hipSuccess) e = pack_cols(M->h[l], M->esrc, ne, H, M->xe[l], 3 * H, H, st);
-      if (e == hipSuccess) e = pack_cols(M->h[l], M->edst, ne, H, M->xe[l], 3 * H, 2 * H, st);
-      a_e.R = ne; a_e.x = M->xe[l];
-      if (e == hipSuccess) e = mlp_fwd(M, M->mlp[2 + 2 * l], a_e, M->e[l]);
-      if (e == hipSuccess) e = hipMemcpyAsync(M->e[l + 1], a_e.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
-      if (e == hipSuccess) e = segsum(M->e[l + 1], M->rstart, n, H, M->agg, st);
+      e = pack_cols(Es(l), nullptr, ne, H, XE(l), 3 * H, 0, st);
+      if (e == hipSuccess) e = pack_cols(Hs(l), M->esrc, ne, H, XE(l), 3 * H, H, st);
+      if (e == hipSuccess) e = pack_cols(Hs(l), M->edst, ne, H, XE(l), 3 * H, 2 * H, st);
+      a_e.R = ne; a_e.x = XE(l);
+      if (e == hipSuccess) e = mlp_fwd(M, M->mlp[2 + 2 * l], a_e, Es(l));
+      if (e == hipSuccess) e = hipMemcpyAsync(Es(l + 1), a_e.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = segsum(Es(l + 1), M->rstart, n, H, M->agg, st);
     } else {
       e = hipMemsetAsync(M->agg, 0, (size_t)n * H * 4, st);
     }
     // node update h' = h + MLP([h, agg])
-    if (e == hipSuccess) e = pack_cols(M->h[l], nullptr, n, H, M->xn[l], 2 * H, 0, st);
-    if (e == hipSuccess) e = pack_cols(M->agg, nullptr, n, H, M->xn[l], 2 * H, H, st);
-    a_n.R = n; a_n.x = M->xn[l];
-    if (e == hipSuccess) e = mlp_fwd(M, M->mlp[3 + 2 * l], a_n, M->h[l]);
-    if (e == hipSuccess) e = hipMemcpyAsync(M->h[l + 1], a_n.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = pack_cols(Hs(l), nullptr, n, H, XN(l), 2 * H, 0, st);
+    if (e == hipSuccess) e = pack_cols(M->agg, nullptr, n, H, XN(l), 2 * H, H, st);
+    a_n.R = n; a_n.x = XN(l);
+    if (e == hipSuccess) e = mlp_fwd(M, M->mlp[3 + 2 * l], a_n, Hs(l));
+    if (e == hipSuccess) e = hipMemcpyAsync(Hs(l + 1), a_n.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
   }
   Act& ad = M->act.back();
   ad.R = n;
-  if (e == hipSuccess) e = pack_cols(M->h[L], nullptr, n, H, M->db16, H, 0, st);
+  if (e == hipSuccess) e = pack_cols(Hs(L), nullptr, n, H, M->db16, H, 0, st);
   ad.x = M->db16;
   if (e == hipSuccess) e = mlp_fwd(M, M->mlp.back(), ad, nullptr);
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "forward");
@@ -292,6 +317,7 @@ int backward(sdem_gnn* M) {
   const int H = M->cfg.hidden, L = L_of(M), n = M->n;
   const int ne = (int)M->ne;
   hipStream_t st = M->stream;
+  if (!M->train_alloc) return M->fail(SDEM_E_STATE, "the graph was set up for inference only");
   hipError_t e = hipMemsetAsync(M->g, 0, M->nparam * 4, st);
   // the decoder's input (h_L in bf16) was packed into scratch that the
   // backward pass reuses: pack it again where nothing else writes before
@@ -324,7 +350,7 @@ int backward(sdem_gnn* M) {
 }
 
 // neighbour graph of n particles on the device (positions double4)
-int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G) {
+int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G, bool train) {
   hipStream_t st = M->stream;
   const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
   auto& A = M->allocs;
@@ -366,7 +392,7 @@ int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G) {
   if (e == hipSuccess) e = hipMemcpyAsync(&total, M->deg + n, 4, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipf(M, e, "graph search");
-  if (int rc = alloc_graph(M, n, total)) return rc;
+  if (int rc = alloc_graph(M, n, total, train)) return rc;
   M->n = n;
   M->ne = total;
   e = hipMemcpyAsync(M->rstart, M->deg, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, st);
@@ -375,9 +401,10 @@ int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G) {
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "graph edges");
 }
 
-GGrid make_ggrid(const double lo[3], const double hi[3], const int per[3], double rc) {
+GGrid make_ggrid(const double lo[3], const double hi[3], const int per[3], double rc, double cfac) {
   GGrid G;
   G.rc = rc;
+  G.cfac = cfac;
   for (int d = 0; d < 3; d++) {
     G.lo[d] = lo[d];
     G.L[d] = hi[d] - lo[d];
@@ -412,7 +439,8 @@ int sdem_gnn_config_default(sdem_gnn_config* c) {
   c->vel_scale = 0.1;
   c->acc_scale = 10.0;
   c->len_scale = 2.0e-3;
-  c->radius = 2.4e-3;
+  c->radius = 4.8e-3;
+  c->cutoff_factor = 1.2;
   c->dx_scale = 1.0e-6;
   c->dv_scale = 1.0e-2;
   c->nsub = 10;
@@ -483,7 +511,8 @@ int sdem_gnn_get_config(sdem_gnn* M, sdem_gnn_config* c) {
 int sdem_gnn_set_scales(sdem_gnn* M, const sdem_gnn_config* c) {
   if (!M || !c) return SDEM_E_ARG;
   M->cfg.vel_scale = c->vel_scale; M->cfg.acc_scale = c->acc_scale; M->cfg.len_scale = c->len_scale;
-  M->cfg.radius = c->radius; M->cfg.floor_z = c->floor_z; M->cfg.dx_scale = c->dx_scale;
+  M->cfg.radius = c->radius; M->cfg.cutoff_factor = c->cutoff_factor;
+  M->cfg.floor_z = c->floor_z; M->cfg.dx_scale = c->dx_scale;
   M->cfg.dv_scale = c->dv_scale; M->cfg.nsub = c->nsub;
   return SDEM_OK;
 }
@@ -537,7 +566,7 @@ int sdem_gnn_set_graph(sdem_gnn* M, int32_t n, const float* node_feat, int64_t n
     if (dst[k] < dst[k - 1]) return M->fail(SDEM_E_ARG, "edges must be sorted by receiver");
   for (int64_t k = 0; k < ne; k++)
     if (src[k] < 0 || src[k] >= n || dst[k] < 0 || dst[k] >= n) return M->fail(SDEM_E_ARG, "edge index out of range");
-  if (int rc = alloc_graph(M, n, ne)) return rc;
+  if (int rc = alloc_graph(M, n, ne, true)) return rc;
   M->n = n;
   M->ne = ne;
   std::vector<int> rs(n + 1, 0);
@@ -621,10 +650,10 @@ int sdem_gnn_train_step(sdem_gnn* M, const float* target, float lr, float clip, 
 }
 
 // graph + features from a DEM context's state (first box, live slots)
-static int graph_from_dem(sdem_gnn* M, sdem_ctx* dem, sdem_dem_view* V) {
+static int graph_from_dem(sdem_gnn* M, sdem_ctx* dem, sdem_dem_view* V, bool train) {
   if (int rc = sdem_internal_view(dem, V)) return M->fail(rc, "DEM context view: " + std::string(sdem_last_error(dem)));
-  const GGrid G = make_ggrid(V->lo, V->hi, V->per, M->cfg.radius);
-  if (int rc = build_graph(M, V->pos, V->n, G)) return rc;
+  const GGrid G = make_ggrid(V->lo, V->hi, V->per, M->cfg.radius, M->cfg.cutoff_factor);
+  if (int rc = build_graph(M, V->pos, V->n, G, train)) return rc;
   const FeatScale F = scales(M->cfg);
   hipError_t e = node_feat(V->pos, V->lv, V->la, V->n, V->cap, F, M->fn, PAD_IN, M->stream);
   if (e == hipSuccess && M->ne > 0)
@@ -632,13 +661,33 @@ static int graph_from_dem(sdem_gnn* M, sdem_ctx* dem, sdem_dem_view* V) {
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "features");
 }
 
-int sdem_gnn_graph_from_dem(sdem_gnn* M, sdem_ctx* dem, int64_t* nedges) {
+int sdem_gnn_graph_from_dem(sdem_gnn* M, sdem_ctx* dem, int32_t for_training, int64_t* nedges) {
   if (!M || !dem) return SDEM_E_ARG;
   sdem_dem_view V;
-  if (int rc = graph_from_dem(M, dem, &V)) return rc;
+  if (int rc = graph_from_dem(M, dem, &V, for_training != 0)) return rc;
+  if (M->ids_cap < V.n) {
+    M->ids.resize(V.n);
+    M->ids_cap = V.n;
+  }
+  hipError_t e0 = hipMemcpy(M->ids.data(), V.id, (size_t)V.n * 4, hipMemcpyDeviceToHost);
+  if (e0 != hipSuccess) return hipf(M, e0, "node ids");
   hipError_t e = hipStreamSynchronize(M->stream);
   if (e != hipSuccess) return hipf(M, e, "graph");
   if (nedges) *nedges = M->ne;
+  return SDEM_OK;
+}
+
+int sdem_gnn_get_num_nodes(sdem_gnn* M, int32_t* n, int64_t* ne) {
+  if (!M) return SDEM_E_ARG;
+  if (n) *n = M->n;
+  if (ne) *ne = M->ne;
+  return SDEM_OK;
+}
+
+int sdem_gnn_get_node_ids(sdem_gnn* M, int32_t cap, int32_t* ids) {
+  if (!M || !ids) return SDEM_E_ARG;
+  if (cap < M->n || (int)M->ids.size() < M->n) return M->fail(SDEM_E_CAPACITY, "no ids or arrays too small");
+  std::copy(M->ids.begin(), M->ids.begin() + M->n, ids);
   return SDEM_OK;
 }
 
@@ -670,9 +719,9 @@ int sdem_gnn_advance(sdem_gnn* M, sdem_ctx* dem, int32_t calls) {
   if (!M || !dem || calls < 0) return SDEM_E_ARG;
   for (int q = 0; q < calls; q++) {
     sdem_dem_view V;
-    if (int rc = graph_from_dem(M, dem, &V)) return rc;
+    if (int rc = graph_from_dem(M, dem, &V, false)) return rc;
     if (int rc = forward(M)) return rc;
-    const GGrid G = make_ggrid(V.lo, V.hi, V.per, M->cfg.radius);
+    const GGrid G = make_ggrid(V.lo, V.hi, V.per, M->cfg.radius, M->cfg.cutoff_factor);
     const FeatScale F = scales(M->cfg);
     const double T = V.dt * M->cfg.nsub;
     hipError_t e = advance(V.pos, V.lv, V.n, V.cap, M->act.back().out, PAD_IN, F, G, T, V.g, M->stream);

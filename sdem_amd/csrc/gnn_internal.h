@@ -25,7 +25,8 @@ struct Epi {
 struct GGrid {
   double lo[3], L[3], inv_w[3];
   int n[3], per[3];
-  double rc;
+  double rc;      // largest cut-off (cells are at least this wide)
+  double cfac;    // pair cut-off = cfac * 2 * max(r_i, r_j)
 };
 
 // feature and output scales: velocity, acceleration, length, the floor's

@@ -274,9 +274,10 @@ __global__ void k_gather_add(const float* __restrict__ src, const int* __restric
 }
 
 // ---------------------------------------------------------------------------
-// neighbour graph: cell keys, cell ranges, degree count, edge fill.  Cells
-// of edge >= the cut-off radius; periodic dimensions wrap with the nearest
-// image.  Edges are grouped by receiver; a receiver's senders come in cell
+// neighbour graph: cell keys, cell ranges, degree count, edge fill.  A pair
+// is an edge when its distance is at most cfac * 2 * max(r_i, r_j); cells
+// are at least the largest such cut-off wide; periodic dimensions wrap
+// with the nearest image.  Edges are grouped by receiver; a receiver's senders come in cell
 // traversal order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int gcell(const GGrid& G, int d, double x) {
@@ -311,7 +312,6 @@ __global__ void k_edges(const double4* __restrict__ pos, int n, GGrid G, const i
   const double xv[3] = {p.x, p.y, p.z};
   int c[3];
   for (int d = 0; d < 3; d++) c[d] = gcell(G, d, xv[d]);
-  const double rc2 = G.rc * G.rc;
   int k = 0;
   const int base = FILL ? rstart[i] : 0;
   for (int dz = -1; dz <= 1; dz++) {
@@ -336,7 +336,8 @@ __global__ void k_edges(const double4* __restrict__ pos, int n, GGrid G, const i
               else if (d3[d] < -0.5 * G.L[d]) d3[d] += G.L[d];
             }
           const double r2 = d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2];
-          if (r2 > rc2) continue;
+          const double cut = G.cfac * 2.0 * (p.w > pj.w ? p.w : pj.w);
+          if (r2 > cut * cut) continue;
           if (FILL) {
             esrc[base + k] = j;
             edst[base + k] = i;

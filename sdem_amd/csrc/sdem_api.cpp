@@ -1578,6 +1578,7 @@ struct sdem_dem_view {
   double4* pos;
   double* lv;
   double* la;
+  int* id;
   int cap, n;
   double dt, g[3], lo[3], hi[3];
   int per[3];
@@ -1597,6 +1598,7 @@ extern "C" int sdem_internal_view(sdem_ctx* c, sdem_dem_view* v) {
   v->pos = A.pos;
   v->lv = A.lv;
   v->la = A.la;
+  v->id = A.id;
   v->cap = (int)c->nslots;
   v->n = B.n_alive;
   v->dt = b.o.dt;

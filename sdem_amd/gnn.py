@@ -17,7 +17,8 @@ _UP = C.POINTER(C.c_uint16)
 
 class Config(C.Structure):
     _fields_ = [("vel_scale", C.c_double), ("acc_scale", C.c_double), ("len_scale", C.c_double),
-                ("radius", C.c_double), ("floor_z", C.c_double), ("dx_scale", C.c_double),
+                ("radius", C.c_double), ("cutoff_factor", C.c_double), ("floor_z", C.c_double),
+                ("dx_scale", C.c_double),
                 ("dv_scale", C.c_double), ("hidden", C.c_int32), ("layers", C.c_int32),
                 ("node_in", C.c_int32), ("edge_in", C.c_int32), ("out", C.c_int32), ("nsub", C.c_int32)]
 
@@ -37,7 +38,9 @@ def _declare(lib) -> None:
         "get_grads": ([V, _FP], C.c_int),
         "get_layout": ([V, C.c_int32, _IP, _LP, _IP], C.c_int),
         "set_graph": ([V, C.c_int32, _FP, C.c_int64, _IP, _IP, _FP], C.c_int),
-        "graph_from_dem": ([V, V, _LP], C.c_int),
+        "graph_from_dem": ([V, V, C.c_int32, _LP], C.c_int),
+        "get_node_ids": ([V, C.c_int32, _IP], C.c_int),
+        "get_num_nodes": ([V, _IP, _LP], C.c_int),
         "get_graph": ([V, C.c_int64, _IP, _IP, _FP, _FP], C.c_int),
         "forward": ([V, _FP], C.c_int),
         "backward": ([V, _FP], C.c_int),
@@ -142,12 +145,22 @@ class Model:
                                                 dst.ctypes.data_as(_IP), _fp(fe)), "set_graph")
         self.n = fn.shape[0]
 
-    def graph_from_dem(self, eng) -> int:
+    def graph_from_dem(self, eng, for_training: bool = True) -> int:
+        """Neighbour graph and features of the DEM context's state; nodes in
+        storage order (node_ids() gives their particle ids)."""
         ne = C.c_int64(0)
-        self._check(self.lib.sdem_gnn_graph_from_dem(self.h, eng.h, C.byref(ne)), "graph_from_dem")
-        self.n = eng.num_particles()
+        self._check(self.lib.sdem_gnn_graph_from_dem(self.h, eng.h, int(for_training), C.byref(ne)),
+                    "graph_from_dem")
+        n = C.c_int32(0)
+        self._check(self.lib.sdem_gnn_get_num_nodes(self.h, C.byref(n), None), "get_num_nodes")
+        self.n = n.value
         self.ne = ne.value
         return ne.value
+
+    def node_ids(self) -> np.ndarray:
+        ids = np.zeros(self.n, np.int32)
+        self._check(self.lib.sdem_gnn_get_node_ids(self.h, self.n, ids.ctypes.data_as(_IP)), "get_node_ids")
+        return ids
 
     def get_graph(self):
         ne = self.ne
