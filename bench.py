@@ -51,9 +51,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick ~15 s of CPU work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("settling", "ensemble"), default="settling",
+    ap.add_argument("--workload", choices=("settling", "ensemble", "surrogate"), default="settling",
                     help="settling: the 1M-sphere bed (the headline line); ensemble: 256 independent "
-                         "sand boxes stepped together")
+                         "sand boxes stepped together; surrogate: the learned surrogate trained on "
+                         "DEM trajectories, then against the DEM on the 1M-sphere bed")
+    ap.add_argument("--train-snapshots", type=int, default=240)
     ap.add_argument("--boxes", type=int, default=256)
     return ap.parse_args()
 
@@ -133,6 +135,53 @@ def ensemble_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def surrogate_bench(args):
+    """Learned surrogate: trained online on three 50k-sphere settling boxes
+    (same physics as the 1M bed).  Speed: 20 surrogate calls (200 DEM
+    steps) on the 1M bed against 200 DEM steps.  Error: a 200k-sphere bed
+    of the same physics over args.steps DEM steps, by the DEM, by the
+    surrogate (with 10 DEM steps after every 500 calls) and by the
+    ballistic step without the network."""
+    from sdem_amd import native, surrogate
+
+    lib = native.load_hip_library()
+    model = surrogate.make_model(hidden=128, layers=8, seed=1, lib=lib)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    losses = surrogate.train(model, n=50_000, seeds=(101, 102, 103), warm=200, snapshots=args.train_snapshots,
+                             log=log, lib=lib)
+    t_train = time.perf_counter() - t0
+    speed = surrogate.rollout(model, n=args.n, seed=args.seed, warm=args.warmup, steps=200, log=log, lib=lib)
+    err = surrogate.rollout(model, n=200_000, seed=args.seed, warm=args.warmup, steps=args.steps, log=log, lib=lib)
+    H, L = 128, 8
+    calls = speed["steps"] // speed["nsub"]
+    ne = speed["edges"]
+    # matrix-core work per call: 2 flop per multiply-add of every GEMM
+    flop_edge = 2 * (3 * H * H + H * H + H * H) * L + 2 * (16 * H + 2 * H * H)
+    flop_node = 2 * (2 * H * H + 2 * H * H) * L + 2 * (16 * H + 2 * H * H) + 2 * (H * H * 2 + H * 16)
+    tflops = (flop_edge * ne + flop_node * args.n) / (speed["surrogate_s"] / calls) / 1e12
+    out = {
+        "metric": "surrogate speed-up over the plain GPU DEM at equal simulated time",
+        "value": speed["speedup"], "unit": "x", "n_gpus": 1, "steps": 200, "warmup": args.warmup,
+        "ms_per_step": speed["surrogate_s"] / 200 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 MFMA, fp32 accumulation, fp64 positions",
+        "data": "synthetic (seeded random loose packs, phi=0.3)",
+        "config": {"workload": f"settling bed, {args.n} spheres; graph network 8 layers x 2x128 SiLU MLPs, "
+                               f"10 DEM steps per call, trained on {args.train_snapshots} snapshots of "
+                               "3 x 50k-sphere boxes", "train_s": t_train,
+                   "train_loss_ratio_last": float(np.mean(losses[-20:]))},
+        "timing_s_1M_200_steps": {"dem": speed["dem_s"], "surrogate": speed["surrogate_s"],
+                                  "ballistic": speed["ballistic_s"]},
+        "errors_vs_dem_200k": {k: v for k, v in err.items()
+                               if "err" in k or "rms" in k or "lost" in k or k.endswith("_dem")},
+        "error_window_steps": args.steps,
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "traffic": None, "edges": ne},
+    }
+    model.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -141,6 +190,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         ensemble_bench(args)
+        return
+    if args.workload == "surrogate":
+        surrogate_bench(args)
         return
     from sdem_amd import native, scenarios
 

@@ -150,15 +150,17 @@ hipError_t mm(sdem_gnn* M, bool ta, bool tb, const __bf16* A, const __bf16* B, i
   return gemm(ta, tb, A, B, m, n, k, lda, ldb, splits, ep, M->stream);
 }
 
-// forward of one MLP on act.x (R x in); out (+)= resid if given
-hipError_t mlp_fwd(sdem_gnn* M, const MLP& p, Act& a, const float* resid) {
+// forward of one MLP on act.x (R x in) into y (R x out); y (+)= resid if
+// given (y may be resid itself: every element is read before it is written,
+// by the same thread)
+hipError_t mlp_fwd(sdem_gnn* M, const MLP& p, Act& a, const float* resid, float* y) {
   const __bf16* W = M->wb;
   const float* Wf = M->w;
   Epi e1{EPI_BIAS | EPI_PRE | EPI_SILU, Wf + p.b1, a.pre1, nullptr, nullptr, a.a1};
   hipError_t e = mm(M, false, false, a.x, W + p.w1, a.R, p.hid, p.in, p.in, p.hid, e1);
   Epi e2{EPI_BIAS | EPI_PRE | EPI_SILU, Wf + p.b2, a.pre2, nullptr, nullptr, a.a2};
   if (e == hipSuccess) e = mm(M, false, false, a.a1, W + p.w2, a.R, p.hid, p.hid, p.hid, p.hid, e2);
-  Epi e3{EPI_BIAS | (resid ? EPI_RESID : 0), Wf + p.b3, nullptr, resid, a.out, nullptr};
+  Epi e3{EPI_BIAS | (resid ? EPI_RESID : 0), Wf + p.b3, nullptr, resid, y, nullptr};
   if (e == hipSuccess) e = mm(M, false, false, a.a2, W + p.w3, a.R, p.out, p.hid, p.hid, p.out, e3);
   return e;
 }
@@ -215,7 +217,8 @@ int alloc_graph(sdem_gnn* M, int n, int64_t ne, bool train) {
   M->gcap_n = 0;
   M->gcap_e = 0;
   const int H = M->cfg.hidden, L = L_of(M);
-  const size_t N = (size_t)std::max(n, 1), E = (size_t)std::max<int64_t>(ne, 1);
+  // room to grow: graphs of a settling bed gain edges call after call
+  const size_t N = (size_t)std::max(n, 1), E = (size_t)std::max<int64_t>(ne + ne / 4, 1);
   auto& A = M->graph_allocs;
   hipError_t e = hipSuccess;
   auto G = [&](auto** p, size_t k) { if (e == hipSuccess) e = galloc(A, p, k); };
@@ -278,10 +281,9 @@ int forward(sdem_gnn* M) {
   an.R = n; an.x = M->fn;
   Act& ae = M->act[1];
   ae.R = ne; ae.x = M->fe;
-  hipError_t e = mlp_fwd(M, M->mlp[0], an, nullptr);
-  if (e == hipSuccess) e = hipMemcpyAsync(Hs(0), an.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
-  if (e == hipSuccess && ne > 0) e = mlp_fwd(M, M->mlp[1], ae, nullptr);
-  if (e == hipSuccess && ne > 0) e = hipMemcpyAsync(Es(0), ae.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
+  // encoder and layer outputs go straight into the residual streams
+  hipError_t e = mlp_fwd(M, M->mlp[0], an, nullptr, Hs(0));
+  if (e == hipSuccess && ne > 0) e = mlp_fwd(M, M->mlp[1], ae, nullptr, Es(0));
   for (int l = 0; l < L && e == hipSuccess; l++) {
     Act& a_e = M->act[2 + 2 * l];
     Act& a_n = M->act[3 + 2 * l];
@@ -291,8 +293,7 @@ int forward(sdem_gnn* M) {
       if (e == hipSuccess) e = pack_cols(Hs(l), M->esrc, ne, H, XE(l), 3 * H, H, st);
       if (e == hipSuccess) e = pack_cols(Hs(l), M->edst, ne, H, XE(l), 3 * H, 2 * H, st);
       a_e.R = ne; a_e.x = XE(l);
-      if (e == hipSuccess) e = mlp_fwd(M, M->mlp[2 + 2 * l], a_e, Es(l));
-      if (e == hipSuccess) e = hipMemcpyAsync(Es(l + 1), a_e.out, (size_t)ne * H * 4, hipMemcpyDeviceToDevice, st);
+      if (e == hipSuccess) e = mlp_fwd(M, M->mlp[2 + 2 * l], a_e, Es(l), Es(l + 1));
       if (e == hipSuccess) e = segsum(Es(l + 1), M->rstart, n, H, M->agg, st);
     } else {
       e = hipMemsetAsync(M->agg, 0, (size_t)n * H * 4, st);
@@ -301,14 +302,13 @@ int forward(sdem_gnn* M) {
     if (e == hipSuccess) e = pack_cols(Hs(l), nullptr, n, H, XN(l), 2 * H, 0, st);
     if (e == hipSuccess) e = pack_cols(M->agg, nullptr, n, H, XN(l), 2 * H, H, st);
     a_n.R = n; a_n.x = XN(l);
-    if (e == hipSuccess) e = mlp_fwd(M, M->mlp[3 + 2 * l], a_n, Hs(l));
-    if (e == hipSuccess) e = hipMemcpyAsync(Hs(l + 1), a_n.out, (size_t)n * H * 4, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = mlp_fwd(M, M->mlp[3 + 2 * l], a_n, Hs(l), Hs(l + 1));
   }
   Act& ad = M->act.back();
   ad.R = n;
   if (e == hipSuccess) e = pack_cols(Hs(L), nullptr, n, H, M->db16, H, 0, st);
   ad.x = M->db16;
-  if (e == hipSuccess) e = mlp_fwd(M, M->mlp.back(), ad, nullptr);
+  if (e == hipSuccess) e = mlp_fwd(M, M->mlp.back(), ad, nullptr, ad.out);
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "forward");
 }
 
@@ -724,7 +724,7 @@ int sdem_gnn_advance(sdem_gnn* M, sdem_ctx* dem, int32_t calls) {
     const GGrid G = make_ggrid(V.lo, V.hi, V.per, M->cfg.radius, M->cfg.cutoff_factor);
     const FeatScale F = scales(M->cfg);
     const double T = V.dt * M->cfg.nsub;
-    hipError_t e = advance(V.pos, V.lv, V.n, V.cap, M->act.back().out, PAD_IN, F, G, T, V.g, M->stream);
+    hipError_t e = advance(V.pos, V.lv, V.la, V.n, V.cap, M->act.back().out, PAD_IN, F, G, T, V.g, M->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(M->stream);
     if (e != hipSuccess) return hipf(M, e, "advance");
     if (int rc = sdem_internal_after_update(dem, M->cfg.nsub)) return M->fail(rc, sdem_last_error(dem));

@@ -44,8 +44,8 @@ hipError_t node_feat(const double4* pos, const double* vel, const double* acc, i
                      const FeatScale& F, __bf16* out, int ld, hipStream_t st);
 hipError_t edge_feat(const double4* pos, const double* vel, int stride, const int* esrc, const int* edst, int ne,
                      const GGrid& G, const FeatScale& F, __bf16* out, int ld, hipStream_t st);
-hipError_t advance(double4* pos, double* vel, int n, int stride, const float* pred, int ld, const FeatScale& F,
-                   const GGrid& G, double T, const double g[3], hipStream_t st);
+hipError_t advance(double4* pos, double* vel, double* acc, int n, int stride, const float* pred, int ld,
+                   const FeatScale& F, const GGrid& G, double T, const double g[3], hipStream_t st);
 hipError_t mfma_probe(const void* a, const void* b, float* d, hipStream_t st);
 hipError_t gemm(bool ta, bool tb, const __bf16* A, const __bf16* B, int M, int N, int K, int lda, int ldb,
                 int splits, const Epi& ep, hipStream_t st);

@@ -174,14 +174,21 @@ __global__ void k_to_bf16(const float* __restrict__ x, size_t n, __bf16* __restr
 }
 
 // copy rows of a [R x C] fp32 matrix into columns [c0, c0 + C) of a
-// [R x ldo] bf16 matrix (MLP inputs); optionally through an index
+// [R x ldo] bf16 matrix (MLP inputs), optionally through a row index; each
+// thread moves 4 consecutive columns (C, ldo, c0 multiples of 4)
 __global__ void k_pack_cols(const float* __restrict__ x, const int* __restrict__ idx, int R, int C,
                             __bf16* __restrict__ out, int ldo, int c0) {
+  const int q = C >> 2;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)R * C) return;
-  const int r = (int)(i / C), c = (int)(i % C);
+  if (i >= (size_t)R * q) return;
+  const int r = (int)(i / q), c = (int)(i - (size_t)r * q) * 4;
   const int src = idx ? idx[r] : r;
-  out[(size_t)r * ldo + c0 + c] = (__bf16)x[(size_t)src * C + c];
+  const float4 v = *(const float4*)(x + (size_t)src * C + c);
+  __bf16* o = out + (size_t)r * ldo + c0 + c;
+  o[0] = (__bf16)v.x;
+  o[1] = (__bf16)v.y;
+  o[2] = (__bf16)v.z;
+  o[3] = (__bf16)v.w;
 }
 
 // agg[v] = sum of rows e[k] over the edges k of receiver v (edges sorted by
@@ -391,10 +398,12 @@ __global__ void k_edge_feat(const double4* __restrict__ pos, const double* __res
 
 // apply a prediction over T = nsub * dt: ballistic step plus the network's
 // corrections (columns 0-2: position, 3-5: velocity, in units of the
-// output scales); periodic wrap, and the floor keeps its particles
-__global__ void k_advance(double4* __restrict__ pos, double* __restrict__ vel, int n, int stride,
-                          const float* __restrict__ pred, int ld, FeatScale F, GGrid G, double T, double gx,
-                          double gy, double gz) {
+// output scales); periodic wrap, and the floor keeps its particles.  The
+// stored acceleration (a node feature of the next call) becomes the mean
+// acceleration of the call.
+__global__ void k_advance(double4* __restrict__ pos, double* __restrict__ vel, double* __restrict__ acc, int n,
+                          int stride, const float* __restrict__ pred, int ld, FeatScale F, GGrid G, double T,
+                          double gx, double gy, double gz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double4 p = pos[i];
@@ -410,6 +419,7 @@ __global__ void k_advance(double4* __restrict__ pos, double* __restrict__ vel, i
     }
     x[d] = xn;
     vel[(size_t)d * stride + i] = vn;
+    if (acc) acc[(size_t)d * stride + i] = (vn - v) / T;
   }
   const double zmin = F.floor_z + 0.5 * p.w;
   if (x[2] < zmin) x[2] = zmin;
@@ -467,7 +477,8 @@ hipError_t to_bf16(const float* x, size_t n, __bf16* y, hipStream_t st) {
 }
 
 hipError_t pack_cols(const float* x, const int* idx, int R, int C, __bf16* out, int ldo, int c0, hipStream_t st) {
-  hipLaunchKernelGGL(k_pack_cols, dim3(nb((size_t)R * C, 256)), dim3(256), 0, st, x, idx, R, C, out, ldo, c0);
+  if ((C & 3) || (ldo & 3) || (c0 & 3)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pack_cols, dim3(nb((size_t)R * (C / 4), 256)), dim3(256), 0, st, x, idx, R, C, out, ldo, c0);
   return hipGetLastError();
 }
 
@@ -550,10 +561,10 @@ hipError_t edge_feat(const double4* pos, const double* vel, int stride, const in
   return hipGetLastError();
 }
 
-hipError_t advance(double4* pos, double* vel, int n, int stride, const float* pred, int ld, const FeatScale& F,
-                   const GGrid& G, double T, const double g[3], hipStream_t st) {
-  hipLaunchKernelGGL(k_advance, dim3(nb(n, 256)), dim3(256), 0, st, pos, vel, n, stride, pred, ld, F, G, T, g[0],
-                     g[1], g[2]);
+hipError_t advance(double4* pos, double* vel, double* acc, int n, int stride, const float* pred, int ld,
+                   const FeatScale& F, const GGrid& G, double T, const double g[3], hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(nb(n, 256)), dim3(256), 0, st, pos, vel, acc, n, stride, pred, ld, F, G, T,
+                     g[0], g[1], g[2]);
   return hipGetLastError();
 }
 

@@ -188,7 +188,11 @@ class Model:
         return loss.value
 
     def advance(self, eng, calls: int = 1) -> None:
+        """`calls` surrogate calls on a DEM context (each nsub steps)."""
         self._check(self.lib.sdem_gnn_advance(self.h, eng.h, int(calls)), "advance")
+        n, ne = C.c_int32(0), C.c_int64(0)
+        self._check(self.lib.sdem_gnn_get_num_nodes(self.h, C.byref(n), C.byref(ne)), "get_num_nodes")
+        self.n, self.ne = n.value, ne.value
 
 
 def mfma_probe(a_bits: np.ndarray, b_bits: np.ndarray, lib=None) -> np.ndarray:

@@ -143,6 +143,7 @@ def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: in
         return st, el, extra
 
     s_sur, out["surrogate_s"], extra = run(model)
+    out["edges"] = int(model.ne) if hasattr(model, "ne") else 0
     zero = gnn.Model(model.cfg, seed=1, lib=lib)
     zero.set_params(np.zeros(zero.num_params(), np.float32))
     s_bal, out["ballistic_s"], _ = run(zero)
@@ -156,12 +157,15 @@ def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: in
     per = np.array(o["is_periodic"], bool)
 
     def rms(s):
-        d = s["posr"][:, :3] - s_ref["posr"][:, :3]
+        # particles present in both (a particle thrown out of the domain is deleted)
+        ids, ia, ib = np.intersect1d(s["id"], s_ref["id"], return_indices=True)
+        d = s["posr"][ia, :3] - s_ref["posr"][ib, :3]
         d[:, per] -= L[per] * np.round(d[:, per] / L[per])
-        low = s_ref["posr"][:, 2] < 0.02
-        return float(np.sqrt((d ** 2).sum(1).mean())), float(np.sqrt((d[low] ** 2).sum(1).mean()))
-    out["rms_surrogate"], out["rms_bottom_surrogate"] = rms(s_sur)
-    out["rms_ballistic"], out["rms_bottom_ballistic"] = rms(s_bal)
+        low = s_ref["posr"][ib, 2] < 0.02
+        return (float(np.sqrt((d ** 2).sum(1).mean())), float(np.sqrt((d[low] ** 2).sum(1).mean())),
+                int(len(s_ref["id"]) - len(ids)))
+    out["rms_surrogate"], out["rms_bottom_surrogate"], out["lost_surrogate"] = rms(s_sur)
+    out["rms_ballistic"], out["rms_bottom_ballistic"], out["lost_ballistic"] = rms(s_bal)
     out["speedup"] = out["dem_s"] / out["surrogate_s"]
     out.update(n=n, steps=steps, nsub=nsub, extra_dem_steps=extra)
     if log:
