@@ -39,6 +39,11 @@ BYTES_PER_CONTACT = 80
 # integrated, but only sdem_get_forces reads them)
 CELL_FACTOR = (0.5, 0.5, 0.5)
 SKIN = 0.05
+# HBM bytes per k_force launch on the default 1M bed, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by what the same
+# counters read for a plain 512 MiB copy (x2.00 and x1.00):
+# 112,487 KiB x 2 read + 218,994 KiB written (profiles/k_force_settling_1M.md)
+KFORCE_TRAFFIC_1M = (112_487 * 2 + 218_994) * 1024
 
 
 def parse():
@@ -251,7 +256,8 @@ def main():
                    "n": args.n, "seed": args.seed, "contact_records_per_step": P,
                    "particles_at_end": n_now},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": KFORCE_TRAFFIC_1M if (args.n == 1_000_000 and args.seed == 20261203) else None,
                      "kernel": "k_force", "kernel_ms_avg": k_avg_s * 1e3,
                      "algorithmic_bytes_per_launch": kernel_bytes},
         "step_breakdown_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},

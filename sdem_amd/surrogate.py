@@ -171,3 +171,39 @@ def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: in
     if log:
         log(f"# rollout: {out}")
     return out
+
+
+def time_calls(n: int = 1_000_000, calls: int = 10, seed: int = 20261203, warm: int = 100, lib=None) -> Dict:
+    """Wall time of surrogate calls on a settling bed of n spheres (an
+    untrained network: the cost does not depend on the weights).  Returns
+    ms per call, the edges of the last graph and ms per DEM step of the
+    same bed for comparison."""
+    lib = lib or native.load_hip_library()
+    model = make_model(lib=lib)
+    sc = scenarios.settling(n=n, seed=seed)
+    eng = native.build_engine(lib, sc, tuning=dict(cell_factor=(0.5, 0.5, 0.5), skin=0.05, keep_forces=0))
+    eng.iterate(warm)
+    t = time.perf_counter()
+    eng.iterate(10 * model.cfg.nsub)
+    dem_ms = (time.perf_counter() - t) / (10 * model.cfg.nsub) * 1e3
+    model.advance(eng, 2)  # buffers sized, kernels loaded
+    t = time.perf_counter()
+    model.advance(eng, calls)
+    call_ms = (time.perf_counter() - t) / calls * 1e3
+    out = dict(n=n, calls=calls, ms_per_call=call_ms, edges=model.ne, dem_ms_per_step=dem_ms,
+               dem_ms_per_call_equivalent=dem_ms * model.cfg.nsub)
+    eng.close()
+    model.close()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="time surrogate calls on a settling bed")
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--calls", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=20261203)
+    a = ap.parse_args()
+    print(json.dumps(time_calls(n=a.n, calls=a.calls, seed=a.seed)), flush=True)
