@@ -241,7 +241,7 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
  * sdem_set_state. */
 int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots);
 
-/* Block sizes of the contact and force kernels (64, 128 or 256; no
+/* Block sizes of the contact-test and force kernels (64, 128 or 256; no
  * counterpart in the reference, results do not depend on them). */
 int sdem_set_block_sizes(sdem_ctx* c, int32_t contact_block, int32_t force_block);
 int sdem_get_tuning(sdem_ctx* c, double cell_factor[3], double* skin_factor, int32_t* contact_block,

@@ -243,7 +243,8 @@ struct sdem_ctx {
   int *keys = nullptr, *rank = nullptr, *counts = nullptr, *start = nullptr, *tile_sum = nullptr;
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
-  int *vcnt = nullptr, *vlist = nullptr;
+  int* pid = nullptr;
+  int *vcnt = nullptr, *vlist = nullptr, *vkey = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
   int64_t nkeys = 0;
@@ -477,8 +478,9 @@ static int upload_state(sdem_ctx* c) {
   hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
   auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
   M(&c->BS, nbox); M(&c->slots, NSLOT); M(&c->bslots, (size_t)nbox * NSLOT);
-  M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap);
-  M(&c->vcnt, cap); M(&c->ccnt, cap); M(&c->clist, cap * c->hcap); M(&c->vlist, cap * c->lcap);
+  M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
+  M(&c->vcnt, cap); M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
+  M(&c->ccnt, cap); M(&c->clist, cap * c->hcap);
   M(&c->d_blk_box, cap / BOX_ALIGN);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
 
@@ -727,8 +729,8 @@ static int upload_params(sdem_ctx* c) {
   D.PB = c->d_params; D.blk_box = c->d_blk_box; D.nbox = nbox; D.nslots = c->nslots; D.nkeys = key;
   D.S = c->S; D.BS = c->BS; D.slots = c->slots; D.bslots = c->bslots;
   D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
-  D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.vcnt = c->vcnt; D.vlist = c->vlist;
-  D.clist = c->clist; D.ccnt = c->ccnt;
+  D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid; D.vcnt = c->vcnt;
+  D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
   D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
   c->params_ready = true;
@@ -1065,6 +1067,7 @@ static int grow_vlist(sdem_ctx* c, int list_need) {
   int neu = std::max(list_need + list_need / 4 + 8, 2 * c->lcap);
   if ((int64_t)neu * c->nslots > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
   hipError_t e = regrow(c, &c->vlist, (size_t)c->nslots * neu);
+  if (e == hipSuccess) e = regrow(c, &c->vkey, (size_t)c->nslots * neu);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
   c->lcap = neu;
   c->params_ready = false;

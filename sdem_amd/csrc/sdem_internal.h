@@ -166,8 +166,9 @@ struct Dev {
   Slot* slots;
   BoxSlot* bslots;       // NSLOT per box
   int *keys, *rank, *counts, *start, *tile_sum, *perm_tmp, *perm;
-  double4* pk;
-  int *vcnt, *vlist;
+  double4* pk;           // positions in cell order (rebuild steps)
+  int* pid;              // ids in cell order (rebuild steps)
+  int *vcnt, *vlist, *vkey;
   unsigned* clist;
   int* ccnt;
   const double4* fpk;

@@ -16,11 +16,13 @@
 //   k_bin                   : (level, cell) key per particle, per-key counts
 //   k_scan_tiles/_sums/_add : exclusive scan of the counts
 //   k_scatter / k_rank      : deterministic counting sort (in-cell order =
-//                             storage order) + packed positions in cell order
+//                             storage order) + packed positions, ids in
+//                             cell order
 //   k_vlist                 : candidate lists, pairs within ri+rj+skin
+//                             (fixed particles included), sorted by
+//                             partner id
 //   -- every step --
-//   k_contact               : exact contact test over the candidate lists,
-//                             partners sorted by id
+//   k_contact               : exact contact test over the candidate lists
 //   k_force                 : history match, contact forces, accelerations,
 //                             integration, wrap/delete, the largest move of
 //                             the step, and the next step's near-wall masks
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __
 }
 
 // Make the in-cell order the storage order (deterministic) and pack the
-// positions in cell order.  A box's live particles are the slots before its
+// positions and ids in cell order.  A box's live particles are the slots before its
 // last key: n_alive = start[dead_key] - off.  Removed and unused slots keep
 // the order the counting sort gave them; nothing reads them.
 __global__ __launch_bounds__(256) void k_rank(const Params* __restrict__ PB, const int* __restrict__ blk_box,
@@ -535,7 +537,8 @@ __global__ __launch_bounds__(256) void k_rank(const Params* __restrict__ PB, con
                                               const int* __restrict__ keys,
                                               const int* __restrict__ start,
                                               const int* __restrict__ perm_tmp,
-                                              int* __restrict__ perm, double4* __restrict__ pk) {
+                                              int* __restrict__ perm, double4* __restrict__ pk,
+                                              int* __restrict__ pid) {
   if (!S->rebuild || S->halt) return;
   const int box = block_box(blk_box);
   const Params& P = PB[box];
@@ -552,20 +555,47 @@ __global__ __launch_bounds__(256) void k_rank(const Params* __restrict__ PB, con
   for (int j = a; j < b; j++) r += (perm_tmp[j] < s);
   perm[a + r] = s;
   pk[a + r] = A.pos[s];
+  pid[a + r] = A.id[s];
 }
 
 // ---------------------------------------------------------------------------
 // candidate (Verlet) lists, rebuild steps only
 // ---------------------------------------------------------------------------
-// Every particle lists, in traversal order, the particles j (any level,
-// periodic images included) with ((dx*dx + dy*dy) + dz*dz) <= (ri+rj+skin)^2
-// over the cells of each level overlapping [x - reach, x + reach], reach =
-// ri + (largest radius of the level) + skin.  Entry q of particle k sits at
-// vlist[q*cap + k] (lanes of a wave read consecutive words).
+// Contact-list entries: index of the partner in cell order (moving) or in the
+// fixed-particle arrays, with a flag for fixed partners.
+constexpr unsigned CL_INDEX = 0x3fffffffu, CL_FIXED = 0x40000000u;
+
+// Every particle lists the particles j (any level, periodic images included)
+// with ((dx*dx + dy*dy) + dz*dz) <= (ri+rj+skin)^2 over the cells of each
+// level overlapping [x - reach, x + reach], reach = ri + (largest radius of
+// the level) + skin, and, with the plain search (CS_Method 1), the fixed
+// particles of the fixed grid's cells within the same distance (the
+// hierarchical search does not look for fixed particles,
+// Prtcl_Hrchl_Munjiza.f90:705-755).  The list is sorted by partner id (2 id,
+// +1 for a fixed partner), so that a particle's contact forces are summed in
+// an order that depends neither on the grid, the skin nor the storage order.
+// Entry q of particle k sits at vlist[q*cap + k] (lanes of a wave read
+// consecutive words); vkey holds the sort keys while the list is built.
+__device__ __forceinline__ void vlist_insert(int* __restrict__ vlist, int* __restrict__ vkey, size_t cap,
+                                             int k, int c, unsigned entry, int key) {
+  int b = c - 1;
+  while (b >= 0) {
+    const int kb = vkey[(size_t)b * cap + k];
+    if (kb <= key) break;
+    vkey[(size_t)(b + 1) * cap + k] = kb;
+    vlist[(size_t)(b + 1) * cap + k] = vlist[(size_t)b * cap + k];
+    b--;
+  }
+  vkey[(size_t)(b + 1) * cap + k] = key;
+  vlist[(size_t)(b + 1) * cap + k] = (int)entry;
+}
+
 __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                                Scalars* S, const BoxState* BS, const double4* __restrict__ pk,
-                                               const int* __restrict__ cstart, int* __restrict__ vcnt,
-                                               int* __restrict__ vlist) {
+                                               const int* __restrict__ pid, const int* __restrict__ cstart,
+                                               int* __restrict__ vcnt, int* __restrict__ vlist,
+                                               int* __restrict__ vkey, const double4* __restrict__ fpk,
+                                               const int* __restrict__ fid, const int* __restrict__ fstart) {
   if (!S->rebuild || S->halt) return;
   const int box = block_box(blk_box);
   const Params& P = PB[box];
@@ -574,7 +604,8 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
-  const int cap = P.cap, lcap = P.lcap;
+  const size_t cap = P.cap;
+  const int lcap = P.lcap;
   int c = 0;
   for (int L = 0; L < P.nlvl; L++) {
     const Grid& G = P.lvl[L];
@@ -620,7 +651,51 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
             if (kj == k && !shifted) continue;
-            if (c < lcap) vlist[(size_t)c * cap + k] = kj;
+            if (c < lcap) vlist_insert(vlist, vkey, cap, k, c, (unsigned)kj, 2 * pid[kj]);
+            c++;
+          }
+        }
+      }
+    }
+  }
+  if (P.search_fixed && P.nfix > 0) {
+    const Grid& G = P.fgrid;
+    const double reach = ri + G.rmax + P.skin;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      const double pad = 1.0e-6 * G.w[d];
+      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+      if (!P.periodic[d]) {
+        lo[d] = lo[d] < 0 ? 0 : lo[d];
+        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      }
+    }
+    for (int cz = lo[2]; cz <= hi[2]; cz++) {
+      int wz = cz;
+      double shz = 0.0;
+      if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
+      else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
+      for (int cy = lo[1]; cy <= hi[1]; cy++) {
+        int wy = cy;
+        double shy = 0.0;
+        if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
+        else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
+        for (int cx = lo[0]; cx <= hi[0]; cx++) {
+          int wx = cx;
+          double shx = 0.0;
+          if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
+          else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
+          const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
+          for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
+            const double4 pj = fpk[kj];
+            const double ddx = pi.x - (pj.x + shx);
+            const double ddy = pi.y - (pj.y + shy);
+            const double ddz = pi.z - (pj.z + shz);
+            const double dr = ri + pj.w + P.skin;
+            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (d2 > dr * dr) continue;
+            if (c < lcap) vlist_insert(vlist, vkey, cap, k, c, (unsigned)kj | CL_FIXED, 2 * fid[kj] + 1);
             c++;
           }
         }
@@ -691,23 +766,15 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
   return xj;
 }
 
-// Contact-list entries: index of the partner in buffer A (moving) or in the
-// fixed-particle arrays, a flag for fixed partners and, for moving ones, a
-// flag for a periodic image.
-constexpr unsigned CL_INDEX = 0x3fffffffu, CL_FIXED = 0x40000000u, CL_SHIFT = 0x80000000u;
-
-// Contact test of one step: every particle walks its candidate list and keeps
-// the partners that touch it under the reference's predicate
-// (NBSM_FineSearch2/3, Prtcl_NBS_Munjiza.f90:503-557), then, with the plain
-// search (CS_Method 1), the fixed particles of its cells (FineSearch1
-// :478-498; the hierarchical search does not look for them).  The list is
-// sorted by partner id (fixed ids follow all moving ones), so that the sums
-// of a particle's contact forces do not depend on the grid, the skin or the
-// storage order.  A light kernel (few registers), so that many waves hide
-// the gather latency.
-__device__ __forceinline__ int cl_id(const Buf& A, const int* __restrict__ fid, unsigned e) {
-  return (e & CL_FIXED) ? fid[e & CL_INDEX] : A.id[e & CL_INDEX];
-}
+// Contact test of one step: every particle walks its candidate list (sorted
+// by partner id at the rebuild) and keeps the partners that touch it under
+// the reference's predicate ((dx*dx + dy*dy) + dz*dz <= (ri+rj)^2 with
+// early-outs, NBSM_FineSearch1/2/3, Prtcl_NBS_Munjiza.f90:478-557) on the
+// image coordinates, so the contact list comes out sorted as well.  Entries:
+// the partner's index in buffer A (moving) or in the fixed arrays, with the
+// fixed flag, and a flag for a periodic image.  A light kernel (few
+// registers), so that many waves hide the gather latency.
+constexpr unsigned CL_SHIFT = 0x80000000u;
 
 __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ blk_box, Buf A, Scalars* S,
@@ -717,29 +784,27 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ vlist,
                                                  unsigned* __restrict__ clist,
                                                  int* __restrict__ ccnt,
-                                                 const double4* __restrict__ fpk,
-                                                 const int* __restrict__ fid,
-                                                 const int* __restrict__ fstart) {
+                                                 const double4* __restrict__ fpk) {
   if (S->halt || S->overflow == 2) return;
   const int box = block_box(blk_box);
   const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k - P.off >= BS[box].n_alive) return;
   const int nq = min(vcnt[k], P.lcap);
-  const bool fixed = P.search_fixed && P.nfix > 0;
-  if (nq == 0 && !fixed) {  // loose beds: most particles have no candidate at all
+  if (nq == 0) {  // loose beds: most particles have no candidate at all
     ccnt[k] = 0;
     return;
   }
-  const bool rb = S->rebuild != 0;
-  const int s = rb ? perm[k] : k;
+  const bool rebuilt = S->rebuild != 0;
+  const int s = rebuilt ? perm[k] : k;
   const double4 pi = A.pos[s];
   const size_t cap = P.cap;
   int c = 0;
   for (int q = 0; q < nq; q++) {
-    const int kj = vlist[(size_t)q * cap + k];
-    const int sj = rb ? perm[kj] : kj;
-    const double4 pj = A.pos[sj];
+    const unsigned e = (unsigned)vlist[(size_t)q * cap + k];
+    const bool fixed = (e & CL_FIXED) != 0;
+    const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+    const double4 pj = fixed ? fpk[j] : A.pos[j];
     int shifted = 0;
     const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
     const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
@@ -755,76 +820,11 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     const double ddz = pi.z - zj;
     d2 = ddz * ddz + d2;
     if (d2 > dr2) continue;
-    if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)sj | (shifted ? CL_SHIFT : 0u);
+    if (c < P.hcap) clist[(size_t)c * cap + k] = fixed ? e : ((unsigned)j | (shifted ? CL_SHIFT : 0u));
     c++;
   }
-  if (fixed) {
-    const Grid& G = P.fgrid;
-    const double xv[3] = {pi.x, pi.y, pi.z};
-    const double reach = pi.w + G.rmax;
-    int lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-      const double pad = 1.0e-6 * G.w[d];
-      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-      if (!P.periodic[d]) {
-        lo[d] = lo[d] < 0 ? 0 : lo[d];
-        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-      }
-    }
-    for (int cz = lo[2]; cz <= hi[2]; cz++) {
-      int wz = cz;
-      double shz = 0.0;
-      if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
-      else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
-      for (int cy = lo[1]; cy <= hi[1]; cy++) {
-        int wy = cy;
-        double shy = 0.0;
-        if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
-        else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
-        for (int cx = lo[0]; cx <= hi[0]; cx++) {
-          int wx = cx;
-          double shx = 0.0;
-          if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
-          else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
-          const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
-          for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
-            const double4 pj = fpk[kj];
-            const double dr = pi.w + pj.w;
-            const double dr2 = dr * dr;
-            const double ddx = pi.x - (pj.x + shx);
-            double d2 = ddx * ddx;
-            if (d2 > dr2) continue;
-            const double ddy = pi.y - (pj.y + shy);
-            d2 = ddy * ddy + d2;
-            if (d2 > dr2) continue;
-            const double ddz = pi.z - (pj.z + shz);
-            d2 = ddz * ddz + d2;
-            if (d2 > dr2) continue;
-            if (c < P.hcap) clist[(size_t)c * cap + k] = (unsigned)kj | CL_FIXED;
-            c++;
-          }
-        }
-      }
-    }
-  }
   ccnt[k] = c;
-  if (c > P.hcap) {
-    atomicMax(&S->overflow, 1);
-    return;
-  }
-  for (int a = 1; a < c; a++) {
-    const unsigned e = clist[(size_t)a * cap + k];
-    const int ida = cl_id(A, fid, e);
-    int b = a - 1;
-    while (b >= 0) {
-      const unsigned eb = clist[(size_t)b * cap + k];
-      if (cl_id(A, fid, eb) <= ida) break;
-      clist[(size_t)(b + 1) * cap + k] = eb;
-      b--;
-    }
-    clist[(size_t)(b + 1) * cap + k] = e;
-  }
+  if (c > P.hcap) atomicMax(&S->overflow, 1);
 }
 
 __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
@@ -836,7 +836,6 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
-                                               const int* __restrict__ fstart,
                                                Slot* __restrict__ slots, BoxSlot* __restrict__ bslots) {
   // a candidate-list overflow in this step's rebuild leaves the lists
   // incomplete: do nothing, the host regrows them and replays the step
@@ -844,7 +843,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
   const int box = block_box(blk_box);
   const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool rb = S->rebuild != 0;
+  const bool rebuilt = S->rebuild != 0;  // this step re-sorted: A is read through perm
   // k_ctl has advanced iterNumber: is the next step a near-wall update?
   const bool upd_next = S->iter >= BS[box].next_wall_update;
   Acc acc;
@@ -857,7 +856,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
   bool dead = false;
   if (k - P.off < BS[box].n_alive) {
     const int cap = P.cap;
-    const int s = rb ? perm[k] : k;
+    const int s = rebuilt ? perm[k] : k;
     const double4 pi = A.pos[s];
     const d3 xi = D3(pi.x, pi.y, pi.z);
     const double ri = pi.w;
@@ -867,10 +866,10 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     const d3 wi = ld3(A.rv, cap, s);
 
     // ---------------- every contact, one call site ------------------------
-    // The contacts k_contact found (moving partners, then fixed ones, by
-    // ascending partner id), then the walls of the near-wall list, newest
-    // first (FindContactsPW :298-319 walks the list from the highest index).
-    // Each contact is reduced to the arguments of the force routine from the
+    // The contacts k_contact found, in ascending partner id, then the walls
+    // of the near-wall list, newest first
+    // (FindContactsPW :298-319 walks the list from the highest index).  Each
+    // contact is reduced to the arguments of the force routine from the
     // perspective the reference evaluates it from (ContactForce_PP :732-756,
     // _PPG :761-798, _PPFix :803-827, _PW :832-856), so the routine is
     // inlined once.
@@ -1137,24 +1136,22 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.S, D.keys, D.rank, D.start, D.perm_tmp);
   hipLaunchKernelGGL(k_rank, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.start,
-                     D.perm_tmp, D.perm, D.pk);
+                     D.perm_tmp, D.perm, D.pk, D.pid);
   return hipGetLastError();
 }
 
 hipError_t launch_vlist(const Dev& D, hipStream_t st) {
-  hipLaunchKernelGGL(k_vlist, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box, D.S, D.BS, D.pk,
-                     D.start, D.vcnt, D.vlist);
+  hipLaunchKernelGGL(k_vlist, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box,
+                     D.S, D.BS, D.pk, D.pid, D.start, D.vcnt, D.vlist, D.vkey, D.fpk, D.fid, D.fstart);
   return hipGetLastError();
 }
 
 hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid) {
   hipLaunchKernelGGL(k_contact, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                     D.blk_box, A, D.S, D.BS, D.perm, D.vcnt, D.vlist, D.clist, D.ccnt, D.fpk, D.fid,
-                     D.fstart);
+                     D.blk_box, A, D.S, D.BS, D.perm, D.vcnt, D.vlist, D.clist, D.ccnt, D.fpk);
   if (mid) (void)hipEventRecord(mid, st);
   hipLaunchKernelGGL(k_force, dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A,
-                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.fstart, D.slots,
-                     D.bslots);
+                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.slots, D.bslots);
   return hipGetLastError();
 }
 
