@@ -18,22 +18,9 @@
 #include "../../include/sdem.h"
 #include "../../include/sdem_gnn.h"
 #include "gnn_internal.h"
+#include "sdem_host.h"
 
 using namespace gnn;
-
-// device view of a DEM context's first box (sdem_api.cpp)
-struct sdem_dem_view {
-  double4* pos;
-  double* lv;
-  double* la;
-  int* id;
-  int cap, n;
-  double dt, g[3], lo[3], hi[3];
-  int per[3];
-  hipStream_t stream;
-};
-extern "C" int sdem_internal_view(sdem_ctx* c, sdem_dem_view* v);
-extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps);
 
 namespace {
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/sdem.h"
+#include "sdem_host.h"
 #include "sdem_internal.h"
 
 namespace sdem {
@@ -1577,17 +1578,6 @@ void sdem_destroy(sdem_ctx* c) {
 // ===========================================================================
 // Access for the learned surrogate (gnn_api.cpp), single-box contexts only.
 // ===========================================================================
-struct sdem_dem_view {
-  double4* pos;
-  double* lv;
-  double* la;
-  int* id;
-  int cap, n;
-  double dt, g[3], lo[3], hi[3];
-  int per[3];
-  hipStream_t stream;
-};
-
 extern "C" int sdem_internal_view(sdem_ctx* c, sdem_dem_view* v) {
   if (!c || !v) return SDEM_E_ARG;
   if (c->boxes.size() != 1) return c->fail(SDEM_E_STATE, "the surrogate drives single-box contexts");
@@ -1648,4 +1638,66 @@ extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps) {
   c->stepped = true;
   c->first_step_pending = false;
   return SDEM_OK;
+}
+
+extern "C" int sdem_internal_box_info(sdem_ctx* c, sdem_box_info* out) {
+  if (!c || !out) return SDEM_E_ARG;
+  Box& b = c->box();
+  if (!b.have_pprop) return c->fail(SDEM_E_STATE, "particle properties first");
+  static thread_local std::vector<double> rad;
+  static thread_local std::vector<int32_t> wid;
+  rad.resize(b.ntype);
+  for (int t = 0; t < b.ntype; t++) rad[t] = b.pprop[t].radius;
+  wid.resize(b.walls.size());
+  for (size_t w = 0; w < b.walls.size(); w++) wid[w] = b.walls[w].wall_id;
+  out->tsize = c->tsize;
+  out->rsize = c->rsize;
+  out->num_prtcl = b.o.num_prtcl;
+  out->ntype = b.ntype;
+  out->nwall = (int32_t)b.walls.size();
+  out->nfix = (int32_t)b.fix_id.size();
+  out->keep_forces = c->keep_forces;
+  for (int d = 0; d < 3; d++) {
+    out->lo[d] = b.o.domain_min[d];
+    out->hi[d] = b.o.domain_max[d];
+    out->periodic[d] = b.o.is_periodic[d] ? 1 : 0;
+  }
+  out->type_radius = rad.data();
+  out->wall_id = wid.data();
+  out->fix_id = b.fix_id.data();
+  return SDEM_OK;
+}
+
+extern "C" int sdem_internal_set_marks(sdem_ctx* c, int32_t n, const int32_t* id, const int32_t* mark) {
+  if (!c || n < 0 || (n > 0 && (!id || !mark))) return SDEM_E_ARG;
+  Box& b = c->box();
+  if (!b.have_state) return c->fail(SDEM_E_STATE, "no state");
+  std::vector<std::pair<int, int>> want(n);
+  for (int i = 0; i < n; i++) want[i] = {id[i], mark[i]};
+  std::sort(want.begin(), want.end());
+  auto find = [&](int pid) -> const std::pair<int, int>* {
+    auto it = std::lower_bound(want.begin(), want.end(), std::make_pair(pid, INT32_MIN));
+    return (it != want.end() && it->first == pid) ? &*it : nullptr;
+  };
+  if (!c->stepped) {
+    for (int i = 0; i < b.n; i++)
+      if (auto p = find(b.id[i])) b.mark[i] = p->second;
+    return SDEM_OK;
+  }
+  std::vector<std::pair<int, int>> idx;
+  if (int rc = live_index(c, idx)) return rc;
+  Buf& A = c->buf[c->cur];
+  std::vector<int> mk;
+  hipError_t e = dl(mk, A.mark, (size_t)c->nslots);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(marks)");
+  for (const auto& q : idx)
+    if (auto p = find(q.first)) mk[q.second] = p->second;
+  e = hipMemcpy(A.mark, mk.data(), mk.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(marks)");
+  return SDEM_OK;
+}
+
+extern "C" int sdem_internal_fail(sdem_ctx* c, int32_t code, const char* msg) {
+  if (!c) return code;
+  return c->fail(code, msg ? msg : "");
 }

@@ -1,5 +1,7 @@
 ! main.f90 -- the reference's driver (src/DEM/main.f90:16-38) on one GPU:
-! read the .prm file, initialise, iterate ifirst..ilast.  No MPI.
+! read the .prm file, initialise (from a restart file with RestartFlag = T),
+! write the visualisation dump of ifirst-1, iterate ifirst..ilast with the
+! reference's dumps and restart backups, close the XDMF index.  No MPI.
 !
 !   sdem_main <file.prm>          run the case
 !   sdem_main <file.prm> <dir>    also write raw state dumps <dir>/state_<iter>.bin
@@ -15,6 +17,7 @@ program main_DEM
   use Prtcl_Geometry
   use Prtcl_Variables
   use Prtcl_DEMSystem
+  use Prtcl_IOAndVisu
   implicit none
   character(len=256) :: chDEMPrm, chDump, fname
   integer :: i, nargs
@@ -55,9 +58,11 @@ program main_DEM
     write(fname, '(A,"/state_",I10.10,".bin")') trim(chDump), DEM_opt%ifirst - 1
     call DEM%dump_state(fname)
   end if
+  call DEM_IO%Dump_visu(DEM%ctx, DEM_opt%ifirst - 1, DEM_opt%np_InDomain)
   do i = DEM_opt%ifirst, DEM_opt%ilast
     call DEM%iterate(i)
   end do
+  call DEM_IO%Final_visu()
   if (len_trim(chDump) > 0) then
     write(fname, '(A,"/state_",I10.10,".bin")') trim(chDump), DEM_opt%ilast
     call DEM%dump_state(fname)

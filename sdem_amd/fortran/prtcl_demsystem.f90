@@ -1,10 +1,11 @@
 ! prtcl_demsystem.f90 -- DEMSystem with the reference's call sequence
 ! (src/DEM/Prtcl_DEMSystem.f90): Initialize reads the .prm file's property,
-! geometry and particle-making namelists and pushes them into a libsdem_hip
-! context (DEMS_Initialize :58-195); iterate(itime) is one DEMS_iterate
-! (:201-317), i.e. one sdem_step, with the reference's terminal report every
-! Cmd_LFile_Freq steps.  Errors stop the program with the library's message,
-! as DEMLogInfo%CheckForError(ErrT_Abort, ...) does.
+! geometry and particle-making namelists, or the restart file, and pushes
+! them into a libsdem_hip context (DEMS_Initialize :58-195); iterate(itime)
+! is one DEMS_iterate (:201-317), i.e. one sdem_step, followed by the
+! reference's visualisation dumps and restart backups and its terminal
+! report every Cmd_LFile_Freq steps.  Errors stop the program with the
+! library's message, as DEMLogInfo%CheckForError(ErrT_Abort, ...) does.
 module Prtcl_DEMSystem
   use, intrinsic :: iso_c_binding
   use sdem_capi
@@ -12,6 +13,7 @@ module Prtcl_DEMSystem
   use Prtcl_Property
   use Prtcl_Geometry
   use Prtcl_Variables
+  use Prtcl_IOAndVisu
   implicit none
   private
 
@@ -48,10 +50,12 @@ contains
     type(sdem_options) :: o
     character(32) :: envseed
     integer :: seed, stat
+    integer(c_int32_t) :: n
 
     seed = DEFAULT_SEED
     call get_environment_variable("SDEM_SEED", envseed, status=stat)
     if (stat == 0) read(envseed, *) seed
+    call execute_command_line('mkdir -p '//trim(DEM_opt%Res_Dir)//' '//trim(DEM_opt%RestartDir)//' 2> /dev/null')
     call DEMProperty%InitPrtclProperty(chFile, seed)
     call DEMProperty%InitWallProperty(chFile)
     call DEMGeometry%MakeGeometry(chFile)
@@ -60,14 +64,19 @@ contains
     call check(this, DEMProperty%push(this%ctx), "properties")
     call check(this, DEMGeometry%push(this%ctx), "geometry")
     if (DEM_opt%RestartFlag) then
-      print *, "Initialize: restart files are not supported by this driver"; stop 1
+      ! PIO_Read_Restart and PIO_RestartCL: state and contact list
+      call DEM_IO%Read_Restart(this%ctx)
+      call check(this, sdem_get_num_particles(this%ctx, n), "sdem_get_num_particles")
+      DEM_opt%np_InDomain = n
+    else
+      call GL_MakingAllPrtcl(chFile)
+      DEM_opt%np_InDomain = nlocal
+      call check(this, sdem_set_state(this%ctx, int(nlocal, c_int32_t), GPrtcl_id, GPrtcl_pType, &
+                                      GPrtcl_UsrMark, GPrtcl_PosR, GPrtcl_linVel, GPrtcl_linAcc, GPrtcl_theta, &
+                                      GPrtcl_rotVel, GPrtcl_rotAcc), "sdem_set_state")
     end if
-    call GL_MakingAllPrtcl(chFile)
-    DEM_opt%np_InDomain = nlocal
-    call check(this, sdem_set_state(this%ctx, int(nlocal, c_int32_t), GPrtcl_id, GPrtcl_pType, GPrtcl_UsrMark, &
-                                    GPrtcl_PosR, GPrtcl_linVel, GPrtcl_linAcc, GPrtcl_theta, GPrtcl_rotVel, &
-                                    GPrtcl_rotAcc), "sdem_set_state")
-    print '(A,I0,A,I0,A)', "DEM initialized: ", nlocal, " particles, ", DEMGeometry%nPW, " walls"
+    call DEM_IO%Init_visu(chFile, DEM_opt%np_InDomain)
+    print '(A,I0,A,I0,A)', "DEM initialized: ", DEM_opt%np_InDomain, " particles, ", DEMGeometry%nPW, " walls"
   end subroutine DEMS_Initialize
 
   subroutine DEMS_iterate(this, itime)
@@ -77,15 +86,23 @@ contains
     integer(c_int64_t) :: t0, t1, rate
     call system_clock(t0, rate)
     call check(this, sdem_step(this%ctx, 1_c_int32_t), "sdem_step")
+    call system_clock(t1)
+    this%t_steps = this%t_steps + real(t1 - t0, RK) / real(rate, RK)
+    ! results and backups (DEMS_iterate :250-265)
+    if (mod(itime, DEM_opt%SaveVisu) == 0 .or. mod(itime, DEM_opt%BackupFreq) == 0 .or. &
+        itime == DEM_opt%ilast) then
+      call check(this, sdem_get_num_particles(this%ctx, n), "sdem_get_num_particles")
+      DEM_opt%np_InDomain = n
+      if (mod(itime, DEM_opt%SaveVisu) == 0) call DEM_IO%Dump_visu(this%ctx, itime, int(n))
+      if (mod(itime, DEM_opt%BackupFreq) == 0 .or. itime == DEM_opt%ilast) then
+        call DEM_IO%Write_Restart(this%ctx, itime)
+        call DEM_IO%Delete_Prev_Restart(itime)
+      end if
+    end if
     if (mod(itime, DEM_opt%Cmd_LFile_Freq) == 0 .or. itime == DEM_opt%ilast) then
       call check(this, sdem_get_counts(this%ctx, cnt), "sdem_get_counts")
       call check(this, sdem_get_num_particles(this%ctx, n), "sdem_get_num_particles")
-      call system_clock(t1)
-      this%t_steps = this%t_steps + real(t1 - t0, RK) / real(rate, RK)
       print '(A,I0,A,I0,A,4(1X,I0))', "iter ", itime, ": ", n, " particles, contacts P-P P-GP P-FP P-W:", cnt
-    else
-      call system_clock(t1)
-      this%t_steps = this%t_steps + real(t1 - t0, RK) / real(rate, RK)
     end if
   end subroutine DEMS_iterate
 

@@ -157,7 +157,45 @@ module sdem_capi
       import :: c_ptr
       type(c_ptr), value :: ctx
     end subroutine
+    ! include/sdem_io.h: the reference's restart and visualisation files
+    integer(c_int) function sdem_write_restart(ctx, path, mark_links) bind(C, name="sdem_write_restart")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: mark_links
+    end function
+    integer(c_int) function sdem_read_restart(ctx, path) bind(C, name="sdem_read_restart")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function sdem_write_visu(ctx, path, fields) bind(C, name="sdem_write_visu")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: fields
+    end function
+    integer(c_int) function sdem_xdmf_begin(path, ifirst, ilast, save_visu) bind(C, name="sdem_xdmf_begin")
+      import :: c_int, c_char, c_int32_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: ifirst, ilast, save_visu
+    end function
+    integer(c_int) function sdem_xdmf_grid(path, data_base, itime, np, fields) bind(C, name="sdem_xdmf_grid")
+      import :: c_int, c_char, c_int32_t
+      character(kind=c_char), intent(in) :: path(*), data_base(*)
+      integer(c_int32_t), value :: itime, np, fields
+    end function
+    integer(c_int) function sdem_xdmf_end(path) bind(C, name="sdem_xdmf_end")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+    end function
   end interface
+
+  ! fields of the visualisation dump (SDEM_VISU_* of include/sdem_io.h)
+  integer(c_int32_t), parameter :: SDEM_VISU_ID = 1, SDEM_VISU_DIAMETER = 2, SDEM_VISU_TYPE = 4, &
+                                   SDEM_VISU_USRMARK = 8, SDEM_VISU_LINVEL = 16, SDEM_VISU_LINACC = 32, &
+                                   SDEM_VISU_THETA = 64, SDEM_VISU_ROTVEL = 128, SDEM_VISU_ROTACC = 256, &
+                                   SDEM_VISU_CNTCTFORCE = 512, SDEM_VISU_TORQUE = 1024
 
 contains
 
@@ -177,5 +215,12 @@ contains
       msg(i:i) = s(i)
     end do
   end function sdem_last_error
+
+  ! a Fortran string as a NUL-terminated C string
+  function c_str(s) result(c)
+    character(*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: c
+    c = trim(s) // c_null_char
+  end function c_str
 
 end module sdem_capi

@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "libsdem_hip.so"
 
 SDEM_OK = 0
+SDEM_E_ARG, SDEM_E_HIP, SDEM_E_CAPACITY, SDEM_E_STATE, SDEM_E_NODEV = -1, -2, -3, -4, -5
 ERRORS = {-1: "SDEM_E_ARG", -2: "SDEM_E_HIP", -3: "SDEM_E_CAPACITY", -4: "SDEM_E_STATE",
           -5: "SDEM_E_NODEV"}
 
@@ -119,6 +120,12 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "set_block_sizes": ([V, C.c_int32, C.c_int32], C.c_int),
         "get_tuning": ([V, _DP, _DP, _IP, _IP], C.c_int),
         "autotune": ([V, C.c_int32, _IP, _DP], C.c_int),
+        "write_restart": ([V, C.c_char_p, C.c_int32], C.c_int),
+        "read_restart": ([V, C.c_char_p], C.c_int),
+        "write_visu": ([V, C.c_char_p, C.c_uint32], C.c_int),
+        "xdmf_begin": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "xdmf_grid": ([C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
+        "xdmf_end": ([C.c_char_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = f(name)
@@ -288,6 +295,19 @@ class Engine:
             _dptr(out["tan_delta"]), _dptr(out["vrel_init"])), "get_contacts")
         return out
 
+    # --------------------------------------------- files (include/sdem_io.h)
+    def write_restart(self, path: str, mark_links: bool = False) -> None:
+        """PIO_Write_Restart: the reference's restart file of this box."""
+        self._check(self._fn("write_restart")(self.h, os.fsencode(path), int(mark_links)), "write_restart")
+
+    def read_restart(self, path: str) -> None:
+        """PIO_Read_Restart + PIO_RestartCL: state and contact records."""
+        self._check(self._fn("read_restart")(self.h, os.fsencode(path)), "read_restart")
+
+    def write_visu(self, path: str, fields: int = 0) -> None:
+        """PIO_Dump_visu: positions and the VISU_* fields selected."""
+        self._check(self._fn("write_visu")(self.h, os.fsencode(path), int(fields)), "write_visu")
+
     def get_counts(self) -> np.ndarray:
         c = np.zeros(4, np.int32)
         self._check(self._fn("get_counts")(self.h, _iptr(c)), "get_counts")
@@ -387,6 +407,33 @@ def _push(eng: Engine, scenario: dict) -> None:
     ct = scenario.get("contacts")
     if ct is not None and len(ct["kind"]):
         eng.set_contacts(ct["kind"], ct["owner"], ct["partner"], ct["tan_delta"], ct["vrel_init"])
+
+
+# fields of the visualisation dump (include/sdem_io.h), in file order
+VISU_FIELDS = ("ID", "Diameter", "Type", "UsrMark", "LinVel", "LinAcc", "Theta", "RotVel", "RotAcc",
+               "CntctForce", "Torque")
+VISU = {name: 1 << k for k, name in enumerate(VISU_FIELDS)}
+
+
+def xdmf_begin(lib: C.CDLL, xmf: str, ifirst: int, ilast: int, save_visu: int) -> None:
+    _declare(lib, "sdem_")
+    rc = lib.sdem_xdmf_begin(os.fsencode(xmf), int(ifirst), int(ilast), int(save_visu))
+    if rc != SDEM_OK:
+        raise SdemError(rc, "sdem_xdmf_begin", xmf)
+
+
+def xdmf_grid(lib: C.CDLL, xmf: str, data_base: str, itime: int, np_: int, fields: int) -> None:
+    _declare(lib, "sdem_")
+    rc = lib.sdem_xdmf_grid(os.fsencode(xmf), os.fsencode(data_base), int(itime), int(np_), int(fields))
+    if rc != SDEM_OK:
+        raise SdemError(rc, "sdem_xdmf_grid", xmf)
+
+
+def xdmf_end(lib: C.CDLL, xmf: str) -> None:
+    _declare(lib, "sdem_")
+    rc = lib.sdem_xdmf_end(os.fsencode(xmf))
+    if rc != SDEM_OK:
+        raise SdemError(rc, "sdem_xdmf_end", xmf)
 
 
 def build_engine(lib: C.CDLL, scenario: dict, prefix: str = "sdem_", tuning: Optional[dict] = None,

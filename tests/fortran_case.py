@@ -8,19 +8,25 @@ CASE = dict(num=1500, steps=40, dt=2.0e-5, box=(0.04, 0.06, 0.04), periodic=(Tru
             diam=(2.0e-3, 3.0e-3), dens=(2500.0, 2600.0), E=5.0e7, nu=0.3, bins=(2.0, 1.0))
 
 
-def prm_text(c=CASE, cf_type=2, pi=2, pri=2) -> str:
+def prm_text(c=CASE, cf_type=2, pi=2, pri=2, restart=False, ifirst=1, ilast=None, save_visu=100000,
+             backup=100000, visu=None) -> str:
+    """visu: None (no &PrtclVisuOption) or a dict of its logicals."""
     T = lambda b: "T" if b else "F"  # noqa: E731
     bx = c["box"]
+    ilast = c["steps"] if ilast is None else ilast
+    visu_nml = ""
+    if visu is not None:
+        visu_nml = "&PrtclVisuOption\n" + "".join(f"  {k} = {T(v)}\n" for k, v in visu.items()) + "/\n"
     return f"""!
 ! settling of two sphere sizes between two infinite planes (y = 0, y = top)
 !
 &DEMOptions
-  RestartFlag = F
+  RestartFlag = {T(restart)}
   numPrtclFix = 0
   numPrtcl = {c['num']}
   dtDEM = {c['dt']:.6E}
-  ifirstDEM = 1
-  ilastDEM = {c['steps']}
+  ifirstDEM = {ifirst}
+  ilastDEM = {ilast}
   gravity = 0.000 -9.81 0.0
   minpoint = 0.000 0.000 0.000
   maxpoint = {bx[0]} {bx[1]} {bx[2]}
@@ -41,8 +47,8 @@ def prm_text(c=CASE, cf_type=2, pi=2, pri=2) -> str:
   RunName = "Settling"
   Res_Dir = "./"
   RestartDir = "./"
-  SaveVisuDEM = 100000
-  BackupFreqDEM = 100000
+  SaveVisuDEM = {save_visu}
+  BackupFreqDEM = {backup}
   Cmd_LFile_Freq = 10
   LF_file_lvl = 5
   LF_cmdw_lvl = 3
@@ -101,7 +107,7 @@ def prm_text(c=CASE, cf_type=2, pi=2, pri=2) -> str:
   Point4 = {bx[0]} 0.00 0.00, 0.00 {bx[1]} {bx[2]},
   TraVel = 0.00 0.00 0.00, 0.00 0.00 0.00,
 /
-"""
+""" + visu_nml
 
 
 def read_dump(path: str) -> dict:

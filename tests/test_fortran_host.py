@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 from fortran_case import prm_text, read_dump, scenario
+from oracle import restart_format as R
 from sdem_amd import native
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -47,7 +48,8 @@ def test_fortran_run_matches_oracle(tmp_path, cf_type, oracle_lib):
     print(f"fortran driver: cf_type {cf_type}, 1500 spheres, 40 steps", flush=True)
     prm = tmp_path / "case.prm"
     prm.write_text(prm_text(cf_type=cf_type))
-    run = subprocess.run([driver(), str(prm), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    run = subprocess.run([driver(), str(prm), str(tmp_path)], capture_output=True, text=True, timeout=120,
+                         cwd=str(tmp_path))
     assert run.returncode == 0, run.stdout + run.stderr
     assert "Good job" in run.stdout
     s0 = read_dump(str(tmp_path / "state_0000000000.bin"))
@@ -60,3 +62,60 @@ def test_fortran_run_matches_oracle(tmp_path, cf_type, oracle_lib):
         scale = np.abs(r[k]).max() + 1e-300
         err = np.abs(s1[k] - r[k]) / (np.abs(r[k]) + scale)
         assert err.max() < 1e-10, (k, err.max())
+
+
+def run_driver(d, name, text, *args):
+    prm = d / name
+    prm.write_text(text)
+    run = subprocess.run([driver(), str(prm), *args], capture_output=True, text=True, timeout=120, cwd=str(d))
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "Good job" in run.stdout
+    return run
+
+
+@pytest.mark.gpu
+def test_fortran_restart_and_visu(tmp_path):
+    """The reference's restart and visualisation cycle through the Fortran
+    host: 40 steps straight with backups every 20 and visualisation dumps
+    every 20 against 20 steps, then RestartFlag = T from the file of step
+    20 for steps 21-40.  The resumed run ends in the same state, bit for bit
+    (the marks aside: the backup at step 20 turns the running context's
+    marks into link counts, the file keeps the old ones, as in the
+    reference); the dumps and the XDMF index read back as the reference
+    lays them out."""
+    print("fortran driver: restart at 20 and visu dumps", flush=True)
+    visu = dict(saveXDMFOnce=False, save_ID=True, save_Diameter=False, save_Type=True, save_UsrMark=False,
+                save_LinVel=True, save_LinAcc=False, save_Theta=False, save_RotVel=True, save_RotAcc=False,
+                save_CntctForce=True, save_Torque=False)
+    a, b = tmp_path / "straight", tmp_path / "resumed"
+    a.mkdir()
+    b.mkdir()
+    run_driver(a, "case.prm", prm_text(ilast=40, save_visu=20, backup=20, visu=visu), str(a))
+    run_driver(b, "first.prm", prm_text(ilast=20, backup=20))
+    assert (b / "RestartForSettling0000000020").exists()
+    run_driver(b, "second.prm", prm_text(restart=True, ifirst=21, ilast=40, backup=20), str(b))
+    sa = read_dump(str(a / "state_0000000040.bin"))
+    sb = read_dump(str(b / "state_0000000040.bin"))
+    for k in sa:
+        if k != "usr_mark":
+            assert np.array_equal(sa[k], sb[k]), k
+    # only the newest restart file stays (PIO_Delete_Prev_Restart)
+    assert (a / "RestartForSettling0000000040").exists()
+    assert not (a / "RestartForSettling0000000020").exists()
+    # restart file of step 40 against the state dumped at 40
+    st, links = R.read_restart(str(a / "RestartForSettling0000000040"), 2, 2, [1.0e-3, 1.5e-3])
+    for k in ("id", "ptype", "lin_vel", "lin_acc", "theta", "rot_vel", "rot_acc"):
+        assert np.array_equal(st[k], sa[k]), k
+    assert np.array_equal(st["posr"], sa["posr"])
+    # visualisation dumps of steps 0, 20, 40 and their index
+    fields = {"ID", "Type", "LinVel", "RotVel", "CntctForce"}
+    s0 = read_dump(str(a / "state_0000000000.bin"))
+    for it, s in ((0, s0), (40, sa)):
+        d = R.read_visu(str(a / f"PartVisuForSettling{it:010d}"), len(s["id"]), fields)
+        assert np.array_equal(d["Pos"], s["posr"][:, :3]) and np.array_equal(d["ID"], s["id"])
+        assert np.array_equal(d["LinVel"], s["lin_vel"][:, 0, :]) and np.array_equal(d["Type"], s["ptype"])
+    assert (a / "PartVisuForSettling0000000020").exists()
+    n = len(sa["id"])
+    want = (R.xdmf_begin(1, 40, 20) + "".join(R.xdmf_grid("PartVisuForSettling", it, n, fields)
+                                               for it in (0, 20, 40)) + R.xdmf_end())
+    assert (a / "PartVisuForSettling.xmf").read_text() == want
