@@ -51,6 +51,15 @@ int sdem_write_restart(sdem_ctx* c, const char* path, int32_t mark_links);
  * particles must be set first. */
 int sdem_read_restart(sdem_ctx* c, const char* path);
 
+/* PIO_ReadFixedCoord (Prtcl_IOAndVisu.f90:164-291): the fixed spheres of
+ * RestartDir/FixedSpheresCoord.dat, num_fix packed 36-byte records
+ * {f64 x, y, z, f64 diameter, int32 type}.  Sphere k (1-based) gets the id
+ * numPrtcl + k and its type's radius; the file size must be 36 num_fix
+ * bytes, each diameter must match its type's within 1e-6 and each sphere
+ * must lie in the domain (the reference stops otherwise).  Call after the
+ * particle properties and before the state. */
+int sdem_read_fixed_coord(sdem_ctx* c, const char* path, int32_t num_fix);
+
 /* Fields of the visualisation dump (&PrtclVisuOption save_*) */
 #define SDEM_VISU_ID          (1u << 0)
 #define SDEM_VISU_DIAMETER    (1u << 1)

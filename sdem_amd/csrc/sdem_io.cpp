@@ -331,6 +331,43 @@ extern "C" int sdem_read_restart(sdem_ctx* c, const char* path) {
                            vri.data());
 }
 
+// PIO_ReadFixedCoord (Prtcl_IOAndVisu.f90:164-291), one rank
+extern "C" int sdem_read_fixed_coord(sdem_ctx* c, const char* path, int32_t num_fix) {
+  if (!c || !path || num_fix < 0) return SDEM_E_ARG;
+  sdem_box_info I;
+  if (int rc = sdem_internal_box_info(c, &I)) return rc;
+  File F(path, "rb");
+  if (!F.f) return fail(c, SDEM_E_ARG, std::string("cannot open file: ") + path);
+  std::fseek(F.f, 0, SEEK_END);
+  const long long fsize = std::ftell(F.f);
+  std::fseek(F.f, 0, SEEK_SET);
+  constexpr int REC = 3 * 8 + 8 + 4;  // real3 + real + int, packed (:185-189, :205)
+  if (fsize != (long long)REC * num_fix) return fail(c, SDEM_E_ARG, "file byte wrong");
+  std::vector<unsigned char> raw((size_t)fsize);
+  if (!F.get(raw.data(), raw.size())) return fail(c, SDEM_E_ARG, "file byte wrong");
+  std::vector<int32_t> id(num_fix), type(num_fix);
+  std::vector<double> posr(4 * (size_t)num_fix);
+  for (int k = 0; k < num_fix; k++) {
+    const unsigned char* r = &raw[(size_t)REC * k];
+    double x[4];
+    int32_t t;
+    std::memcpy(x, r, 32);
+    std::memcpy(&t, r + 32, 4);
+    if (t < 1 || t > I.ntype) return fail(c, SDEM_E_ARG, "fixed sphere type out of range");
+    const double radius = I.type_radius[t - 1];
+    if (std::fabs(2.0 * radius / x[3] - 1.0) > 1.0e-6) return fail(c, SDEM_E_ARG, "Diameter not coordinate");
+    for (int d = 0; d < 3; d++)
+      if (!(x[d] >= I.lo[d] && x[d] < I.hi[d])) return fail(c, SDEM_E_ARG, "nfix_sum/= numPrtclFix");
+    id[k] = I.num_prtcl + k + 1;  // "NOTE HERE" (:249)
+    type[k] = t;
+    posr[4 * (size_t)k] = x[0];
+    posr[4 * (size_t)k + 1] = x[1];
+    posr[4 * (size_t)k + 2] = x[2];
+    posr[4 * (size_t)k + 3] = radius;
+  }
+  return sdem_set_fixed_particles(c, num_fix, id.data(), type.data(), posr.data());
+}
+
 // ---------------------------------------------------------------------------
 // visualisation dump and its XDMF index
 // ---------------------------------------------------------------------------

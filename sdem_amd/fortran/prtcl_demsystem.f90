@@ -63,6 +63,9 @@ contains
     call check(this, sdem_create(o, this%ctx), "sdem_create")
     call check(this, DEMProperty%push(this%ctx), "properties")
     call check(this, DEMGeometry%push(this%ctx), "geometry")
+    ! fixed spheres (PIO_ReadFixedCoord): the library takes them before the
+    ! moving particles' state
+    if (DEM_opt%numPrtclFix > 0) call DEM_IO%ReadFixedCoord(this%ctx)
     if (DEM_opt%RestartFlag) then
       ! PIO_Read_Restart and PIO_RestartCL: state and contact list
       call DEM_IO%Read_Restart(this%ctx)

@@ -1,6 +1,7 @@
 ! prtcl_ioandvisu.f90 -- DEM_IO with the reference's procedures
 ! (src/DEM/Prtcl_IOAndVisu.f90) for one rank: the restart files
-! RestartFor<RunName><itime> and the visualisation dumps
+! RestartFor<RunName><itime>, the fixed spheres' FixedSpheresCoord.dat and
+! the visualisation dumps
 ! PartVisuFor<RunName><itime> with their XDMF index, all written and read by
 ! libsdem_hip.so (include/sdem_io.h) in the reference's binary layouts.
 ! &PrtclVisuOption is read as PIO_Init_visu does (:64-132); a .prm file
@@ -22,6 +23,7 @@ module Prtcl_IOAndVisu
     procedure :: Dump_visu => PIO_Dump_visu
     procedure :: Final_visu => PIO_Final_visu
     procedure :: Read_Restart => PIO_Read_Restart
+    procedure :: ReadFixedCoord => PIO_ReadFixedCoord
     procedure :: Write_Restart => PIO_Write_Restart
     procedure :: Delete_Prev_Restart => PIO_Delete_Prev_Restart
   end type Prtcl_IO_Visu
@@ -138,6 +140,14 @@ contains
     type(c_ptr), intent(in) :: ctx
     call io_check(ctx, sdem_read_restart(ctx, c_str(restart_file(DEM_opt%ifirst - 1))), "sdem_read_restart")
   end subroutine PIO_Read_Restart
+
+  ! PIO_ReadFixedCoord (:164-291): RestartDir/FixedSpheresCoord.dat
+  subroutine PIO_ReadFixedCoord(this, ctx)
+    class(Prtcl_IO_Visu) :: this
+    type(c_ptr), intent(in) :: ctx
+    call io_check(ctx, sdem_read_fixed_coord(ctx, c_str(trim(DEM_opt%RestartDir)//"FixedSpheresCoord.dat"), &
+                  int(DEM_opt%numPrtclFix, c_int32_t)), "sdem_read_fixed_coord")
+  end subroutine PIO_ReadFixedCoord
 
   ! PIO_Write_Restart (:849-1031); the marks become the link counts, as
   ! the reference's writer leaves them

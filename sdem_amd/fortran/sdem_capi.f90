@@ -169,6 +169,12 @@ module sdem_capi
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: path(*)
     end function
+    integer(c_int) function sdem_read_fixed_coord(ctx, path, num_fix) bind(C, name="sdem_read_fixed_coord")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: num_fix
+    end function
     integer(c_int) function sdem_write_visu(ctx, path, fields) bind(C, name="sdem_write_visu")
       import :: c_int, c_ptr, c_char, c_int32_t
       type(c_ptr), value :: ctx

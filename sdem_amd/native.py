@@ -123,6 +123,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "write_restart": ([V, C.c_char_p, C.c_int32], C.c_int),
         "read_restart": ([V, C.c_char_p], C.c_int),
         "write_visu": ([V, C.c_char_p, C.c_uint32], C.c_int),
+        "read_fixed_coord": ([V, C.c_char_p, C.c_int32], C.c_int),
         "xdmf_begin": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "xdmf_grid": ([C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
         "xdmf_end": ([C.c_char_p], C.c_int),
@@ -304,6 +305,10 @@ class Engine:
         """PIO_Read_Restart + PIO_RestartCL: state and contact records."""
         self._check(self._fn("read_restart")(self.h, os.fsencode(path)), "read_restart")
 
+    def read_fixed_coord(self, path: str, num_fix: int) -> None:
+        """PIO_ReadFixedCoord: fixed spheres from FixedSpheresCoord.dat."""
+        self._check(self._fn("read_fixed_coord")(self.h, os.fsencode(path), int(num_fix)), "read_fixed_coord")
+
     def write_visu(self, path: str, fields: int = 0) -> None:
         """PIO_Dump_visu: positions and the VISU_* fields selected."""
         self._check(self._fn("write_visu")(self.h, os.fsencode(path), int(fields)), "write_visu")
@@ -403,6 +408,8 @@ def _push(eng: Engine, scenario: dict) -> None:
     fx = scenario.get("fixed")
     if fx is not None and len(fx["id"]):
         eng.set_fixed_particles(fx["id"], fx["ptype"], fx["posr"])
+    if scenario.get("fixed_file"):  # (path of FixedSpheresCoord.dat, numPrtclFix)
+        eng.read_fixed_coord(*scenario["fixed_file"])
     eng.set_state(scenario["state"])
     ct = scenario.get("contacts")
     if ct is not None and len(ct["kind"]):

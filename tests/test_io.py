@@ -247,3 +247,50 @@ def test_visu_dump(tmp_path, hip_lib):
     eng.write_visu(path, native.VISU["Type"])
     d = R.read_visu(path, len(st["id"]), {"Type"})
     assert np.array_equal(d["Type"], st["ptype"])
+
+
+def write_fixed_coord(path, fx):
+    """FixedSpheresCoord.dat as PIO_ReadFixedCoord reads it
+    (Prtcl_IOAndVisu.f90:164-291): packed {f64 x, y, z, f64 diameter, int32 type}."""
+    rec = np.dtype([("p", "<f8", 3), ("d", "<f8"), ("t", "<i4")])
+    a = np.zeros(len(fx["id"]), rec)
+    a["p"] = fx["posr"][:, :3]
+    a["d"] = 2 * fx["posr"][:, 3]
+    a["t"] = fx["ptype"]
+    assert a.itemsize == 36
+    a.tofile(path)
+
+
+@pytest.mark.gpu
+def test_fixed_spheres_file(tmp_path, hip_lib):
+    """Fixed spheres read from FixedSpheresCoord.dat (ids numPrtcl + k) run
+    bit for bit as the same spheres passed by sdem_set_fixed_particles; a
+    diameter that does not match its type, or a file of the wrong size, is
+    refused."""
+    print("fixed spheres file", flush=True)
+    sc = case(seed=31)
+    fx = sc["fixed"]
+    assert np.array_equal(fx["id"], sc["options"]["num_prtcl"] + 1 + np.arange(len(fx["id"])))
+    path = str(tmp_path / "FixedSpheresCoord.dat")
+    write_fixed_coord(path, fx)
+    a = native.build_engine(hip_lib, sc)
+    s2 = {k: v for k, v in sc.items() if k != "fixed"}
+    s2["fixed_file"] = (path, len(fx["id"]))
+    b = native.build_engine(hip_lib, s2)
+    a.iterate(10)
+    b.iterate(10)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    assert record_set(a.get_contacts()) == record_set(b.get_contacts())
+    assert (a.get_contacts()["kind"] == 3).any()
+    bad = dict(fx, posr=fx["posr"].copy())
+    bad["posr"][3, 3] *= 1.01
+    write_fixed_coord(path, bad)
+    s2["fixed_file"] = (path, len(fx["id"]))
+    with pytest.raises(native.SdemError):
+        native.build_engine(hip_lib, s2)
+    write_fixed_coord(path, fx)
+    s2["fixed_file"] = (path, len(fx["id"]) + 1)
+    with pytest.raises(native.SdemError):
+        native.build_engine(hip_lib, s2)
