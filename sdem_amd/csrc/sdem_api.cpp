@@ -237,7 +237,6 @@ struct sdem_ctx {
   Dev D{};
   Scalars* S = nullptr;
   BoxState* BS = nullptr;
-  Slot* slots = nullptr;
   BoxSlot* bslots = nullptr;
   Params* d_params = nullptr;
   int* d_blk_box = nullptr;
@@ -478,7 +477,7 @@ static int upload_state(sdem_ctx* c) {
     if (int r = alloc_buf(c, c->buf[k])) return r;
   hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
   auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
-  M(&c->BS, nbox); M(&c->slots, NSLOT); M(&c->bslots, (size_t)nbox * NSLOT);
+  M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSLOT);
   M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
   M(&c->vcnt, cap); M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
   M(&c->ccnt, cap); M(&c->clist, cap * c->hcap);
@@ -559,7 +558,6 @@ static int upload_state(sdem_ctx* c) {
   s.iter = c->boxes[0].o.ifirst - 1;
   s.need_rebuild = 1;
   H(c->S, &s, sizeof(s));
-  if (e == hipSuccess) e = hipMemset(c->slots, 0, sizeof(Slot) * NSLOT);
   if (e == hipSuccess) e = hipMemset(c->bslots, 0, sizeof(BoxSlot) * NSLOT * nbox);
   if (e != hipSuccess) return hip_fail(c, e, "upload state");
   c->cur = 0;
@@ -584,7 +582,6 @@ static int upload_params(sdem_ctx* c) {
   std::vector<int> tlev;
   std::vector<size_t> o_pp(nbox), o_pw(nbox), o_m(nbox), o_w(nbox), o_t(nbox);
   int64_t key = 0;
-  double rebuild_disp = 1e300;
   Grid fgrid{};
   int nfix = 0;
   for (int bi = 0; bi < nbox; bi++) {
@@ -623,7 +620,7 @@ static int upload_params(sdem_ctx* c) {
     p.max_wall_vel = mwv;
     p.wall_max_update_iter = b.o.wall_max_update_iter;
     p.skin = c->skin_factor * maxd;
-    rebuild_disp = fmn(rebuild_disp, 0.5 * p.skin * (1.0 - 1.0e-6));
+    p.rebuild_disp = 0.5 * p.skin * (1.0 - 1.0e-6);
     for (auto& w : b.walls)
       if (w.wall_type > b.nwtype) return c->fail(SDEM_E_ARG, "wall_type exceeds numWall_type");
     // search levels: radii never change, so the per-level largest radius is
@@ -637,6 +634,7 @@ static int upload_params(sdem_ctx* c) {
       b.level_rmax[L] = fmx(b.level_rmax[L], b.posr[4 * (size_t)i + 3]);
     }
     p.nlvl = (int)b.cells.size();
+    p.key_first = (int)key;
     for (int L = 0; L < p.nlvl; L++) {
       if (int rc = make_grid(c, b, b.cells[L], b.level_rmax[L], key, &p.lvl[L])) return rc;
       key += p.lvl[L].ncell;
@@ -723,12 +721,11 @@ static int upload_params(sdem_ctx* c) {
   if (e == hipSuccess) e = hipMemset(c->counts, 0, ((size_t)key + 1) * sizeof(int));
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)key + 1);
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)scan_tiles(key + 1) + 1);
-  if (e == hipSuccess) e = put_field(&c->S->rebuild_disp, rebuild_disp);
   if (e == hipSuccess) e = put_field(&c->S->need_rebuild, 1);  // grids or skin may have changed
   if (e != hipSuccess) return hip_fail(c, e, "upload parameters");
   Dev& D = c->D;
   D.PB = c->d_params; D.blk_box = c->d_blk_box; D.nbox = nbox; D.nslots = c->nslots; D.nkeys = key;
-  D.S = c->S; D.BS = c->BS; D.slots = c->slots; D.bslots = c->bslots;
+  D.S = c->S; D.BS = c->BS; D.bslots = c->bslots;
   D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
   D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid; D.vcnt = c->vcnt;
   D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
@@ -1195,7 +1192,6 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     if (e == hipSuccess) e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
     // the slots hold what the failing and the halted steps folded in; the
     // replayed step rebuilds its lists, so only its own maxima are needed
-    if (e == hipSuccess) e = hipMemset(c->slots, 0, NSLOT * sizeof(Slot));
     if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSLOT * sizeof(BoxSlot));
     if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
     c->need_pre = true;
@@ -1625,13 +1621,11 @@ extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps) {
   s.iter += steps;
   s.need_rebuild = 1;
   s.have_parts = 0;
-  s.disp_acc = 0.0;
   B.next_wall_update = s.iter;
   B.n_stored = B.n_alive;
   for (int k = 0; k < 4; k++) B.num_cntcts[k] = 0;
   e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->BS, &B, sizeof(B), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(c->slots, 0, NSLOT * sizeof(Slot));
   if (e == hipSuccess) e = hipMemset(c->bslots, 0, NSLOT * sizeof(BoxSlot));
   if (e != hipSuccess) return hip_fail(c, e, "surrogate update");
   c->need_pre = true;

@@ -86,6 +86,8 @@ struct Params {
   int nfix;
   double dxmax1, dxmax2, max_wall_vel;
   double skin;                // candidate-list skin (search tuning)
+  double rebuild_disp;        // half the skin, with a margin: the box's lists are
+                              // rebuilt once its particles may have moved that far
   int lcap;                   // candidate-list entries per particle
   int wall_max_update_iter;
   const Bnry* pp;             // ntype*ntype, row = type of the record's lower-id particle
@@ -96,22 +98,20 @@ struct Params {
   // the box's place in the context
   int box;                    // index of the box
   int off, bcap;              // storage slots [off, off + bcap)
+  int key_first;              // first sort key of the box (its keys are
+                              // [key_first, dead_key])
   int dead_key;               // sort key of the box's removed / unused slots
 };
 
 // Partial results of the per-particle kernels, accumulated with atomics
-// spread over NSLOT addresses (block b uses slot b % NSLOT) so that no
-// single address takes thousands of atomics; reduced by the next step's
+// spread over NSLOT addresses per box (block b uses slot b % NSLOT) so that
+// no single address takes thousands of atomics; reduced by the next step's
 // control kernels and by k_finish.
 constexpr int NSLOT = 64;
-struct Slot {                    // whole context
-  unsigned long long step_bits;  // largest distance a particle moved in the step
-  int ndel;                      // particles removed
-  int pad;
-};
 struct BoxSlot {                 // one box (NSLOT of them per box)
   unsigned long long max_v_bits; // near-wall list: max |v|, max |a| (as bits of
   unsigned long long max_a_bits; // non-negative doubles)
+  unsigned long long step_bits;  // largest distance a particle moved in the step
   int cnt[4];                    // contacts by kind
   int ndel;                      // particles removed
   int pad;
@@ -133,6 +133,11 @@ struct BoxState {
   int use_host_max_a;      // first step after the state was set: max|a| from the host
   double host_max_a;
   int num_cntcts[4];       // numCntcts of the last step
+  int rebuild;             // this step re-sorts the box and rebuilds its lists
+  int pad;
+  double disp_acc;         // sum over steps since the box's last rebuild of the
+                           // largest move of one of its particles: bounds every
+                           // displacement since then
 };
 
 // device-side scalars of one context
@@ -142,16 +147,12 @@ struct Scalars {
   int overflow;            // 1: history slots, 2: candidate-list entries
   int halt;                // set by the step after an overflow: no more work
   int halted_steps;        // steps of the call skipped after an overflow
-  int rebuild;             // this step rebuilds the cell sort and the lists
-  int need_rebuild;        // forced rebuild at the next step
-  int rebuilds;            // rebuilds so far (statistics)
+  int rebuild;             // some box rebuilds this step (the sort kernels run)
+  int need_rebuild;        // every box rebuilds at the next step
+  int forced;              // ... this step (need_rebuild, or no partial results)
+  int rebuilds;            // box rebuilds so far (statistics)
   int have_parts;          // the force kernel's slots are current
-  unsigned spare;
   int list_need;           // largest candidate count seen when a list overflowed
-  int pad;
-  double rebuild_disp;     // smallest half skin over the boxes (with a margin)
-  double disp_acc;         // sum over steps since the last rebuild of the largest
-                           // move of a particle: bounds every displacement
 };
 
 // Device pointers and launch shapes the host hands to the launchers.
@@ -163,7 +164,6 @@ struct Dev {
   int64_t nkeys;         // sort keys of all boxes (cells + one last key each)
   Scalars* S;
   BoxState* BS;
-  Slot* slots;
   BoxSlot* bslots;       // NSLOT per box
   int *keys, *rank, *counts, *start, *tile_sum, *perm_tmp, *perm;
   double4* pk;           // positions in cell order (rebuild steps)

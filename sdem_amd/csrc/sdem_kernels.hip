@@ -309,22 +309,36 @@ __global__ __launch_bounds__(256) void k_pre(const Params* __restrict__ PB, cons
   }
 }
 
-// Per-box part of the step control, one wave: reduces the box's slots
-// (this step's max|v| and max|a|), clears them for the step, and runs the
-// near-wall cadence (Prtcl_ContactSearchPW.f90:201-229) at iterNumber `iter`.
-__device__ void box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl) {
+// Per-box part of the step control, one wave: reduces the box's slots (the
+// last step's largest move, deletions, max|v|, max|a|), clears them for the
+// step, decides whether the box is re-sorted and its candidate lists rebuilt
+// (forced, a particle was removed, or the summed largest moves since the
+// last rebuild may exceed half the skin: the exact contact test runs every
+// step, so the decision never changes results), and runs the near-wall
+// cadence (Prtcl_ContactSearchPW.f90:201-229) at iterNumber `iter`.
+// Returns the decision on lane 0.
+__device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl, bool forced) {
   const int lane = threadIdx.x;
   BoxSlot& m = sl[lane];
-  double mv = bitsd(m.max_v_bits), ma = bitsd(m.max_a_bits);
+  double mv = bitsd(m.max_v_bits), ma = bitsd(m.max_a_bits), step = bitsd(m.step_bits);
+  int ndel = m.ndel;
   for (int off = 32; off > 0; off >>= 1) {
     mv = fmax(mv, __shfl_down(mv, off, 64));
     ma = fmax(ma, __shfl_down(ma, off, 64));
+    step = fmax(step, __shfl_down(step, off, 64));
+    ndel += __shfl_down(ndel, off, 64);
   }
   m.max_v_bits = 0ull;
   m.max_a_bits = 0ull;
+  m.step_bits = 0ull;
   m.ndel = 0;
   for (int q = 0; q < 4; q++) m.cnt[q] = 0;
-  if (lane != 0) return;
+  if (lane != 0) return false;
+  // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
+  const double disp = B.disp_acc + step;
+  const bool rebuild = forced || ndel > 0 || !(disp <= P.rebuild_disp);
+  B.rebuild = rebuild ? 1 : 0;
+  B.disp_acc = rebuild ? 0.0 : disp;
   const int next = B.next_wall_update;
   B.n_stored = B.n_alive;
   B.next_wall_at_begin = next;
@@ -349,18 +363,17 @@ __device__ void box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl)
     B.next_wall_update = iter + num_iter;
   }
   B.use_host_max_a = 0;
+  return rebuild;
 }
 
-// Step control, one wave: reduces the context's slots (the previous step's
-// largest move and deletions), decides whether the cell sort and the
-// candidate lists are rebuilt (for all boxes at once: the results do not
-// depend on it) and advances iterNumber (Prtcl_DEMSystem.f90:248); with a
-// single box it also runs that box's control.  After a capacity overflow it
-// only marks the call halted, so every later kernel of the call does nothing
-// and the failing step's input buffer stays intact for the replay; every
-// halted step is counted so that the host knows how many steps completed.
-__global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, Slot* slots,
-                                            BoxState* BS, BoxSlot* bslots, int nbox) {
+// Step control, one wave: advances iterNumber (Prtcl_DEMSystem.f90:248) and
+// decides whether this step is forced to rebuild every box; with a single
+// box it also runs that box's control.  After a capacity overflow it only
+// marks the call halted, so every later kernel of the call does nothing and
+// the failing step's input buffer stays intact for the replay; every halted
+// step is counted so that the host knows how many steps completed.
+__global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
+                                            BoxSlot* bslots, int nbox) {
   const int lane = threadIdx.x;
   if (S->halt || S->overflow) {
     if (lane == 0) {
@@ -370,36 +383,35 @@ __global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scala
     return;
   }
   const int iter = S->iter;
-  Slot& sl = slots[lane];
-  double step = bitsd(sl.step_bits);
-  int ndel = sl.ndel;
-  for (int off = 32; off > 0; off >>= 1) {
-    step = fmax(step, __shfl_down(step, off, 64));
-    ndel += __shfl_down(ndel, off, 64);
-  }
-  sl.step_bits = 0ull;
-  sl.ndel = 0;
+  const bool forced = S->need_rebuild || !S->have_parts;
+  __syncthreads();  // every lane has read the flags before lane 0 clears them
   if (lane == 0) {
-    const bool have = S->have_parts != 0;
-    // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
-    const double disp = S->disp_acc + step;
-    S->rebuild = (S->need_rebuild || !have || ndel > 0 || !(disp <= S->rebuild_disp)) ? 1 : 0;
-    S->disp_acc = S->rebuild ? 0.0 : disp;
+    S->forced = forced ? 1 : 0;
     S->need_rebuild = 0;
     S->have_parts = 0;
+    S->rebuild = 0;  // the boxes' decisions are folded in below / by k_box
     S->iter_at_begin = iter;
-    if (S->rebuild) S->rebuilds++;
     S->iter = iter + 1;  // iterNumber advances once per step
   }
-  if (nbox == 1) box_control(PB[0], iter, BS[0], bslots);
+  if (nbox == 1) {
+    const bool r = box_control(PB[0], iter, BS[0], bslots, forced);
+    if (lane == 0 && r) {
+      S->rebuild = 1;
+      S->rebuilds++;
+    }
+  }
 }
 
 // Per-box step control of an ensemble, one wave per box, after k_ctl.
-__global__ __launch_bounds__(64) void k_box(const Params* __restrict__ PB, const Scalars* S, BoxState* BS,
+__global__ __launch_bounds__(64) void k_box(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
                                             BoxSlot* bslots) {
   if (S->halt) return;
   const int box = blockIdx.x;
-  box_control(PB[box], S->iter_at_begin, BS[box], bslots + (size_t)box * NSLOT);
+  const bool r = box_control(PB[box], S->iter_at_begin, BS[box], bslots + (size_t)box * NSLOT, S->forced != 0);
+  if (threadIdx.x == 0 && r) {
+    atomicOr(&S->rebuild, 1);
+    atomicAdd(&S->rebuilds, 1);
+  }
 }
 
 // counters of the last step for the host (once per call), one wave per box
@@ -423,16 +435,24 @@ __global__ __launch_bounds__(64) void k_finish(const Scalars* S, BoxState* BS, c
 // ---------------------------------------------------------------------------
 // cell sort (rebuild steps only)
 // ---------------------------------------------------------------------------
-// (level, cell) key and per-key counts; dead particles go to the last key.
-// Lanes of a wave holding the same key (neighbours in storage order usually
-// share a cell) take their slots with one atomic per run.  The counts array
-// is all zero on entry (zeroed at allocation and by k_scan_add).
+// (level, cell) key and per-key counts of the boxes that rebuild; dead
+// particles go to the box's last key.  Lanes of a wave holding the same key
+// (neighbours in storage order usually share a cell) take their slots with
+// one atomic per run.  The counts array is all zero on entry (zeroed at
+// allocation and by k_scan_add).  The scan then runs over every key; the
+// boxes that do not rebuild contribute no counts, so a box's offsets are
+// taken relative to its first key (box_start).
+__device__ __forceinline__ int box_start_base(const Params& P, const int* __restrict__ start) {
+  return start[P.key_first] - P.off;  // start[key] - base = storage slot of the key's first particle
+}
+
 __global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                              Buf A, const Scalars* S, const BoxState* BS,
                                              int* __restrict__ keys, int* __restrict__ rank,
                                              int* __restrict__ counts) {
   if (!S->rebuild || S->halt) return;
   const int box = block_box(blk_box);
+  if (!BS[box].rebuild) return;
   const Params& P = PB[box];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -519,13 +539,16 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ in, int* 
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const Scalars* S, const int* __restrict__ keys,
-                                                 const int* __restrict__ rank,
-                                                 const int* __restrict__ start,
-                                                 int* __restrict__ perm_tmp) {
+__global__ __launch_bounds__(256) void k_scatter(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                                 const Scalars* S, const BoxState* BS,
+                                                 const int* __restrict__ keys, const int* __restrict__ rank,
+                                                 const int* __restrict__ start, int* __restrict__ perm_tmp) {
   if (!S->rebuild || S->halt) return;
+  const int box = block_box(blk_box);
+  if (!BS[box].rebuild) return;
+  const int base = box_start_base(PB[box], start);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  perm_tmp[start[keys[s]] + rank[s]] = s;
+  perm_tmp[start[keys[s]] - base + rank[s]] = s;
 }
 
 // Make the in-cell order the storage order (deterministic) and pack the
@@ -541,16 +564,18 @@ __global__ __launch_bounds__(256) void k_rank(const Params* __restrict__ PB, con
                                               int* __restrict__ pid) {
   if (!S->rebuild || S->halt) return;
   const int box = block_box(blk_box);
+  if (!BS[box].rebuild) return;
   const Params& P = PB[box];
+  const int base = box_start_base(P, start);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == P.off) BS[box].n_alive = start[P.dead_key] - P.off;
+  if (k == P.off) BS[box].n_alive = start[P.dead_key] - base - P.off;
   const int s = perm_tmp[k];
   const int key = keys[s];
   if (key == P.dead_key) {
     perm[k] = s;
     return;
   }
-  const int a = start[key], b = start[key + 1];
+  const int a = start[key] - base, b = start[key + 1] - base;
   int r = 0;
   for (int j = a; j < b; j++) r += (perm_tmp[j] < s);
   perm[a + r] = s;
@@ -598,9 +623,11 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
                                                const int* __restrict__ fid, const int* __restrict__ fstart) {
   if (!S->rebuild || S->halt) return;
   const int box = block_box(blk_box);
+  if (!BS[box].rebuild) return;
   const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k - P.off >= BS[box].n_alive) return;
+  const int base = box_start_base(P, cstart);
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
@@ -641,7 +668,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
           if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
           if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
           const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
-          const int ja = cstart[rowkey + xa], jb = cstart[rowkey + xb + 1];
+          const int ja = cstart[rowkey + xa] - base, jb = cstart[rowkey + xb + 1] - base;
           for (int kj = ja; kj < jb; kj++) {
             const double4 pj = pk[kj];
             const double ddx = pi.x - (pj.x + shx);
@@ -795,7 +822,7 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     ccnt[k] = 0;
     return;
   }
-  const bool rebuilt = S->rebuild != 0;
+  const bool rebuilt = BS[box].rebuild != 0;
   const int s = rebuilt ? perm[k] : k;
   const double4 pi = A.pos[s];
   const size_t cap = P.cap;
@@ -836,14 +863,14 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
                                                const int* __restrict__ ftype,
-                                               Slot* __restrict__ slots, BoxSlot* __restrict__ bslots) {
+                                               BoxSlot* __restrict__ bslots) {
   // a candidate-list overflow in this step's rebuild leaves the lists
   // incomplete: do nothing, the host regrows them and replays the step
   if (S->halt || S->overflow == 2) return;
   const int box = block_box(blk_box);
   const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool rebuilt = S->rebuild != 0;  // this step re-sorted: A is read through perm
+  const bool rebuilt = BS[box].rebuild != 0;  // this step re-sorted the box: A is read through perm
   // k_ctl has advanced iterNumber: is the next step a near-wall update?
   const bool upd_next = S->iter >= BS[box].next_wall_update;
   Acc acc;
@@ -1083,10 +1110,8 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     ba = block_max(anext, shd);
   }
   if (threadIdx.x == 0) {
-    Slot& sl = slots[blockIdx.x % NSLOT];
     BoxSlot& bl = bslots[box * NSLOT + blockIdx.x % NSLOT];
-    if (bmove > 0.0) atomicMax(&sl.step_bits, dbits(sqrt(bmove)));
-    if (bdel) atomicAdd(&sl.ndel, bdel);
+    if (bmove > 0.0) atomicMax(&bl.step_bits, dbits(sqrt(bmove)));
     if (upd_next) {
       atomicMax(&bl.max_v_bits, dbits(bv));
       atomicMax(&bl.max_a_bits, dbits(ba));
@@ -1115,7 +1140,7 @@ hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st) {
 }
 
 hipError_t launch_ctl(const Dev& D, hipStream_t st) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.slots, D.BS, D.bslots, D.nbox);
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots, D.nbox);
   if (D.nbox > 1) hipLaunchKernelGGL(k_box, dim3(D.nbox), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots);
   return hipGetLastError();
 }
@@ -1134,7 +1159,8 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, st, D.tile_sum, (int)nt, D.S);
   hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
-  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.S, D.keys, D.rank, D.start, D.perm_tmp);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, D.S, D.BS, D.keys, D.rank, D.start,
+                     D.perm_tmp);
   hipLaunchKernelGGL(k_rank, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.start,
                      D.perm_tmp, D.perm, D.pk, D.pid);
   return hipGetLastError();
@@ -1151,7 +1177,7 @@ hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
                      D.blk_box, A, D.S, D.BS, D.perm, D.vcnt, D.vlist, D.clist, D.ccnt, D.fpk);
   if (mid) (void)hipEventRecord(mid, st);
   hipLaunchKernelGGL(k_force, dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A,
-                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.slots, D.bslots);
+                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
   return hipGetLastError();
 }
 
