@@ -27,6 +27,7 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_vlist(const Dev& D, hipStream_t st);
 hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid);
 int64_t scan_tiles(int64_t n);
+int scan_tile_keys();
 }  // namespace sdem
 
 using namespace sdem;
@@ -721,12 +722,31 @@ static int upload_params(sdem_ctx* c) {
   if (e == hipSuccess) e = hipMemset(c->counts, 0, ((size_t)key + 1) * sizeof(int));
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->start, (size_t)key + 1);
   if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &c->tile_sum, (size_t)scan_tiles(key + 1) + 1);
+  // boxes covered by each scan tile (keys are box-contiguous; the end
+  // sentinel belongs to the last box)
+  int2* d_tile_box = nullptr;
+  {
+    const int64_t nt = scan_tiles(key + 1), tk = scan_tile_keys();
+    std::vector<int2> tb(nt);
+    int bx = 0;
+    auto box_of = [&](int64_t k) {
+      while (bx + 1 < nbox && k > P[bx].dead_key) bx++;
+      return bx;
+    };
+    for (int64_t t = 0; t < nt; t++) {
+      tb[t].x = box_of(t * tk);
+      tb[t].y = box_of(std::min<int64_t>((t + 1) * tk - 1, key));
+    }
+    if (e == hipSuccess) e = dalloc(c, c->allocs_grid, &d_tile_box, tb.size());
+    if (e == hipSuccess) e = hipMemcpy(d_tile_box, tb.data(), tb.size() * sizeof(int2), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = put_field(&c->S->need_rebuild, 1);  // grids or skin may have changed
   if (e != hipSuccess) return hip_fail(c, e, "upload parameters");
   Dev& D = c->D;
   D.PB = c->d_params; D.blk_box = c->d_blk_box; D.nbox = nbox; D.nslots = c->nslots; D.nkeys = key;
   D.S = c->S; D.BS = c->BS; D.bslots = c->bslots;
   D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
+  D.tile_box = d_tile_box;
   D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid; D.vcnt = c->vcnt;
   D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;

@@ -166,6 +166,7 @@ struct Dev {
   BoxState* BS;
   BoxSlot* bslots;       // NSLOT per box
   int *keys, *rank, *counts, *start, *tile_sum, *perm_tmp, *perm;
+  const int2* tile_box;  // first and last box whose keys a scan tile covers
   double4* pk;           // positions in cell order (rebuild steps)
   int* pid;              // ids in cell order (rebuild steps)
   int *vcnt, *vlist, *vkey;

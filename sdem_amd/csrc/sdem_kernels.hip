@@ -483,20 +483,34 @@ __global__ __launch_bounds__(256) void k_bin(const Params* __restrict__ PB, cons
 
 // Exclusive scan of the counts in three kernels: tile scans, the scan of the
 // tile sums, and the tile offsets added (while the counts are zeroed for the
-// next rebuild).
+// next rebuild).  A tile whose keys all belong to boxes that do not rebuild
+// holds only zero counts and nobody reads its offsets: it is skipped (its
+// sum is zero).
 constexpr int SCAN_T = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_T * SCAN_ITEMS;
+
+__device__ __forceinline__ bool tile_active(const int2* __restrict__ tile_box, const BoxState* BS, int t) {
+  const int2 r = tile_box[t];
+  for (int b = r.x; b <= r.y; b++)
+    if (BS[b].rebuild) return true;
+  return false;
+}
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out,
                                                         int* __restrict__ tile_sum, int64_t n,
-                                                        const Scalars* S) {
+                                                        const Scalars* S, const BoxState* BS,
+                                                        const int2* __restrict__ tile_box) {
   if (!S->rebuild || S->halt) return;
-  using BS = hipcub::BlockScan<int, SCAN_T>;
-  __shared__ typename BS::TempStorage tmp;
+  if (!tile_active(tile_box, BS, blockIdx.x)) {
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = 0;
+    return;
+  }
+  using Scan = hipcub::BlockScan<int, SCAN_T>;
+  __shared__ typename Scan::TempStorage tmp;
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   for (int i = 0; i < SCAN_ITEMS; i++) v[i] = (base + i < n) ? in[base + i] : 0;
   int total;
-  BS(tmp).ExclusiveSum(v, v, total);
+  Scan(tmp).ExclusiveSum(v, v, total);
   for (int i = 0; i < SCAN_ITEMS; i++)
     if (base + i < n) out[base + i] = v[i];
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
@@ -528,8 +542,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_sums(int* __restrict__ tile_sum
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_add(int* __restrict__ in, int* __restrict__ out,
                                                       const int* __restrict__ tile_sum, int64_t n,
-                                                      const Scalars* S) {
+                                                      const Scalars* S, const BoxState* BS,
+                                                      const int2* __restrict__ tile_box) {
   if (!S->rebuild || S->halt) return;
+  if (!tile_active(tile_box, BS, blockIdx.x)) return;
   const int add = tile_sum[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_T)
@@ -1132,6 +1148,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+int scan_tile_keys() { return SCAN_TILE; }
 
 hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st) {
   hipLaunchKernelGGL(k_pre, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS,
@@ -1156,9 +1173,11 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
   const unsigned nb = nblk(D.nslots, 256);
   hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.rank,
                      D.counts);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S, D.BS,
+                     D.tile_box);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, st, D.tile_sum, (int)nt, D.S);
-  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S);
+  hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(SCAN_T), 0, st, D.counts, D.start, D.tile_sum, n, D.S, D.BS,
+                     D.tile_box);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, D.S, D.BS, D.keys, D.rank, D.start,
                      D.perm_tmp);
   hipLaunchKernelGGL(k_rank, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS, D.keys, D.start,
