@@ -52,6 +52,8 @@ struct sdem_gnn {
   size_t nparam = 0;
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   __bf16* wb = nullptr;
+  __bf16* wbt = nullptr;   // every weight matrix transposed ([out][in]), for the fused inference MLPs
+  bool wt_stale = true;    // wbt lags behind wb
   int64_t adam_t = 0;
   // graph
   int n = 0;
@@ -195,6 +197,37 @@ hipError_t mlp_bwd(sdem_gnn* M, const MLP& p, const Act& a, const float* dy, flo
 
 int L_of(const sdem_gnn* M) { return M->cfg.layers; }
 
+// Inference with 128 hidden units runs the message-passing MLPs as one
+// fused kernel each (same bits as the three GEMMs, no activations in HBM).
+bool fused_inference(const sdem_gnn* M, bool train) { return !train && M->cfg.hidden == FUSED_H; }
+
+// the transposed bf16 weights of every MLP, after the weights changed
+hipError_t refresh_transposes(sdem_gnn* M) {
+  if (!M->wt_stale) return hipSuccess;
+  hipError_t e = hipSuccess;
+  for (const MLP& p : M->mlp) {
+    if (e == hipSuccess) e = transpose_bf16(M->wb + p.w1, p.in, p.hid, M->wbt + p.w1, M->stream);
+    if (e == hipSuccess) e = transpose_bf16(M->wb + p.w2, p.hid, p.hid, M->wbt + p.w2, M->stream);
+    if (e == hipSuccess) e = transpose_bf16(M->wb + p.w3, p.hid, p.out, M->wbt + p.w3, M->stream);
+  }
+  if (e == hipSuccess) M->wt_stale = false;
+  return e;
+}
+
+FusedMLP fused_args(const sdem_gnn* M, const MLP& p, int R, const float* resid, float* y) {
+  FusedMLP f{};
+  f.R = R;
+  f.w1t = M->wbt + p.w1;
+  f.w2t = M->wbt + p.w2;
+  f.w3t = M->wbt + p.w3;
+  f.b1 = M->w + p.b1;
+  f.b2 = M->w + p.b2;
+  f.b3 = M->w + p.b3;
+  f.resid = resid;
+  f.y = y;
+  return f;
+}
+
 // Activation buffers for n nodes and ne edges.  Training keeps every
 // layer's activations for the backward pass; inference keeps one layer's
 // (residual streams ping-pong, the MLP buffers are shared by all layers).
@@ -212,12 +245,14 @@ int alloc_graph(sdem_gnn* M, int n, int64_t ne, bool train) {
   G(&M->esrc, E); G(&M->edst, E); G(&M->rstart, N + 1);
   G(&M->fn, N * PAD_IN); G(&M->fe, E * PAD_IN);
   M->act.assign(M->mlp.size(), Act{});
+  const bool fused = fused_inference(M, train);
   // MLP applications: 0 enc_node (N), 1 enc_edge (E), per layer edge (E), node (N), dec (N)
   for (size_t q = 0; q < M->mlp.size(); q++) {
     const MLP& p = M->mlp[q];
     const bool layer = q >= 2 && q < 2 + 2 * (size_t)L;
     const bool on_edges = q == 1 || (layer && (q - 2) % 2 == 0);
     Act& a = M->act[q];
+    if (fused && layer) continue;     // the fused kernel keeps its activations on chip
     if (!train && layer && q >= 4) {  // share the first layer's buffers
       a = M->act[q - 2];
       continue;
@@ -227,7 +262,7 @@ int alloc_graph(sdem_gnn* M, int n, int64_t ne, bool train) {
     G(&a.pre1, R * p.hid); G(&a.pre2, R * p.hid); G(&a.a1, R * p.hid); G(&a.a2, R * p.hid);
     G(&a.out, R * p.out);
   }
-  const int ns = train ? L + 1 : 2, nx = train ? L : 1;
+  const int ns = train ? L + 1 : 2, nx = train ? L : (fused ? 0 : 1);
   M->h.assign(ns, nullptr);
   M->e.assign(ns, nullptr);
   M->xe.assign(nx, nullptr);
@@ -268,13 +303,23 @@ int forward(sdem_gnn* M) {
   an.R = n; an.x = M->fn;
   Act& ae = M->act[1];
   ae.R = ne; ae.x = M->fe;
+  const bool fused = fused_inference(M, tr);
+  hipError_t e = fused ? refresh_transposes(M) : hipSuccess;
   // encoder and layer outputs go straight into the residual streams
-  hipError_t e = mlp_fwd(M, M->mlp[0], an, nullptr, Hs(0));
+  if (e == hipSuccess) e = mlp_fwd(M, M->mlp[0], an, nullptr, Hs(0));
   if (e == hipSuccess && ne > 0) e = mlp_fwd(M, M->mlp[1], ae, nullptr, Es(0));
   for (int l = 0; l < L && e == hipSuccess; l++) {
     Act& a_e = M->act[2 + 2 * l];
     Act& a_n = M->act[3 + 2 * l];
-    if (ne > 0) {
+    if (ne > 0 && fused) {
+      FusedMLP f = fused_args(M, M->mlp[2 + 2 * l], ne, Es(l), Es(l + 1));
+      f.seg[0] = {Es(l), nullptr};
+      f.seg[1] = {Hs(l), M->esrc};
+      f.seg[2] = {Hs(l), M->edst};
+      f.nseg = 3;
+      e = mlp_fused(f, st);
+      if (e == hipSuccess) e = segsum(Es(l + 1), M->rstart, n, H, M->agg, st);
+    } else if (ne > 0) {
       // edge update e' = e + MLP([e, h_src, h_dst])
       e = pack_cols(Es(l), nullptr, ne, H, XE(l), 3 * H, 0, st);
       if (e == hipSuccess) e = pack_cols(Hs(l), M->esrc, ne, H, XE(l), 3 * H, H, st);
@@ -286,6 +331,14 @@ int forward(sdem_gnn* M) {
       e = hipMemsetAsync(M->agg, 0, (size_t)n * H * 4, st);
     }
     // node update h' = h + MLP([h, agg])
+    if (fused) {
+      FusedMLP f = fused_args(M, M->mlp[3 + 2 * l], n, Hs(l), Hs(l + 1));
+      f.seg[0] = {Hs(l), nullptr};
+      f.seg[1] = {M->agg, nullptr};
+      f.nseg = 2;
+      if (e == hipSuccess) e = mlp_fused(f, st);
+      continue;
+    }
     if (e == hipSuccess) e = pack_cols(Hs(l), nullptr, n, H, XN(l), 2 * H, 0, st);
     if (e == hipSuccess) e = pack_cols(M->agg, nullptr, n, H, XN(l), 2 * H, H, st);
     a_n.R = n; a_n.x = XN(l);
@@ -475,7 +528,7 @@ int sdem_gnn_create(const sdem_gnn_config* cfg, uint64_t seed, sdem_gnn** out) {
   }
   hipError_t e = hipSuccess;
   auto G = [&](auto** p, size_t k) { if (e == hipSuccess) e = galloc(M->allocs, p, k); };
-  G(&M->w, off); G(&M->g, off); G(&M->m, off); G(&M->v, off); G(&M->wb, off); G(&M->loss, 1);
+  G(&M->w, off); G(&M->g, off); G(&M->m, off); G(&M->v, off); G(&M->wb, off); G(&M->wbt, off); G(&M->loss, 1);
   if (e == hipSuccess) e = hipMemcpy(M->w, w.data(), off * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(M->m, 0, off * 4);
   if (e == hipSuccess) e = hipMemset(M->v, 0, off * 4);
@@ -522,6 +575,7 @@ int sdem_gnn_set_params(sdem_gnn* M, const float* w) {
   hipError_t e = hipMemcpy(M->w, w, M->nparam * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = to_bf16(M->w, M->nparam, M->wb, M->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(M->stream);
+  M->wt_stale = true;
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "set_params");
 }
 
@@ -631,6 +685,7 @@ int sdem_gnn_train_step(sdem_gnn* M, const float* target, float lr, float clip, 
   const float bc1 = 1.0f - std::pow(b1, (float)M->adam_t), bc2 = 1.0f - std::pow(b2, (float)M->adam_t);
   e = adam(M->w, M->g, M->m, M->v, M->wb, M->nparam, lr, b1, b2, 1e-8f, bc1, bc2, cs, M->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(M->stream);
+  M->wt_stale = true;
   if (e != hipSuccess) return hipf(M, e, "adam");
   if (loss_out) *loss_out = loss;
   return SDEM_OK;

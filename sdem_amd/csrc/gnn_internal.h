@@ -21,6 +21,26 @@ struct Epi {
   __bf16* out_bf16;
 };
 
+// Fused inference MLP (hidden = out = FUSED_H): y = resid + MLP(x) with the
+// input row x[r] = [seg0 row, seg1 row, seg2 row] gathered from fp32
+// matrices of width FUSED_H (optional row index per segment), weights read
+// transposed ([out][in], bf16).
+constexpr int FUSED_H = 128;
+struct FusedSeg {
+  const float* x;
+  const int* idx;
+};
+struct FusedMLP {
+  FusedSeg seg[3];
+  int nseg, R;
+  const __bf16 *w1t, *w2t, *w3t;
+  const float *b1, *b2, *b3;
+  const float* resid;
+  float* y;
+};
+hipError_t mlp_fused(const FusedMLP& a, hipStream_t st);
+hipError_t transpose_bf16(const __bf16* w, int rows, int cols, __bf16* wt, hipStream_t st);
+
 // neighbour-search grid of the surrogate's graph (cells >= cut-off)
 struct GGrid {
   double lo[3], L[3], inv_w[3];

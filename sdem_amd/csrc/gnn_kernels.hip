@@ -138,6 +138,155 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, cons
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference MLP: y[r] = resid[r] + W3^T a2 + b3, a2 = SiLU(W2^T a1 +
+// b2), a1 = SiLU(W1^T x[r] + b1), hidden = out = 128, for a tile of 128
+// rows per block.  Layer 1 streams K in chunks of 32: the input rows
+// (gathered from their fp32 sources, rounded to bf16) and the matching
+// columns of W1^T are staged in LDS, double-buffered, with the next chunk's
+// global loads in flight during the current chunk's products.  a1 and a2
+// stay in LDS as bf16; W2^T and W3^T are staged in LDS in turn.  Four
+// waves each own a 64 x 64 quarter of the output (2 x 2 MFMA tiles).  The
+// products run in the same k order, with the same roundings, as k_gemm's
+// three launches on pack_cols inputs, so both paths give the same bits.
+// ---------------------------------------------------------------------------
+constexpr int FR = 128, FKC = 32, FXK = FKC + 8, FHK = FUSED_H + 8;
+
+struct FusedStage {
+  __bf16 X[FR][FXK];       // input rows, chunk of FKC columns
+  __bf16 W[FUSED_H][FXK];  // W1^T rows (output units), same columns
+};
+union FusedStaging {
+  FusedStage st[2];
+  __bf16 Wl[FUSED_H][FHK];  // W2^T or W3^T
+};
+
+// products of a 128 x 128 (K) tile: A rows from LDS (stride lda), B = W^T
+// rows from LDS (stride ldb), k from 0 to K in steps of 16
+__device__ __forceinline__ void fused_mma(const __bf16* As, int lda, const __bf16* Bs, int ldb, int K, int wm,
+                                          int wn, int lane, f32x16 acc[2][2]) {
+  for (int kk = 0; kk < K; kk += 16) {
+    const int kq = kk + 8 * (lane >> 5);
+    bf16x8 af[2], bfr[2];
+    for (int i = 0; i < 2; i++) af[i] = *(const bf16x8*)(As + (wm + i * 32 + (lane & 31)) * lda + kq);
+    for (int j = 0; j < 2; j++) bfr[j] = *(const bf16x8*)(Bs + (wn + j * 32 + (lane & 31)) * ldb + kq);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// a = bf16(SiLU(acc + bias)) into the LDS activation tile; acc cleared
+__device__ __forceinline__ void fused_act(f32x16 acc[2][2], const float* __restrict__ bias, __bf16 (*Hs)[FHK],
+                                          int wm, int wn, int lane) {
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      const int col = wn + j * 32 + (lane & 31);
+      const float b = bias[col];
+      for (int r = 0; r < 16; r++) {
+        const int row = wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        Hs[row][col] = (__bf16)silu(acc[i][j][r] + b);
+        acc[i][j][r] = 0.0f;
+      }
+    }
+}
+
+// a 128 x 128 bf16 weight matrix (row-major, stride 128) into LDS rows of FHK
+__device__ __forceinline__ void fused_load_w(const __bf16* __restrict__ wt, __bf16 (*Wl)[FHK], int t) {
+  for (int q = 0; q < (FUSED_H * FUSED_H / 8) / 256; q++) {
+    const int u = q * 256 + t;
+    const int n = u / (FUSED_H / 8), c = (u % (FUSED_H / 8)) * 8;
+    *(bf16x8*)&Wl[n][c] = *(const bf16x8*)(wt + (size_t)n * FUSED_H + c);
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_fused(FusedMLP a) {
+  __shared__ __attribute__((aligned(16))) FusedStaging U;
+  __shared__ __attribute__((aligned(16))) __bf16 Hs[FR][FHK];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int r0 = blockIdx.x * FR;
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int K1 = a.nseg * FUSED_H, nch = K1 / FKC;
+  f32x16 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+
+  // per thread and chunk: 4 float4 of input (rows u/8, columns (u%8)*4) and
+  // 2 x 8 bf16 of W1^T (rows u/4, columns (u%4)*8)
+  float4 xr[4];
+  bf16x8 wr[2];
+  auto load_chunk = [&](int c) {
+    const int k0 = c * FKC;
+    const FusedSeg sg = a.seg[k0 / FUSED_H];
+    const int c0 = k0 % FUSED_H;
+    for (int q = 0; q < 4; q++) {
+      const int u = q * 256 + t;
+      const int r = u >> 3, cc = (u & 7) * 4;
+      const int gr = r0 + r;
+      xr[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (gr < a.R) {
+        const int src = sg.idx ? sg.idx[gr] : gr;
+        xr[q] = *(const float4*)(sg.x + (size_t)src * FUSED_H + c0 + cc);
+      }
+    }
+    for (int q = 0; q < 2; q++) {
+      const int u = q * 256 + t;
+      const int n = u >> 2, cc = (u & 3) * 8;
+      wr[q] = *(const bf16x8*)(a.w1t + (size_t)n * K1 + k0 + cc);
+    }
+  };
+  load_chunk(0);
+  for (int c = 0; c < nch; c++) {
+    FusedStage& S = U.st[c & 1];
+    for (int q = 0; q < 4; q++) {
+      const int u = q * 256 + t;
+      const int r = u >> 3, cc = (u & 7) * 4;
+      S.X[r][cc] = (__bf16)xr[q].x;
+      S.X[r][cc + 1] = (__bf16)xr[q].y;
+      S.X[r][cc + 2] = (__bf16)xr[q].z;
+      S.X[r][cc + 3] = (__bf16)xr[q].w;
+    }
+    for (int q = 0; q < 2; q++) {
+      const int u = q * 256 + t;
+      *(bf16x8*)&S.W[u >> 2][(u & 3) * 8] = wr[q];
+    }
+    __syncthreads();
+    if (c + 1 < nch) load_chunk(c + 1);
+    fused_mma(&S.X[0][0], FXK, &S.W[0][0], FXK, FKC, wm, wn, lane, acc);
+  }
+  fused_act(acc, a.b1, Hs, wm, wn, lane);
+  __syncthreads();  // layer 1 done everywhere: the staging area is free, a1 complete
+  fused_load_w(a.w2t, U.Wl, t);
+  __syncthreads();
+  fused_mma(&Hs[0][0], FHK, &U.Wl[0][0], FHK, FUSED_H, wm, wn, lane, acc);
+  __syncthreads();  // a1 and W2^T read everywhere
+  fused_act(acc, a.b2, Hs, wm, wn, lane);
+  fused_load_w(a.w3t, U.Wl, t);
+  __syncthreads();
+  fused_mma(&Hs[0][0], FHK, &U.Wl[0][0], FHK, FUSED_H, wm, wn, lane, acc);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      const int col = wn + j * 32 + (lane & 31);
+      const float b = a.b3[col];
+      for (int r = 0; r < 16; r++) {
+        const int row = r0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row >= a.R) continue;
+        const size_t o = (size_t)row * FUSED_H + col;
+        float v = acc[i][j][r] + b;
+        if (a.resid) v += a.resid[o];
+        a.y[o] = v;
+      }
+    }
+}
+
+// wt[c][r] = w[r][c] (bf16, row-major)
+__global__ void k_transpose_bf16(const __bf16* __restrict__ w, int rows, int cols, __bf16* __restrict__ wt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)rows * cols) return;
+  const int r = (int)(i / cols), c = (int)(i % cols);
+  wt[(size_t)c * rows + r] = w[i];
+}
+
 // sum of the split-K partials: out[i] (+)= sum_z part[z*len + i]
 __global__ void k_reduce_parts(const float* __restrict__ part, int nz, size_t len, float* __restrict__ out,
                                int accumulate) {
@@ -454,6 +603,18 @@ hipError_t gemm(bool ta, bool tb, const __bf16* A, const __bf16* B, int M, int N
 int gemm_splits(int K, int kchunk_target) {
   int s = (K + kchunk_target - 1) / kchunk_target;
   return s < 1 ? 1 : s;
+}
+
+hipError_t mlp_fused(const FusedMLP& a, hipStream_t st) {
+  if (a.R <= 0) return hipSuccess;
+  if (a.nseg < 1 || a.nseg > 3) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mlp_fused, dim3(nb(a.R, FR)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t transpose_bf16(const __bf16* w, int rows, int cols, __bf16* wt, hipStream_t st) {
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(nb((size_t)rows * cols, 256)), dim3(256), 0, st, w, rows, cols, wt);
+  return hipGetLastError();
 }
 
 hipError_t reduce_parts(const float* part, int nz, size_t len, float* out, bool accumulate, hipStream_t st) {

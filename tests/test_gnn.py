@@ -221,3 +221,24 @@ def test_surrogate_advances_a_dem_context(hip_lib):
     eng.iterate(20)
     assert eng.get_iter()[0] == it0 + 50
     assert np.all(np.isfinite(eng.get_state()["posr"]))
+
+
+def test_fused_inference_matches_training_path(hip_lib):
+    """Inference runs each message-passing MLP as one fused kernel (input
+    gather, three matrix-core products, activations on chip); it gives the
+    bits of the training path's separate GEMMs on the same graph."""
+    print("surrogate: fused inference vs the three-GEMM path", flush=True)
+    from sdem_amd import surrogate
+    sc = scenarios.settling(n=20000, seed=8)
+    eng = native.build_engine(hip_lib, sc)
+    eng.iterate(5)
+    m = surrogate.make_model(seed=4, lib=hip_lib)  # 8 layers, 128 hidden units
+    rng = np.random.default_rng(2)
+    m.set_params(m.params() * (1.0 + 0.5 * rng.standard_normal(m.num_params())).astype(np.float32))
+    m.graph_from_dem(eng, for_training=True)
+    ids_t, out_t = m.node_ids(), m.forward()
+    m.graph_from_dem(eng, for_training=False)
+    ids_i, out_i = m.node_ids(), m.forward()
+    assert np.array_equal(ids_t, ids_i)
+    assert np.abs(out_t).max() > 0
+    assert np.array_equal(out_t, out_i)
