@@ -21,7 +21,7 @@
 
 namespace sdem {
 hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st);
-hipError_t launch_ctl(const Dev& D, hipStream_t st);
+hipError_t launch_ctl(const Dev& D, hipStream_t st, bool fast);
 hipError_t launch_finish(const Dev& D, hipStream_t st);
 hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_vlist(const Dev& D, hipStream_t st);
@@ -250,7 +250,13 @@ struct sdem_ctx {
   int* ccnt = nullptr;
   int64_t nkeys = 0;
   bool use_graph = true;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps, by starting buffer
+  // graphs of one and of two steps, without / with the speculative fast
+  // path (single box: no sort and list kernels), by starting buffer
+  hipGraphExec_t gexec[2][2][2] = {};  // [fast][nsteps - 1][buffer]
+  bool spec_ready = false;  // the next step may try the fast path
+  int spec_chunk = 8;       // steps queued per fast batch (adapted)
+  int spec_full = 1;        // steps queued per full batch between fast ones: grows
+                            // while fast batches fail at their first step
 
   bool prof = false;
   double prof_ms[SDEM_NUM_PHASES] = {};
@@ -272,11 +278,14 @@ int hip_fail(sdem_ctx* c, hipError_t e, const char* where) {
 }
 
 void invalidate_graphs(sdem_ctx* c) {
-  for (auto& g : c->gexec)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto& a : c->gexec)
+    for (auto& b : a)
+      for (auto& g : b)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
+  c->spec_ready = false;
 }
 
 template <class T>
@@ -1093,16 +1102,18 @@ static int grow_vlist(sdem_ctx* c, int list_need) {
 }
 
 // Queue one step on the context's stream; ev (SDEM_NUM_PHASES+1 events) may
-// be null.  The buffers swap roles afterwards.
-static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
+// be null.  The buffers swap roles afterwards.  A fast step (single box)
+// leaves out the sort and list kernels; its control kernel halts the call
+// if the box must rebuild.
+static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev, bool fast = false) {
   Buf& A = c->buf[c->cur];
   Buf& B = c->buf[1 - c->cur];
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipError_t e = launch_ctl(c->D, c->stream);
+  hipError_t e = launch_ctl(c->D, c->stream, fast);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
-  if (e == hipSuccess) e = launch_sort(c->D, A, c->stream);
+  if (e == hipSuccess && !fast) e = launch_sort(c->D, A, c->stream);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
-  if (e == hipSuccess) e = launch_vlist(c->D, c->stream);
+  if (e == hipSuccess && !fast) e = launch_vlist(c->D, c->stream);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
   if (e == hipSuccess) e = launch_force(c->D, A, B, c->stream, ev ? ev[4] : nullptr);
   if (ev) (void)hipEventRecord(ev[5], c->stream);
@@ -1110,37 +1121,51 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev) {
   return e;
 }
 
-// A graph of two consecutive steps starting from buffer c->cur (the kernels
-// read their control flags from device memory, so the graph is fixed).
-static hipError_t pair_graph(sdem_ctx* c, hipGraphExec_t* out) {
-  if (c->gexec[c->cur]) {
-    *out = c->gexec[c->cur];
+// A graph of `n` (1 or 2) consecutive steps starting from buffer c->cur
+// (the kernels read their control flags from device memory, so the graph
+// is fixed); c->cur is left as it was.
+static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out) {
+  hipGraphExec_t& slot = c->gexec[fast ? 1 : 0][n - 1][c->cur];
+  if (slot) {
+    *out = slot;
     return hipSuccess;
   }
+  const int cur0 = c->cur;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  hipError_t e1 = enqueue_step(c, nullptr);
-  if (e1 == hipSuccess) e1 = enqueue_step(c, nullptr);
+  hipError_t e1 = hipSuccess;
+  for (int k = 0; k < n && e1 == hipSuccess; k++) e1 = enqueue_step(c, nullptr, fast);
   e = hipStreamEndCapture(c->stream, &g);
+  c->cur = cur0;
   if (e1 != hipSuccess) e = e1;
-  if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec[c->cur], g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
-  if (e == hipSuccess) *out = c->gexec[c->cur];
+  if (e == hipSuccess) *out = slot;
   return e;
 }
 
+// Steps in batches.  With graphs and a single box, steps are queued on the
+// fast path (no sort and list kernels) in batches of spec_chunk, growing
+// while they run clean; when a step finds that its box must rebuild, it
+// halts the batch before any side effect, the host replays it on the full
+// path and speculation resumes after it.  Results do not depend on the
+// path: the same criterion decides every rebuild.  A capacity overflow
+// halts a batch too: the capacity grows and the failing step is replayed.
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
   if (!c) return SDEM_E_ARG;
   if (nsteps < 0) return c->fail(SDEM_E_ARG, "nsteps < 0");
   if (int rc = prepare(c)) return rc;
   const int nbox = (int)c->boxes.size();
   const int NE = SDEM_NUM_PHASES + 1;
+  const bool spec_mode = !c->prof && c->use_graph && nbox == 1;
   int remaining = nsteps, retries = 0;
   while (remaining > 0) {
     if (int rc = prepare(c)) return rc;
+    const bool fast = spec_mode && c->spec_ready;
+    const int batch = !spec_mode ? remaining : std::min(remaining, fast ? c->spec_chunk : c->spec_full);
     std::vector<hipEvent_t> evs;
-    if (c->prof) evs.resize((size_t)remaining * NE);
+    if (c->prof) evs.resize((size_t)batch * NE);
     for (auto& x : evs)
       if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
     const int c0 = c->cur;
@@ -1152,13 +1177,20 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     int it = 0;
     if (!c->prof && c->use_graph) {
-      for (; it + 2 <= remaining && e == hipSuccess; it += 2) {
+      for (; it + 2 <= batch && e == hipSuccess; it += 2) {
         hipGraphExec_t g = nullptr;
-        e = pair_graph(c, &g);
+        e = step_graph(c, 2, fast, &g);
         if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
       }
+      if (it < batch && e == hipSuccess) {
+        hipGraphExec_t g = nullptr;
+        e = step_graph(c, 1, fast, &g);
+        if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
+        c->cur = 1 - c->cur;
+        it++;
+      }
     }
-    for (; it < remaining && e == hipSuccess; it++)
+    for (; it < batch && e == hipSuccess; it++)
       e = enqueue_step(c, c->prof ? &evs[(size_t)it * NE] : nullptr);
     if (e == hipSuccess) e = launch_finish(c->D, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1172,7 +1204,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       for (auto& x : evs) (void)hipEventDestroy(x);
       return hip_fail(c, e, "hipMemcpy(scalars)");
     }
-    const int ran = remaining - s.halted_steps;  // steps that did work, the last one failed if overflow
+    const int ran = batch - s.halted_steps;  // steps that did work, the last one failed if overflow
     const int ok_steps = s.overflow ? ran - 1 : ran;
     for (int k = 0; k < ok_steps && c->prof; k++)
       for (int q = 0; q < SDEM_NUM_PHASES; q++) {
@@ -1187,20 +1219,38 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       c->stepped = true;
       c->first_step_pending = false;
     }
-    if (!s.overflow) break;
+    if (!s.overflow && !s.spec_fail) {
+      remaining -= batch;
+      if (fast) c->spec_chunk = std::min(2 * c->spec_chunk, 1024);
+      c->spec_ready = spec_mode;
+      continue;
+    }
+    c->cur = (c0 + ok_steps) % 2;
+    remaining -= ok_steps;
+    s.halt = 0;
+    s.halted_steps = 0;
+    if (s.spec_fail) {
+      // the step after the clean ones must rebuild: replay it on the full
+      // path (nothing of it ran), then speculate again in batches of about
+      // the run just seen
+      s.spec_fail = 0;
+      e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
+      c->spec_ready = false;
+      c->spec_chunk = std::max(4, ok_steps);
+      // boxes that rebuild every step gain nothing from speculation: back off
+      c->spec_full = ok_steps == 0 ? std::min(2 * c->spec_full + 1, 256) : 1;
+      continue;
+    }
     // a capacity ran out in step `ok_steps` of this batch: its input buffer
     // is intact; grow the capacity and replay from there
     if (++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");
-    c->cur = (c0 + ok_steps) % 2;
     const int kind = s.overflow, list_need = s.list_need;
     s.iter = s.iter_at_begin;
     s.need_rebuild = 1;
     s.have_parts = 0;
-
     s.list_need = 0;
     s.overflow = 0;
-    s.halt = 0;
-    s.halted_steps = 0;
     std::vector<BoxState> bs;
     e = dl(bs, c->BS, (size_t)nbox);
     for (auto& B : bs) {
@@ -1215,9 +1265,9 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSLOT * sizeof(BoxSlot));
     if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
     c->need_pre = true;
+    c->spec_ready = false;
     int rc = kind == 2 ? grow_vlist(c, list_need) : grow_history(c);
     if (rc) return rc;
-    remaining -= ok_steps;
   }
   return SDEM_OK;
 }
@@ -1649,6 +1699,7 @@ extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps) {
   if (e == hipSuccess) e = hipMemset(c->bslots, 0, NSLOT * sizeof(BoxSlot));
   if (e != hipSuccess) return hip_fail(c, e, "surrogate update");
   c->need_pre = true;
+  c->spec_ready = false;  // the next step rebuilds
   c->stepped = true;
   c->first_step_pending = false;
   return SDEM_OK;

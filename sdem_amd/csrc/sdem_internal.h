@@ -153,6 +153,8 @@ struct Scalars {
   int rebuilds;            // box rebuilds so far (statistics)
   int have_parts;          // the force kernel's slots are current
   int list_need;           // largest candidate count seen when a list overflowed
+  int spec_fail;           // a step queued without the rebuild kernels needed them:
+                           // it halted the call before doing anything
 };
 
 // Device pointers and launch shapes the host hands to the launchers.

@@ -372,8 +372,11 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
 // marks the call halted, so every later kernel of the call does nothing and
 // the failing step's input buffer stays intact for the replay; every halted
 // step is counted so that the host knows how many steps completed.
+// `fast` (single box): the step was queued without the sort and list
+// kernels; if the box must rebuild, the step halts the call before any
+// side effect (spec_fail), and the host replays it with them.
 __global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
-                                            BoxSlot* bslots, int nbox) {
+                                            BoxSlot* bslots, int nbox, int fast) {
   const int lane = threadIdx.x;
   if (S->halt || S->overflow) {
     if (lane == 0) {
@@ -384,6 +387,26 @@ __global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scala
   }
   const int iter = S->iter;
   const bool forced = S->need_rebuild || !S->have_parts;
+  if (fast) {
+    // box_control's rebuild decision, read-only
+    const BoxSlot& m = bslots[lane];
+    double step = bitsd(m.step_bits);
+    int ndel = m.ndel;
+    for (int off = 32; off > 0; off >>= 1) {
+      step = fmax(step, __shfl_down(step, off, 64));
+      ndel += __shfl_down(ndel, off, 64);
+    }
+    int need = 0;
+    if (lane == 0) need = (forced || ndel > 0 || !(BS[0].disp_acc + step <= PB[0].rebuild_disp)) ? 1 : 0;
+    if (__shfl(need, 0, 64)) {
+      if (lane == 0) {
+        S->halt = 1;
+        S->spec_fail = 1;
+        S->halted_steps++;
+      }
+      return;
+    }
+  }
   __syncthreads();  // every lane has read the flags before lane 0 clears them
   if (lane == 0) {
     S->forced = forced ? 1 : 0;
@@ -1156,8 +1179,8 @@ hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_ctl(const Dev& D, hipStream_t st) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots, D.nbox);
+hipError_t launch_ctl(const Dev& D, hipStream_t st, bool fast) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots, D.nbox, (fast && D.nbox == 1) ? 1 : 0);
   if (D.nbox > 1) hipLaunchKernelGGL(k_box, dim3(D.nbox), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots);
   return hipGetLastError();
 }

@@ -104,6 +104,31 @@ def test_two_runs_are_bitwise_identical(hip_lib):
     assert out[0] == out[1]
 
 
+@pytest.mark.parametrize("skin", [0.0, 0.02, 0.3])
+def test_speculative_steps_match_full_steps(hip_lib, skin):
+    """A single-box context queues its steps without the sort and list
+    kernels and replays a step on the full path when its box must rebuild
+    (skin 0: every step; 0.02: often; 0.3: seldom).  The bits are those of
+    steps queued with every kernel (profiling on), over calls of several
+    lengths."""
+    print(f"speculative steps, skin {skin}", flush=True)
+    sc = scenarios.random_cloud(n=600, seed=7, periodic=(1, 1, 0), walls="z", vel=2.0, dt=1.0e-5)
+    a = native.build_engine(hip_lib, sc, tuning=dict(skin=skin))
+    b = native.build_engine(hip_lib, sc, tuning=dict(skin=skin))
+    b.set_profiling(True)
+    for k in (1, 7, 30, 2, 64):
+        a.iterate(k)
+        b.iterate(k)
+        sa, sb = a.get_state(), b.get_state()
+        for key in sa:
+            assert np.array_equal(sa[key], sb[key]), (k, key)
+        ca, cb = a.get_contacts(), b.get_contacts()
+        for key in ca:
+            assert np.array_equal(ca[key], cb[key]), (k, key)
+        assert a.get_iter() == b.get_iter()
+        assert np.array_equal(a.get_counts(), b.get_counts())
+
+
 def test_tuning_does_not_change_bits(hip_lib):
     """Contact forces are summed in ascending partner id, so the cell shape,
     the candidate-list skin and the block sizes leave every bit of the result
