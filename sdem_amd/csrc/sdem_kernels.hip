@@ -863,14 +863,41 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   }
   const bool rebuilt = BS[box].rebuild != 0;
   const int s = rebuilt ? perm[k] : k;
-  const double4 pi = A.pos[s];
   const size_t cap = P.cap;
+  // the first CPF candidates: every list entry, then every partner position
+  // in flight at once (one latency each instead of one chain per candidate)
+  constexpr int CPF = 4;
+  unsigned ent[CPF];
+  double4 pjs[CPF];
+  for (int q = 0; q < CPF; q++)
+    if (q < nq) ent[q] = (unsigned)vlist[(size_t)q * cap + k];
+  for (int q = 0; q < CPF; q++)
+    if (q < nq) {
+      const unsigned e = ent[q];
+      const bool fixed = (e & CL_FIXED) != 0;
+      const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+      pjs[q] = fixed ? fpk[j] : A.pos[j];
+      ent[q] = fixed ? e : (unsigned)j;
+    }
+  const double4 pi = A.pos[s];
   int c = 0;
   for (int q = 0; q < nq; q++) {
-    const unsigned e = (unsigned)vlist[(size_t)q * cap + k];
-    const bool fixed = (e & CL_FIXED) != 0;
-    const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
-    const double4 pj = fixed ? fpk[j] : A.pos[j];
+    unsigned e;
+    double4 pj;
+    bool fixed;
+    int j;
+    if (q < CPF) {
+      e = ent[q];
+      fixed = (e & CL_FIXED) != 0;
+      j = (int)(e & CL_INDEX);
+      pj = pjs[q];
+    } else {
+      e = (unsigned)vlist[(size_t)q * cap + k];
+      fixed = (e & CL_FIXED) != 0;
+      j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+      pj = fixed ? fpk[j] : A.pos[j];
+      if (fixed) j = (int)(e & CL_INDEX);
+    }
     int shifted = 0;
     const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
     const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
