@@ -1,0 +1,93 @@
+"""Edge cases of the step against the oracle: an empty box, a single sphere,
+particles leaving through open sides until the box is empty, and particle
+ids at the top of their range (needs the GPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from parity import compare
+from sdem_amd import native, scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(sc, hip_lib, oracle_lib):
+    return native.build_engine(hip_lib, sc), native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+
+
+def subset(sc, keep):
+    """The scenario with only the particles `keep` (indices)."""
+    s2 = dict(sc)
+    s2["state"] = {k: np.asarray(v)[keep].copy() for k, v in sc["state"].items()}
+    return s2
+
+
+def test_empty_box(hip_lib, oracle_lib):
+    print("empty box: 10 steps", flush=True)
+    sc = subset(scenarios.random_cloud(n=50, seed=3), np.zeros(0, int))
+    gpu, ref = pair(sc, hip_lib, oracle_lib)
+    for k in (1, 9):
+        gpu.iterate(k)
+        ref.iterate(k)
+        assert gpu.num_particles() == 0 == ref.num_particles()
+        assert gpu.get_iter() == ref.get_iter()
+        assert gpu.get_counts().tolist() == [0, 0, 0, 0]
+        assert len(gpu.get_contacts()["kind"]) == 0
+
+
+def test_single_sphere_on_a_wall(hip_lib, oracle_lib):
+    """One sphere falls onto the bottom wall and bounces (PW history only)."""
+    print("single sphere: 400 steps", flush=True)
+    sc = subset(scenarios.random_cloud(n=50, seed=5, vel=0.0, omega=0.0), np.array([0]))
+    sc["options"] = dict(sc["options"], gravity=(0.0, 0.0, -9.81))
+    sc["state"]["posr"][0, 2] = sc["state"]["posr"][0, 3] * 1.02
+    gpu, ref = pair(sc, hip_lib, oracle_lib)
+    seen_contact = False
+    for _ in range(8):
+        gpu.iterate(50)
+        ref.iterate(50)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, errs
+        seen_contact |= gpu.get_counts()[3] > 0
+    assert seen_contact
+
+
+def test_particles_leave_through_open_sides(hip_lib, oracle_lib):
+    """Only a bottom wall: fast particles leave through the open sides and
+    top and are removed, down to an empty box; the survivors and the step
+    counter stay the oracle's throughout."""
+    print("open sides: deletions down to an empty box", flush=True)
+    sc = scenarios.random_cloud(n=400, seed=9, walls="z", vel=0.5, dt=1.0e-5)
+    st = sc["state"]
+    rng = np.random.default_rng(1)
+    st["lin_vel"][:, 0, 2] += rng.uniform(20.0, 60.0, len(st["id"]))  # upwards, out of the top
+    st["lin_vel"][:, 0, 1] += rng.uniform(-30.0, 30.0, len(st["id"]))
+    st["lin_vel"][:, 0, 0] += rng.uniform(-30.0, 30.0, len(st["id"]))
+    gpu, ref = pair(sc, hip_lib, oracle_lib)
+    counts = []
+    for k in range(12):
+        gpu.iterate(10)
+        ref.iterate(10)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, (k, errs)
+        counts.append(gpu.num_particles())
+    assert counts[0] < 400 and counts[-1] == 0, counts
+    gpu.iterate(5)
+    ref.iterate(5)
+    assert gpu.num_particles() == 0 and gpu.get_iter() == ref.get_iter()
+
+
+def test_largest_ids(hip_lib, oracle_lib):
+    """Particle ids up to 2^29 - 1, the largest the contact keys hold."""
+    print("ids near 2^29", flush=True)
+    sc = scenarios.random_cloud(n=300, seed=12, periodic=(1, 1, 0), walls="z")
+    top = (1 << 29) - 1
+    sc["state"]["id"] = (top - np.asarray(sc["state"]["id"]) + 1).astype(np.int32)
+    sc["options"] = dict(sc["options"], base_wall_id=1000)
+    gpu, ref = pair(sc, hip_lib, oracle_lib)
+    for _ in range(3):
+        gpu.iterate(10)
+        ref.iterate(10)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, errs
+    assert gpu.get_contacts()["owner"].max() > top - 400
