@@ -92,14 +92,18 @@ def train(model: gnn.Model, n: int = 50_000, seeds=(101, 102, 103), warm: int = 
 
 
 def bed_metrics(state: Dict, area: float, slab: float = 0.02) -> Dict[str, float]:
-    """Bed height 2 <z> (the height of a uniform column with the same
-    centroid) and the solid fraction of the bottom slab [0, slab]."""
+    """Ensemble statistics of a bed (one density): height 2 <z> (the height
+    of a uniform column with the same centroid), the solid fraction of the
+    bottom slab [0, slab], and the translational kinetic energy per unit
+    mass 0.5 <|v|^2> (mass-weighted)."""
     z = state["posr"][:, 2]
     r = state["posr"][:, 3]
     vol = 4.0 / 3.0 * np.pi * r ** 3
     low = z < slab
+    v2 = (np.asarray(state["lin_vel"])[:, 0, :] ** 2).sum(1)
     return dict(height=float(2.0 * np.average(z, weights=vol)),
-                phi_bottom=float(vol[low].sum() / (area * slab)))
+                phi_bottom=float(vol[low].sum() / (area * slab)),
+                ke=float(0.5 * np.average(v2, weights=vol)))
 
 
 def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: int = 200, steps: int = 2000,
@@ -149,7 +153,7 @@ def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: in
     s_bal, out["ballistic_s"], _ = run(zero)
     zero.close()
     mr, ms, mb = bed_metrics(s_ref, area), bed_metrics(s_sur, area), bed_metrics(s_bal, area)
-    for k in ("height", "phi_bottom"):
+    for k in ("height", "phi_bottom", "ke"):
         out[k + "_dem"] = mr[k]
         out[k + "_err_surrogate"] = abs(ms[k] - mr[k]) / mr[k]
         out[k + "_err_ballistic"] = abs(mb[k] - mr[k]) / mr[k]
