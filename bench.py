@@ -42,8 +42,8 @@ SKIN = 0.05
 # HBM bytes per k_force launch on the default 1M bed, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by what the same
 # counters read for a plain 512 MiB copy (x2.00 and x1.00):
-# 112,487 KiB x 2 read + 218,994 KiB written (profiles/k_force_settling_1M.md)
-KFORCE_TRAFFIC_1M = (112_487 * 2 + 218_994) * 1024
+# 112,682 KiB x 2 read + 218,994 KiB written (profiles/k_force_settling_1M.md)
+KFORCE_TRAFFIC_1M = (112_682 * 2 + 218_994) * 1024
 
 
 def parse():
@@ -56,10 +56,12 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick ~15 s of CPU work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("settling", "ensemble", "surrogate"), default="settling",
-                    help="settling: the 1M-sphere bed (the headline line); ensemble: 256 independent "
-                         "sand boxes stepped together; surrogate: the learned surrogate trained on "
-                         "DEM trajectories, then against the DEM on the 1M-sphere bed")
+    ap.add_argument("--workload", choices=("settling", "sandbox", "polydisperse", "ensemble", "surrogate"),
+                    default="settling",
+                    help="settling: the 1M-sphere bed (the headline line); sandbox: the 10k-sphere sand "
+                         "box; polydisperse: 4M spheres of three sizes on an incline; ensemble: 256 "
+                         "independent sand boxes stepped together; surrogate: the learned surrogate "
+                         "trained on DEM trajectories, then against the DEM on the 1M-sphere bed")
     ap.add_argument("--train-snapshots", type=int, default=240)
     ap.add_argument("--boxes", type=int, default=256)
     return ap.parse_args()
@@ -187,6 +189,20 @@ def surrogate_bench(args):
     print(json.dumps(out), flush=True)
 
 
+WORKLOADS = {
+    "settling": ("particle-steps/s at 1M polydisperse spheres",
+                 "settling bed: 1M log-normal spheres, periodic x/y 0.2x0.2 m, bottom wall, "
+                 "Hertz-Mindlin with history, dt=5e-6 s, AB2"),
+    "sandbox": ("particle-steps/s, 10k-sphere sand box",
+                "sand box: 10k spheres d U[2,3] mm (11 types), 6 plane walls 0.05x0.05x0.2 m, linear "
+                "kn=1e5 N/m, kt=0.8 kn, e=0.9, mu=0.3, jittered lattice, dt=1e-5 s, AB2"),
+    "polydisperse": ("particle-steps/s, 4M spheres of three size classes",
+                     "wide polydispersity: 4M spheres of 1/2.5/5 mm (0.85/0.12/0.03), closed 1:1:2 box with a "
+                     "30-degree plane, Hertz-Mindlin mu=0.5, random pack phi=0.2, hierarchical search, "
+                     "dt=2e-6 s, AB2"),
+}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -202,16 +218,24 @@ def main():
     from sdem_amd import native, scenarios
 
     t0 = time.perf_counter()
-    sc = scenarios.settling(n=args.n, seed=args.seed)
+    if args.workload == "sandbox":
+        n = 10_000
+        sc = scenarios.sandbox(n=n, seed=args.seed)
+    elif args.workload == "polydisperse":
+        n = 4_000_000
+        sc = scenarios.polydisperse(n=n, seed=args.seed)
+    else:
+        n = args.n
+        sc = scenarios.settling(n=n, seed=args.seed)
     t_gen = time.perf_counter() - t0
-    print(f"# generated {args.n} spheres in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    print(f"# generated {n} spheres in {t_gen:.1f}s", file=sys.stderr, flush=True)
     lib = native.load_hip_library()
     eng = native.build_engine(lib, sc, tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN, keep_forces=0))
     eng.iterate(args.warmup)
     print(f"# warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
-    # Timed region: steps queued as HIP graphs (two steps per graph);
-    # Engine.iterate returns after synchronising the library's stream, so the
-    # region is bracketed by device synchronisation on both sides.
+    # Timed region: steps queued as HIP graphs; Engine.iterate returns after
+    # synchronising the library's stream, so the region is bracketed by
+    # device synchronisation on both sides.
     t0 = time.perf_counter()
     done, counts_sum, nsamp = 0, np.zeros(4, np.int64), 0
     chunk = max(2, (args.steps // 10) // 2 * 2)
@@ -224,22 +248,24 @@ def main():
         print(f"# {done}/{args.steps} steps", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
-    value = args.n * args.steps / wall
+    value = n * args.steps / wall
     P = float(counts_sum.sum()) / max(1, nsamp)  # contact records per step (sampled)
     # Kernel durations: HIP events around every kernel on the library's
     # stream, over the same number of further steps queued one by one.
     eng.set_profiling(True)
-    eng.iterate(args.steps)
+    eng.iterate(min(args.steps, 1000))
     prof = eng.get_profile()
     eng.set_profiling(False)
     n_now = eng.num_particles()
     k_ms, k_n = prof["force"]
     k_avg_s = k_ms / max(1, k_n) / 1e3
-    kernel_bytes = BYTES_PER_PARTICLE_KERNEL * args.n + BYTES_PER_CONTACT * P
+    kernel_bytes = BYTES_PER_PARTICLE_KERNEL * n + BYTES_PER_CONTACT * P
     achieved = kernel_bytes / k_avg_s / 1e9
-    step_bytes = BYTES_PER_PARTICLE_STEP * args.n + BYTES_PER_CONTACT * P
+    step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * P
+    metric, workload = WORKLOADS[args.workload]
+    headline = args.workload == "settling" and n == 1_000_000 and args.seed == 20261203
     out = {
-        "metric": "particle-steps/s at 1M polydisperse spheres",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -250,21 +276,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random loose pack, phi=0.3)",
-        "config": {"workload": "settling bed: 1M log-normal spheres, periodic x/y 0.2x0.2 m, bottom wall, "
-                               "Hertz-Mindlin with history, dt=5e-6 s, AB2",
-                   "n": args.n, "seed": args.seed, "contact_records_per_step": P,
+        "data": "synthetic (seeded initial packing)",
+        "config": {"workload": workload, "n": n, "seed": args.seed, "contact_records_per_step": P,
                    "particles_at_end": n_now},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": KFORCE_TRAFFIC_1M if (args.n == 1_000_000 and args.seed == 20261203) else None,
+                     "traffic": KFORCE_TRAFFIC_1M if headline else None,
                      "kernel": "k_force", "kernel_ms_avg": k_avg_s * 1e3,
                      "algorithmic_bytes_per_launch": kernel_bytes},
         "step_breakdown_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},
         "whole_step_hbm_frac": step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
     }
     eng.close()
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "settling":
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -288,8 +288,11 @@ def sandbox(n=10_000, seed=DEFAULT_SEED, box=(0.05, 0.05, 0.2), ntypes=11, jitte
 def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
     """Three size classes 1 / 2.5 / 5 mm (number fractions 0.85/0.12/0.03) in a
     closed box with a 30-degree plane; Hertz-Mindlin, mu=0.5, dt=2e-6 s,
-    hierarchical search.  The box keeps the 1:1:2 aspect of 0.3x0.3x0.6 m and
-    is sized so the pack fits at fraction `phi` in its lower 80 %."""
+    hierarchical search.  The box keeps the 1:1:2 aspect of 0.3x0.3x0.6 m,
+    sized so the pack fits at fraction `phi` in its lower 80 %, and is
+    raised by the incline's height: the pack starts above the incline (the
+    plane rises along +x to half the box width times tan 30 degrees) and
+    collapses onto it."""
     rng = np.random.default_rng(seed)
     centers = np.array([1.0e-3, 2.5e-3, 5.0e-3])
     t = rng.choice(3, size=n, p=[0.85, 0.12, 0.03])
@@ -297,18 +300,20 @@ def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
     vol_solid = float(np.sum(np.pi / 6 * dq ** 3))
     if box is None:
         s = (vol_solid / phi / 0.8 / 2.0) ** (1 / 3)
-        box = (s, s, 2 * s)
+        zr0 = s * math.tan(math.radians(30.0)) * 0.5
+        box = (s, s, 2 * s + zr0)
     box = np.asarray(box, float)
+    L = box[0]
+    zr = L * math.tan(math.radians(30.0)) * 0.5
     height = vol_solid / phi / (box[0] * box[1])
-    pos = rsa_pack(dq / 2, (0, 0, 0.0), (box[0], box[1], min(height, box[2] * 0.999)), seed)
+    z0 = zr + centers.max()
+    pos = rsa_pack(dq / 2, (0, 0, z0), (box[0], box[1], min(z0 + height, box[2] * 0.999)), seed)
     order = np.lexsort((pos[:, 0], pos[:, 1], pos[:, 2]))
     pos, t, dq = pos[order], t[order], dq[order]
     ids = np.arange(1, n + 1, dtype=np.int32)
     st = zero_state(ids, t + 1, np.concatenate([pos, dq[:, None] / 2], 1))
     walls = box_walls((0, 0, 0), box)
     # 30-degree incline rising along +x from the bottom, facing up
-    L = box[0]
-    zr = L * math.tan(math.radians(30.0)) * 0.5
     walls.append(dict(p1=(0, 0, 0), p2=(L, 0, zr), p3=(L, box[1], zr), p4=(0, box[1], 0), user_id=7,
                       wall_type=1, both_side=False, is_infinite=False, trans_vel=(0.0, 0.0, 0.0)))
     opts = base_options(dt=2.0e-6, domain_max=tuple(box), num_prtcl=n, cf_type=CFT_NLIN,
