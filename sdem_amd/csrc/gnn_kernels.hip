@@ -344,12 +344,33 @@ __global__ void k_pack_cols(const float* __restrict__ x, const int* __restrict__
 // receiver: rstart[v] .. rstart[v+1])
 __global__ void k_segsum(const float* __restrict__ e, const int* __restrict__ rstart, int nv, int C,
                          float* __restrict__ agg) {
+  // four columns per thread; edges added in ascending order (the loads of
+  // four edges issued together)
+  const int q = C >> 2;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nv * C) return;
-  const int v = (int)(i / C), c = (int)(i % C);
-  float s = 0.0f;
-  for (int k = rstart[v]; k < rstart[v + 1]; k++) s += e[(size_t)k * C + c];
-  agg[i] = s;
+  if (i >= (size_t)nv * q) return;
+  const int v = (int)(i / q), c = (int)(i % q) * 4;
+  const int k0 = rstart[v], k1 = rstart[v + 1];
+  float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    float4 x[4];
+    for (int u = 0; u < 4; u++) x[u] = *(const float4*)(e + (size_t)(k + u) * C + c);
+    for (int u = 0; u < 4; u++) {
+      s.x += x[u].x;
+      s.y += x[u].y;
+      s.z += x[u].z;
+      s.w += x[u].w;
+    }
+  }
+  for (; k < k1; k++) {
+    const float4 x = *(const float4*)(e + (size_t)k * C + c);
+    s.x += x.x;
+    s.y += x.y;
+    s.z += x.z;
+    s.w += x.w;
+  }
+  *(float4*)(agg + (size_t)v * C + c) = s;
 }
 
 // out[idx[r]] += x[r] (rows of C columns taken from column c0 of x, ldx wide)
@@ -644,7 +665,8 @@ hipError_t pack_cols(const float* x, const int* idx, int R, int C, __bf16* out, 
 }
 
 hipError_t segsum(const float* e, const int* rstart, int nv, int C, float* agg, hipStream_t st) {
-  hipLaunchKernelGGL(k_segsum, dim3(nb((size_t)nv * C, 256)), dim3(256), 0, st, e, rstart, nv, C, agg);
+  if (C % 4 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_segsum, dim3(nb((size_t)nv * (C / 4), 256)), dim3(256), 0, st, e, rstart, nv, C, agg);
   return hipGetLastError();
 }
 
