@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, cons
 // products run in the same k order, with the same roundings, as k_gemm's
 // three launches on pack_cols inputs, so both paths give the same bits.
 // ---------------------------------------------------------------------------
-constexpr int FR = 128, FKC = 32, FXK = FKC + 8, FHK = FUSED_H + 8;
+constexpr int FR = 128, FNT = 512, FKC = 32, FXK = FKC + 8, FHK = FUSED_H + 8;
 
 struct FusedStage {
   __bf16 X[FR][FXK];       // input rows, chunk of FKC columns
@@ -161,66 +161,63 @@ union FusedStaging {
   __bf16 Wl[FUSED_H][FHK];  // W2^T or W3^T
 };
 
-// products of a 128 x 128 (K) tile: A rows from LDS (stride lda), B = W^T
-// rows from LDS (stride ldb), k from 0 to K in steps of 16
+// products of a 64 x 32 slice (rows wm..wm+63, columns wn..wn+31) over K:
+// A rows from LDS (stride lda), B = W^T rows from LDS (stride ldb), k from 0
+// to K in steps of 16
 __device__ __forceinline__ void fused_mma(const __bf16* As, int lda, const __bf16* Bs, int ldb, int K, int wm,
-                                          int wn, int lane, f32x16 acc[2][2]) {
+                                          int wn, int lane, f32x16 acc[2]) {
   for (int kk = 0; kk < K; kk += 16) {
     const int kq = kk + 8 * (lane >> 5);
-    bf16x8 af[2], bfr[2];
+    bf16x8 af[2];
     for (int i = 0; i < 2; i++) af[i] = *(const bf16x8*)(As + (wm + i * 32 + (lane & 31)) * lda + kq);
-    for (int j = 0; j < 2; j++) bfr[j] = *(const bf16x8*)(Bs + (wn + j * 32 + (lane & 31)) * ldb + kq);
-    for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    const bf16x8 bfr = *(const bf16x8*)(Bs + (wn + (lane & 31)) * ldb + kq);
+    for (int i = 0; i < 2; i++) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr, acc[i], 0, 0, 0);
   }
 }
 
 // a = bf16(SiLU(acc + bias)) into the LDS activation tile; acc cleared
-__device__ __forceinline__ void fused_act(f32x16 acc[2][2], const float* __restrict__ bias, __bf16 (*Hs)[FHK],
+__device__ __forceinline__ void fused_act(f32x16 acc[2], const float* __restrict__ bias, __bf16 (*Hs)[FHK],
                                           int wm, int wn, int lane) {
+  const int col = wn + (lane & 31);
+  const float b = bias[col];
   for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) {
-      const int col = wn + j * 32 + (lane & 31);
-      const float b = bias[col];
-      for (int r = 0; r < 16; r++) {
-        const int row = wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        Hs[row][col] = (__bf16)silu(acc[i][j][r] + b);
-        acc[i][j][r] = 0.0f;
-      }
+    for (int r = 0; r < 16; r++) {
+      const int row = wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      Hs[row][col] = (__bf16)silu(acc[i][r] + b);
+      acc[i][r] = 0.0f;
     }
 }
 
 // a 128 x 128 bf16 weight matrix (row-major, stride 128) into LDS rows of FHK
 __device__ __forceinline__ void fused_load_w(const __bf16* __restrict__ wt, __bf16 (*Wl)[FHK], int t) {
-  for (int q = 0; q < (FUSED_H * FUSED_H / 8) / 256; q++) {
-    const int u = q * 256 + t;
+  for (int q = 0; q < (FUSED_H * FUSED_H / 8) / FNT; q++) {
+    const int u = q * FNT + t;
     const int n = u / (FUSED_H / 8), c = (u % (FUSED_H / 8)) * 8;
     *(bf16x8*)&Wl[n][c] = *(const bf16x8*)(wt + (size_t)n * FUSED_H + c);
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_mlp_fused(FusedMLP a) {
+__global__ __launch_bounds__(FNT, 2) void k_mlp_fused(FusedMLP a) {
   __shared__ __attribute__((aligned(16))) FusedStaging U;
   __shared__ __attribute__((aligned(16))) __bf16 Hs[FR][FHK];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int r0 = blockIdx.x * FR;
-  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int wm = (w >> 2) * 64, wn = (w & 3) * 32;  // eight waves: 64 rows x 32 output columns each
   const int K1 = a.nseg * FUSED_H, nch = K1 / FKC;
-  f32x16 acc[2][2];
+  f32x16 acc[2];
   for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++)
-      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+    for (int r = 0; r < 16; r++) acc[i][r] = 0.0f;
 
-  // per thread and chunk: 4 float4 of input (rows u/8, columns (u%8)*4) and
-  // 2 x 8 bf16 of W1^T (rows u/4, columns (u%4)*8)
-  float4 xr[4];
-  bf16x8 wr[2];
+  // per thread and chunk: 2 float4 of input (rows u/8, columns (u%8)*4) and
+  // 8 bf16 of W1^T (row u/4, columns (u%4)*8)
+  float4 xr[2];
+  bf16x8 wr;
   auto load_chunk = [&](int c) {
     const int k0 = c * FKC;
     const FusedSeg sg = a.seg[k0 / FUSED_H];
     const int c0 = k0 % FUSED_H;
-    for (int q = 0; q < 4; q++) {
-      const int u = q * 256 + t;
+    for (int q = 0; q < 2; q++) {
+      const int u = q * FNT + t;
       const int r = u >> 3, cc = (u & 7) * 4;
       const int gr = r0 + r;
       xr[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -229,27 +226,20 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(FusedMLP a) {
         xr[q] = *(const float4*)(sg.x + (size_t)src * FUSED_H + c0 + cc);
       }
     }
-    for (int q = 0; q < 2; q++) {
-      const int u = q * 256 + t;
-      const int n = u >> 2, cc = (u & 3) * 8;
-      wr[q] = *(const bf16x8*)(a.w1t + (size_t)n * K1 + k0 + cc);
-    }
+    wr = *(const bf16x8*)(a.w1t + (size_t)(t >> 2) * K1 + k0 + (t & 3) * 8);
   };
   load_chunk(0);
   for (int c = 0; c < nch; c++) {
     FusedStage& S = U.st[c & 1];
-    for (int q = 0; q < 4; q++) {
-      const int u = q * 256 + t;
+    for (int q = 0; q < 2; q++) {
+      const int u = q * FNT + t;
       const int r = u >> 3, cc = (u & 7) * 4;
       S.X[r][cc] = (__bf16)xr[q].x;
       S.X[r][cc + 1] = (__bf16)xr[q].y;
       S.X[r][cc + 2] = (__bf16)xr[q].z;
       S.X[r][cc + 3] = (__bf16)xr[q].w;
     }
-    for (int q = 0; q < 2; q++) {
-      const int u = q * 256 + t;
-      *(bf16x8*)&S.W[u >> 2][(u & 3) * 8] = wr[q];
-    }
+    *(bf16x8*)&S.W[t >> 2][(t & 3) * 8] = wr;
     __syncthreads();
     if (c + 1 < nch) load_chunk(c + 1);
     fused_mma(&S.X[0][0], FXK, &S.W[0][0], FXK, FKC, wm, wn, lane, acc);
@@ -264,18 +254,16 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(FusedMLP a) {
   fused_load_w(a.w3t, U.Wl, t);
   __syncthreads();
   fused_mma(&Hs[0][0], FHK, &U.Wl[0][0], FHK, FUSED_H, wm, wn, lane, acc);
+  const int col = wn + (lane & 31);
+  const float b = a.b3[col];
   for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) {
-      const int col = wn + j * 32 + (lane & 31);
-      const float b = a.b3[col];
-      for (int r = 0; r < 16; r++) {
-        const int row = r0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (row >= a.R) continue;
-        const size_t o = (size_t)row * FUSED_H + col;
-        float v = acc[i][j][r] + b;
-        if (a.resid) v += a.resid[o];
-        a.y[o] = v;
-      }
+    for (int r = 0; r < 16; r++) {
+      const int row = r0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      if (row >= a.R) continue;
+      const size_t o = (size_t)row * FUSED_H + col;
+      float v = acc[i][r] + b;
+      if (a.resid) v += a.resid[o];
+      a.y[o] = v;
     }
 }
 
@@ -629,7 +617,7 @@ int gemm_splits(int K, int kchunk_target) {
 hipError_t mlp_fused(const FusedMLP& a, hipStream_t st) {
   if (a.R <= 0) return hipSuccess;
   if (a.nseg < 1 || a.nseg > 3) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_mlp_fused, dim3(nb(a.R, FR)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_mlp_fused, dim3(nb(a.R, FR)), dim3(FNT), 0, st, a);
   return hipGetLastError();
 }
 
