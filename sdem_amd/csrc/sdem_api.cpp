@@ -483,8 +483,10 @@ static int upload_state(sdem_ctx* c) {
   c->hcap = c->hist_slots_hint > 0 ? c->hist_slots_hint : hc;
   c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
   const int nbox = (int)c->boxes.size();
-  for (int k = 0; k < 2; k++)
+  for (int k = 0; k < 2; k++) {
     if (int r = alloc_buf(c, c->buf[k])) return r;
+    c->buf[k].idx = k;
+  }
   hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
   auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
   M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSLOT);
@@ -567,6 +569,7 @@ static int upload_state(sdem_ctx* c) {
   std::memset(&s, 0, sizeof(s));
   s.iter = c->boxes[0].o.ifirst - 1;
   s.need_rebuild = 1;
+  s.rv2_stored[0] = s.rv2_stored[1] = 1;
   H(c->S, &s, sizeof(s));
   if (e == hipSuccess) e = hipMemset(c->bslots, 0, sizeof(BoxSlot) * NSLOT * nbox);
   if (e != hipSuccess) return hip_fail(c, e, "upload state");
@@ -760,6 +763,7 @@ static int upload_params(sdem_ctx* c) {
   D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
   D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
+  D.pim = c->boxes[0].o.pi_method; D.prim = c->boxes[0].o.pri_method;
   c->params_ready = true;
   return SDEM_OK;
 }
@@ -1315,6 +1319,34 @@ int sdem_get_num_particles(sdem_ctx* c, int32_t* n) {
   return SDEM_OK;
 }
 
+// With AB2 rotation the force kernel leaves rotVel(2) out of the buffers
+// it writes (it equals their linVel(1)); *stored tells whether the current
+// buffer's rv array holds it.
+static hipError_t rv2_stored(sdem_ctx* c, bool* stored) {
+  *stored = true;
+  if (c->rsize != 2) return hipSuccess;
+  int v = 1;
+  const char* at = (const char*)c->S + offsetof(Scalars, rv2_stored) + sizeof(int) * c->cur;
+  hipError_t e = hipMemcpy(&v, at, sizeof(int), hipMemcpyDeviceToHost);
+  *stored = v != 0;
+  return e;
+}
+
+// Write rotVel(2) into the current buffer's rv array, so that code outside
+// the step kernels may change linVel(1).
+static hipError_t store_rv2(sdem_ctx* c) {
+  bool st = true;
+  hipError_t e = rv2_stored(c, &st);
+  if (e != hipSuccess || st) return e;
+  const Buf& A = c->buf[c->cur];
+  const size_t cap = (size_t)c->nslots;
+  e = hipMemcpy(A.rv + 3 * cap, A.lv, 3 * cap * sizeof(double), hipMemcpyDeviceToDevice);
+  const int one = 1;
+  char* at = (char*)c->S + offsetof(Scalars, rv2_stored) + sizeof(int) * c->cur;
+  if (e == hipSuccess) e = hipMemcpy(at, &one, sizeof(int), hipMemcpyHostToDevice);
+  return e;
+}
+
 int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_t* ptype,
                    int32_t* usr_mark, double* posr, double* lin_vel, double* lin_acc, double* theta,
                    double* rot_vel, double* rot_acc) {
@@ -1359,6 +1391,8 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
   if (e == hipSuccess) e = dl(ra, A.ra, cap * 3 * c->tr);
   if (e == hipSuccess) e = dl(ty, A.type, cap);
   if (e == hipSuccess) e = dl(mk, A.mark, cap);
+  bool rv2 = true;
+  if (e == hipSuccess) e = rv2_stored(c, &rv2);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
   for (int q = 0; q < n; q++) {
     const size_t k = (size_t)idx[q].second;
@@ -1375,7 +1409,9 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
     if (theta) for (int d = 0; d < 3; d++) theta[3 * q + d] = th[(size_t)d * cap + k];
     for (int l = 0; l < rs; l++)
       for (int d = 0; d < 3; d++) {
-        if (rot_vel) rot_vel[((size_t)q * rs + l) * 3 + d] = rv[((size_t)l * 3 + d) * cap + k];
+        if (rot_vel)
+          rot_vel[((size_t)q * rs + l) * 3 + d] =
+              (l == 1 && !rv2) ? lv[(size_t)d * cap + k] : rv[((size_t)l * 3 + d) * cap + k];
         if (rot_acc) rot_acc[((size_t)q * rs + l) * 3 + d] = ra[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * cap + k];
       }
   }
@@ -1649,6 +1685,7 @@ extern "C" int sdem_internal_view(sdem_ctx* c, sdem_dem_view* v) {
   if (c->boxes.size() != 1) return c->fail(SDEM_E_STATE, "the surrogate drives single-box contexts");
   if (int rc = prepare(c)) return rc;
   hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = store_rv2(c);  // the caller may change linVel(1)
   BoxState B;
   if (e == hipSuccess) e = hipMemcpy(&B, c->BS, sizeof(B), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(c, e, "surrogate view");

@@ -64,6 +64,7 @@ struct Buf {
   int *hkey;        // hcap x cap partner keys, record c of particle s at c*cap+s
   double4 *hval;    // hcap x cap (TanDelta.x,.y,.z, VelRel_Init), same layout
   double *force;    // 6 x cap (F, T) when forces are kept
+  int idx;          // which of the context's two buffers this is
 };
 
 // everything the kernels read that does not change inside a step
@@ -155,6 +156,10 @@ struct Scalars {
   int list_need;           // largest candidate count seen when a list overflowed
   int spec_fail;           // a step queued without the rebuild kernels needed them:
                            // it halted the call before doing anything
+  int rv2_stored[2];       // with AB2 rotation, rotVel(2) sits in buffer b's rv
+                           // array; 0 once the force kernel wrote the buffer:
+                           // rotVel(2) is then linVel(1) of the same buffer
+                           // (Prtcl_Integration.f90:64 stores one as the other)
 };
 
 // Device pointers and launch shapes the host hands to the launchers.
@@ -177,6 +182,7 @@ struct Dev {
   const double4* fpk;
   const int *fid, *ftype, *fstart;
   int bs_contact, bs_force;  // block sizes (divisors of BOX_ALIGN)
+  int pim, prim;             // integration methods (the same in every box)
 };
 
 }  // namespace sdem

@@ -920,6 +920,11 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   if (c > P.hcap) atomicMax(&S->overflow, 1);
 }
 
+// PIM / PRIM: translational / rotational integration method fixed at
+// compile time (0: read from Params).  With fixed methods the integration
+// inputs are loaded before the contact loop, so that their latency overlaps
+// it; otherwise after the loop, where the registers are free.
+template <int PIM, int PRIM>
 __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
                                                const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
                                                BoxState* BS,
@@ -957,6 +962,34 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     const int ti = A.type[s];
     const d3 vi = ld3(A.lv, cap, s);
     const d3 wi = ld3(A.rv, cap, s);
+    const int pim = PIM ? PIM : P.pi_method, prim = PRIM ? PRIM : P.pri_method;
+    const d3 z3 = D3(0.0, 0.0, 0.0);
+    d3 v2 = z3, v3 = z3, a2 = z3, a3 = z3, th = z3, w2 = z3, w3 = z3, al2 = z3, al3 = z3;
+    int mark = 0;
+    // rotVel(2) of a state this kernel wrote is its linVel(1) (below)
+    const bool rv2 = S->rv2_stored[A.idx] != 0;
+    if (PIM) {
+      if (pim >= 2) {
+        v2 = ld3(A.lv + 3 * cap, cap, s);
+        a2 = ld3(A.la, cap, s);
+      }
+      if (pim == 3) {
+        v3 = ld3(A.lv + 6 * cap, cap, s);
+        a3 = ld3(A.la + 3 * cap, cap, s);
+      }
+      th = ld3(A.th, cap, s);
+      if (prim == 2) {
+        if (rv2) w2 = ld3(A.rv + 3 * cap, cap, s);
+        else w2 = vi;
+        al2 = ld3(A.ra, cap, s);
+      } else if (prim == 3) {
+        w2 = ld3(A.rv + 3 * cap, cap, s);
+        w3 = ld3(A.rv + 6 * cap, cap, s);
+        al2 = ld3(A.ra, cap, s);
+        al3 = ld3(A.ra + 3 * cap, cap, s);
+      }
+      mark = A.mark[s];
+    }
 
     // ---------------- every contact, one call site ------------------------
     // The contacts k_contact found, in ascending partner id, then the walls
@@ -967,7 +1000,8 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     // _PPG :761-798, _PPFix :803-827, _PW :832-856), so the routine is
     // inlined once.
     const int nc = min(ccnt[k], P.hcap);
-    uint64_t mask = A.wmask[s];
+    const uint64_t mask0 = A.wmask[s];
+    uint64_t mask = mask0;
     const int ntot = nc + __popcll(mask);
     for (int q = 0; q < ntot; q++) {
       d3 nvec, va, vb, wa, wb;
@@ -1069,16 +1103,18 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     const double dt = P.dt;
     d3 pos = xi;
     d3 lin1;  // linVel(1) after the translational update
-    if (P.pi_method == 1) {
+    if (pim == 1) {
       d3 sft = mul3(vi, dt);
       pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
       lin1 = add3(vi, mul3(a1, dt));
       st3(B.lv, cap, k, lin1);
       st3(B.la, cap, k, a1);
-    } else if (P.pi_method == 2) {
+    } else if (pim == 2) {
       const double c1 = 3.0 / 2.0, c2 = -1.0 / 2.0;
-      d3 v2 = ld3(A.lv + 3 * cap, cap, s);
-      d3 a2 = ld3(A.la, cap, s);
+      if (!PIM) {
+        v2 = ld3(A.lv + 3 * cap, cap, s);
+        a2 = ld3(A.la, cap, s);
+      }
       d3 sft = mul3(add3(mul3(vi, c1), mul3(v2, c2)), dt);
       pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
       lin1 = add3(vi, mul3(add3(mul3(a1, c1), mul3(a2, c2)), dt));
@@ -1087,10 +1123,12 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       st3(B.la, cap, k, a1);  // linAcc(2) <- linAcc(1)
     } else {
       const double c1 = 23.0 / 12.0, c2 = -16.0 / 12.0, c3 = 5.0 / 12.0;
-      d3 v2 = ld3(A.lv + 3 * cap, cap, s);
-      d3 v3 = ld3(A.lv + 6 * cap, cap, s);
-      d3 a2 = ld3(A.la, cap, s);
-      d3 a3 = ld3(A.la + 3 * cap, cap, s);
+      if (!PIM) {
+        v2 = ld3(A.lv + 3 * cap, cap, s);
+        v3 = ld3(A.lv + 6 * cap, cap, s);
+        a2 = ld3(A.la, cap, s);
+        a3 = ld3(A.la + 3 * cap, cap, s);
+      }
       d3 sft = mul3(add3(add3(mul3(vi, c1), mul3(v2, c2)), mul3(v3, c3)), dt);
       pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
       lin1 = add3(vi, mul3(add3(add3(mul3(a1, c1), mul3(a2, c2)), mul3(a3, c3)), dt));
@@ -1100,27 +1138,34 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       st3(B.la, cap, k, a1);
       st3(B.la + 3 * cap, cap, k, a2);
     }
-    d3 th = ld3(A.th, cap, s);
-    if (P.pri_method == 1) {
+    if (!PIM) th = ld3(A.th, cap, s);
+    if (prim == 1) {
       th = add3(th, mul3(wi, dt));
       st3(B.rv, cap, k, add3(wi, mul3(al1, dt)));
       st3(B.ra, cap, k, al1);
-    } else if (P.pri_method == 2) {
-      // Prtcl_Integration.f90:64: rotVel1 = GPrtcl_linVel(1,i)
+    } else if (prim == 2) {
+      // Prtcl_Integration.f90:64: rotVel1 = GPrtcl_linVel(1,i), and
+      // rotVel(2) <- rotVel1 (:69), so rotVel(2) of the new state is its
+      // linVel(1): it is not stored, and read back as linVel(1) (vi) from a
+      // buffer this kernel wrote
       const double c1 = 3.0 / 2.0, c2 = -1.0 / 2.0;
       d3 w1 = lin1;
-      d3 w2 = ld3(A.rv + 3 * cap, cap, s);
-      d3 al2 = ld3(A.ra, cap, s);
+      if (!PIM) {
+        if (rv2) w2 = ld3(A.rv + 3 * cap, cap, s);
+        else w2 = vi;
+        al2 = ld3(A.ra, cap, s);
+      }
       th = add3(th, mul3(add3(mul3(w1, c1), mul3(w2, c2)), dt));
       st3(B.rv, cap, k, add3(w1, mul3(add3(mul3(al1, c1), mul3(al2, c2)), dt)));
-      st3(B.rv + 3 * cap, cap, k, w1);
       st3(B.ra, cap, k, al1);
     } else {
       const double c1 = 23.0 / 12.0, c2 = -16.0 / 12.0, c3 = 5.0 / 12.0;
-      d3 w2 = ld3(A.rv + 3 * cap, cap, s);
-      d3 w3 = ld3(A.rv + 6 * cap, cap, s);
-      d3 al2 = ld3(A.ra, cap, s);
-      d3 al3 = ld3(A.ra + 3 * cap, cap, s);
+      if (!PIM) {
+        w2 = ld3(A.rv + 3 * cap, cap, s);
+        w3 = ld3(A.rv + 6 * cap, cap, s);
+        al2 = ld3(A.ra, cap, s);
+        al3 = ld3(A.ra + 3 * cap, cap, s);
+      }
       th = add3(th, mul3(add3(add3(mul3(wi, c1), mul3(w2, c2)), mul3(w3, c3)), dt));
       st3(B.rv, cap, k, add3(wi, mul3(add3(add3(mul3(al1, c1), mul3(al2, c2)), mul3(al3, c3)), dt)));
       st3(B.rv + 3 * cap, cap, k, wi);
@@ -1149,10 +1194,10 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     B.pos[k] = make_double4(pv[0], pv[1], pv[2], ri);
     B.id[k] = idi;
     B.type[k] = dead ? -ti : ti;
-    B.mark[k] = A.mark[s];
+    B.mark[k] = PIM ? mark : A.mark[s];
     // the next step's near-wall list is built here from the new state
     // (UpdateNearPrtclsPW runs at the start of that step, after deletion)
-    B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : A.wmask[s];
+    B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : (PIM ? mask0 : A.wmask[s]);
     if (upd_next && !dead) {
       vnext = norm3(lin1);
       anext = norm3(a1);  // linAcc(1) at the start of the next step
@@ -1185,7 +1230,10 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     for (int q = 0; q < 4; q++)
       if (bc[q]) atomicAdd(&bl.cnt[q], bc[q]);
     if (bdel) atomicAdd(&bl.ndel, bdel);
-    if (blockIdx.x == 0) S->have_parts = 1;
+    if (blockIdx.x == 0) {
+      S->have_parts = 1;
+      S->rv2_stored[B.idx] = 0;
+    }
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
 }
@@ -1245,8 +1293,12 @@ hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
   hipLaunchKernelGGL(k_contact, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
                      D.blk_box, A, D.S, D.BS, D.perm, D.vcnt, D.vlist, D.clist, D.ccnt, D.fpk);
   if (mid) (void)hipEventRecord(mid, st);
-  hipLaunchKernelGGL(k_force, dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A,
-                     B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
+  if (D.pim == 2 && D.prim == 2)
+    hipLaunchKernelGGL((k_force<2, 2>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
+                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
+  else
+    hipLaunchKernelGGL((k_force<0, 0>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
+                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
   return hipGetLastError();
 }
 
