@@ -245,7 +245,8 @@ struct sdem_ctx {
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
   int* pid = nullptr;
-  int *vcnt = nullptr, *vlist = nullptr, *vkey = nullptr;
+  int *vcnt = nullptr, *vlist = nullptr, *vkey = nullptr, *cand = nullptr;
+  int2* act = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
   int64_t nkeys = 0;
@@ -492,6 +493,7 @@ static int upload_state(sdem_ctx* c) {
   M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSLOT);
   M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
   M(&c->vcnt, cap); M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
+  M(&c->act, cap); M(&c->cand, cap * c->lcap);
   M(&c->ccnt, cap); M(&c->clist, cap * c->hcap);
   M(&c->d_blk_box, cap / BOX_ALIGN);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
@@ -761,6 +763,7 @@ static int upload_params(sdem_ctx* c) {
   D.tile_box = d_tile_box;
   D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid; D.vcnt = c->vcnt;
   D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
+  D.act = c->act; D.cand = c->cand;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
   D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
   D.pim = c->boxes[0].o.pi_method; D.prim = c->boxes[0].o.pri_method;
@@ -1099,6 +1102,7 @@ static int grow_vlist(sdem_ctx* c, int list_need) {
   if ((int64_t)neu * c->nslots > ((int64_t)1 << 34)) return c->fail(SDEM_E_CAPACITY, "candidate lists too large");
   hipError_t e = regrow(c, &c->vlist, (size_t)c->nslots * neu);
   if (e == hipSuccess) e = regrow(c, &c->vkey, (size_t)c->nslots * neu);
+  if (e == hipSuccess) e = regrow(c, &c->cand, (size_t)c->nslots * neu);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(candidate lists)");
   c->lcap = neu;
   c->params_ready = false;

@@ -135,7 +135,7 @@ struct BoxState {
   double host_max_a;
   int num_cntcts[4];       // numCntcts of the last step
   int rebuild;             // this step re-sorts the box and rebuilds its lists
-  int pad;
+  int n_act;               // particles with candidates (packed by the list kernel)
   double disp_acc;         // sum over steps since the box's last rebuild of the
                            // largest move of one of its particles: bounds every
                            // displacement since then
@@ -177,6 +177,8 @@ struct Dev {
   double4* pk;           // positions in cell order (rebuild steps)
   int* pid;              // ids in cell order (rebuild steps)
   int *vcnt, *vlist, *vkey;
+  int2* act;             // (particle, candidate count) per packed slot
+  int* cand;             // candidate entries by packed slot: entry q of slot t at q*cap+t
   unsigned* clist;
   int* ccnt;
   const double4* fpk;
