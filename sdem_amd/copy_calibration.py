@@ -3,9 +3,11 @@
 It calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) that rocprofv3
 reports for the DEM kernels: run it under the same `rocprofv3 --pmc` command
 and divide the counter by the bytes moved.  It also prints the copy's own
-bandwidth (bytes read + written per second), the achievable HBM rate that
-the kernels' figures are compared with.  Uses torch only for the allocation
-and the copy.
+bandwidth (bytes read + written per second).  The runtime's copy is not the
+fastest streaming kernel on this part: `sdem_amd/hbm_probe` measures the
+rates of plain copy, read and write kernels, the ceiling the DEM kernels'
+traffic is compared with.  Uses torch only for the allocation and the
+copy.
 
     python3 -m sdem_amd.copy_calibration [MiB]
 """
