@@ -1244,29 +1244,41 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       st3n(B.force + 3 * cap, cap, k, acc.T);
     }
   }
-  // block reductions, then one atomic per block on slot (block % NSLOT)
-  __shared__ double shd[4];
-  __shared__ int shi[4];
-  const double bmove = block_max(move2, shd);
-  int bc[4];
-  for (int q = 0; q < 4; q++) bc[q] = block_sum((int)((acc.nkind >> (16 * q)) & 0xffffull), shi);
-  const int bdel = block_sum(dead ? 1 : 0, shi);
-  double bv = 0.0, ba = 0.0;
-  if (upd_next) {
-    bv = block_max(vnext, shd);
-    ba = block_max(anext, shd);
+  // wave reductions, then one atomic per wave and value on slot
+  // (wave % NSLOT) of the box: no barrier in this kernel.  The counts are
+  // exact integer sums and the maxima exact, whatever the order.
+  double wmove = move2;
+  unsigned long long c01 = (acc.nkind & 0xffffull) | (((acc.nkind >> 16) & 0xffffull) << 32);
+  unsigned long long c23 = ((acc.nkind >> 32) & 0xffffull) | (((acc.nkind >> 48) & 0xffffull) << 32);
+  for (int off = 32; off > 0; off >>= 1) {
+    wmove = fmax(wmove, __shfl_xor(wmove, off, 64));
+    c01 += __shfl_xor(c01, off, 64);
+    c23 += __shfl_xor(c23, off, 64);
   }
-  if (threadIdx.x == 0) {
-    BoxSlot& bl = bslots[box * NSLOT + blockIdx.x % NSLOT];
-    if (bmove > 0.0) atomicMax(&bl.step_bits, dbits(sqrt(bmove)));
-    if (upd_next) {
-      atomicMax(&bl.max_v_bits, dbits(bv));
-      atomicMax(&bl.max_a_bits, dbits(ba));
+  const int wdel = __popcll(__ballot(dead));
+  double wv = 0.0, wa = 0.0;
+  if (upd_next) {
+    wv = vnext;
+    wa = anext;
+    for (int off = 32; off > 0; off >>= 1) {
+      wv = fmax(wv, __shfl_xor(wv, off, 64));
+      wa = fmax(wa, __shfl_xor(wa, off, 64));
     }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    BoxSlot& bl = bslots[box * NSLOT + wave % NSLOT];
+    if (wmove > 0.0) atomicMax(&bl.step_bits, dbits(sqrt(wmove)));
+    if (upd_next) {
+      atomicMax(&bl.max_v_bits, dbits(wv));
+      atomicMax(&bl.max_a_bits, dbits(wa));
+    }
+    const int cnt[4] = {(int)(c01 & 0xffffffffull), (int)(c01 >> 32), (int)(c23 & 0xffffffffull),
+                        (int)(c23 >> 32)};
     for (int q = 0; q < 4; q++)
-      if (bc[q]) atomicAdd(&bl.cnt[q], bc[q]);
-    if (bdel) atomicAdd(&bl.ndel, bdel);
-    if (blockIdx.x == 0) {
+      if (cnt[q]) atomicAdd(&bl.cnt[q], cnt[q]);
+    if (wdel) atomicAdd(&bl.ndel, wdel);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       S->have_parts = 1;
       S->rv2_stored[B.idx] = 0;
     }
