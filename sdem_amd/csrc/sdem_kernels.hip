@@ -669,6 +669,42 @@ __device__ __forceinline__ void vlist_insert(int* __restrict__ vlist, int* __res
   vlist[(size_t)(b + 1) * cap + k] = (int)entry;
 }
 
+// The first VL_LDS entries of a particle's list are kept sorted in LDS
+// (entry q of thread tid at q*256 + tid: a wave's lanes hit consecutive
+// banks); a longer list moves to vlist/vkey and is sorted there.  One
+// 256-thread block holds 32 KB.
+constexpr int VL_LDS = 16, VL_T = 256;
+struct VlStage {
+  int key[VL_LDS * VL_T];
+  int ent[VL_LDS * VL_T];
+};
+
+__device__ __forceinline__ void vlist_add(VlStage& L, int tid, int* __restrict__ vlist, int* __restrict__ vkey,
+                                          size_t cap, int k, int lcap, int& c, unsigned entry, int key) {
+  if (c < lcap) {
+    if (c < VL_LDS) {
+      int b = c - 1;
+      while (b >= 0) {
+        const int kb = L.key[b * VL_T + tid];
+        if (kb <= key) break;
+        L.key[(b + 1) * VL_T + tid] = kb;
+        L.ent[(b + 1) * VL_T + tid] = L.ent[b * VL_T + tid];
+        b--;
+      }
+      L.key[(b + 1) * VL_T + tid] = key;
+      L.ent[(b + 1) * VL_T + tid] = (int)entry;
+    } else {
+      if (c == VL_LDS)
+        for (int q = 0; q < VL_LDS; q++) {
+          vkey[(size_t)q * cap + k] = L.key[q * VL_T + tid];
+          vlist[(size_t)q * cap + k] = L.ent[q * VL_T + tid];
+        }
+      vlist_insert(vlist, vkey, cap, k, c, entry, key);
+    }
+  }
+  c++;
+}
+
 __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                                Scalars* S, BoxState* BS, const double4* __restrict__ pk,
                                                const int* __restrict__ pid, const int* __restrict__ cstart,
@@ -681,6 +717,8 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const int box = block_box(blk_box);
   if (!BS[box].rebuild) return;
   const Params& P = PB[box];
+  __shared__ VlStage stage;
+  const int tid = threadIdx.x;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = k - P.off < BS[box].n_alive;
   const size_t cap = P.cap;
@@ -735,8 +773,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
             if (kj == k && !shifted) continue;
-            if (c < lcap) vlist_insert(vlist, vkey, cap, k, c, (unsigned)kj, 2 * pid[kj]);
-            c++;
+            vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj, 2 * pid[kj]);
           }
         }
       }
@@ -779,8 +816,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
             const double dr = ri + pj.w + P.skin;
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
-            if (c < lcap) vlist_insert(vlist, vkey, cap, k, c, (unsigned)kj | CL_FIXED, 2 * fid[kj] + 1);
-            c++;
+            vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj | CL_FIXED, 2 * fid[kj] + 1);
           }
         }
       }
@@ -809,7 +845,10 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const int t = P.off + t0 + __popcll(bal & ((1ull << lane) - 1ull));
   act[t] = make_int2(k, c);
   const int nq = c < lcap ? c : lcap;
-  for (int q = 0; q < nq; q++) cand[(size_t)q * cap + t] = vlist[(size_t)q * cap + k];
+  if (c <= VL_LDS || lcap <= VL_LDS)  // the list never left LDS
+    for (int q = 0; q < nq; q++) cand[(size_t)q * cap + t] = stage.ent[q * VL_T + tid];
+  else
+    for (int q = 0; q < nq; q++) cand[(size_t)q * cap + t] = vlist[(size_t)q * cap + k];
 }
 
 // ---------------------------------------------------------------------------
