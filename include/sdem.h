@@ -249,8 +249,11 @@ int sdem_get_tuning(sdem_ctx* c, double cell_factor[3], double* skin_factor, int
 
 /* Autotuner (no counterpart in the reference): tries cell factors, skins
  * and block sizes, each for steps_per_trial timed steps (plus two settling
- * steps), and keeps the fastest.  The trials are real steps of the run
- * (the results do not depend on the tuning); *steps_run says how many. */
+ * steps) in a forward and a backward pass over the settings, and keeps the
+ * one with the smallest mean (a run whose cost drifts while the trials run
+ * favours no setting for its place in the order).  The trials are real
+ * steps of the run (the results do not depend on the tuning); *steps_run
+ * says how many. */
 int sdem_autotune(sdem_ctx* c, int32_t steps_per_trial, int32_t* steps_run, double* best_ms_per_step);
 
 /* Memory the context holds on the device, in bytes. */
