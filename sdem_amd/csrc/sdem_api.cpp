@@ -246,7 +246,7 @@ struct sdem_ctx {
   double4* pk = nullptr;
   int* pid = nullptr;
   int *vcnt = nullptr, *vlist = nullptr, *vkey = nullptr, *cand = nullptr;
-  int2* act = nullptr;
+  int4* act = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
   int64_t nkeys = 0;

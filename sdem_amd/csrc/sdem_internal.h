@@ -177,7 +177,7 @@ struct Dev {
   double4* pk;           // positions in cell order (rebuild steps)
   int* pid;              // ids in cell order (rebuild steps)
   int *vcnt, *vlist, *vkey;
-  int2* act;             // (particle, candidate count) per packed slot
+  int4* act;             // (particle, candidate count, smallest gap as a double) per packed slot
   int* cand;             // candidate entries by packed slot: entry q of slot t at q*cap+t
   unsigned* clist;
   int* ccnt;

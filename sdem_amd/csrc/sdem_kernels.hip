@@ -709,7 +709,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
                                                Scalars* S, BoxState* BS, const double4* __restrict__ pk,
                                                const int* __restrict__ pid, const int* __restrict__ cstart,
                                                int* __restrict__ vcnt, int* __restrict__ vlist,
-                                               int* __restrict__ vkey, int2* __restrict__ act,
+                                               int* __restrict__ vkey, int4* __restrict__ act,
                                                int* __restrict__ cand, int* __restrict__ ccnt,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid, const int* __restrict__ fstart) {
@@ -724,11 +724,14 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const size_t cap = P.cap;
   const int lcap = P.lcap;
   int c = 0;
+  double gmin = 1.0e300;  // smallest gap to a candidate, |xi - xj| - (ri + rj)
+  double ri_live = 0.0;
   if (live) {
   const int base = box_start_base(P, cstart);
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
+  ri_live = ri;
   for (int L = 0; L < P.nlvl; L++) {
     const Grid& G = P.lvl[L];
     const double reach = ri + G.rmax + P.skin;
@@ -773,6 +776,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
             if (kj == k && !shifted) continue;
+            gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
             vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj, 2 * pid[kj]);
           }
         }
@@ -816,6 +820,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
             const double dr = ri + pj.w + P.skin;
             const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 > dr * dr) continue;
+            gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
             vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj | CL_FIXED, 2 * fid[kj] + 1);
           }
         }
@@ -843,7 +848,10 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   t0 = __shfl(t0, leader, 64);
   if (!has) return;
   const int t = P.off + t0 + __popcll(bal & ((1ull << lane) - 1ull));
-  act[t] = make_int2(k, c);
+  // the gap less a margin for the roundings of this and the contact test
+  // (both are within a few units in the last place of ri + rj)
+  const double gstore = gmin - 1.0e-9 * ri_live;
+  act[t] = make_int4(k, c, __double2loint(gstore), __double2hiint(gstore));
   const int nq = c < lcap ? c : lcap;
   if (c <= VL_LDS || lcap <= VL_LDS)  // the list never left LDS
     for (int q = 0; q < nq; q++) cand[(size_t)q * cap + t] = stage.ent[q * VL_T + tid];
@@ -922,7 +930,7 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ blk_box, Buf A, Scalars* S,
                                                  const BoxState* BS,
                                                  const int* __restrict__ perm,
-                                                 const int2* __restrict__ act,
+                                                 const int4* __restrict__ act,
                                                  const int* __restrict__ cand,
                                                  unsigned* __restrict__ clist,
                                                  int* __restrict__ ccnt,
@@ -941,9 +949,19 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   unsigned ent[CPF];
   for (int q = 0; q < CPF; q++)
     if (q < P.lcap) ent[q] = (unsigned)cand[(size_t)q * cap + t];
-  const int2 a = act[t];
+  const int4 a = act[t];
   const int k = a.x, nq = min(a.y, P.lcap);
   const bool rebuilt = BS[box].rebuild != 0;
+  // No particle of the box has moved more than disp_acc since the lists
+  // were built, so no listed pair has closed its gap by more than twice
+  // that: while the smallest gap of the list (less a rounding margin) is
+  // larger, no candidate touches and nothing more is loaded.  disp_acc
+  // grows until the next rebuild, so the count stays the 0 written then.
+  const double gap = __hiloint2double(a.w, a.z);
+  if (gap > 2.0 * BS[box].disp_acc * (1.0 + 1.0e-9)) {
+    if (rebuilt) ccnt[k] = 0;
+    return;
+  }
   const int s = rebuilt ? perm[k] : k;
   double4 pjs[CPF];
   for (int q = 0; q < CPF; q++)
