@@ -81,3 +81,23 @@ def compare(gpu: native.Engine, ref: native.Engine, exact_positions=True, rtol=R
     if (gi if wall_cadence else gi[0]) != (ri if wall_cadence else ri[0]):
         errs.append(f"iter/next wall update {gi} vs {ri}")
     return errs
+
+
+def aging_lists_scene(seed=31, speed=3.0):
+    """400 sparse spheres (no contact at the start), every one at the same
+    speed in a random direction, periodic in x, y and z, dt 1e-5 s: pairs
+    close head-on at up to twice the largest single move, so a candidate
+    list ages over several steps and the gap bound of the contact test is
+    exercised (tests/test_list_bounds.py, tests/test_gpu_parity.py)."""
+    from sdem_amd import scenarios
+
+    sc = scenarios.random_cloud(n=400, seed=seed, spacing=1.8, vel=1.0, omega=0.0, dt=1.0e-5,
+                                periodic=(1, 1, 1), walls="")
+    st = sc["state"]
+    rng = np.random.default_rng(seed + 1)
+    u = rng.normal(size=(len(st["id"]), 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    st["lin_vel"][:, :, :] = (speed * u)[:, None, :]
+    st["lin_acc"][:, :, :] = 0.0
+    return sc
+

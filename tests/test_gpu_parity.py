@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from parity import compare
+from parity import aging_lists_scene, compare
 from sdem_amd import native, scenarios
 
 pytestmark = pytest.mark.gpu
@@ -70,6 +70,30 @@ def test_candidate_list_skin_does_not_change_results(skin, hip_lib, oracle_lib):
         ref.iterate(10)
         errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
         assert not errs, (k, errs)
+
+
+def test_contacts_found_while_lists_age(hip_lib, oracle_lib):
+    """Sparse, fast spheres and a skin of 0.3 d_max: the candidate lists
+    stay in use for several steps, and a particle skips its contact test
+    while its closest listed neighbour cannot have closed the gap (twice the
+    box's summed largest moves since the lists were built).  Pairs close in
+    head-on from every gap (tests/test_list_bounds.py checks on the oracle's
+    trajectory that this scene separates twice the summed moves from once);
+    the contact set and the state are the oracle's after every step."""
+    print("aging lists: 400 sparse fast spheres, skin 0.3, 80 single steps", flush=True)
+    sc = aging_lists_scene()
+    gpu = native.build_engine(hip_lib, sc, tuning=dict(skin=0.3))
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    first = None
+    for step in range(80):
+        gpu.iterate(1)
+        ref.iterate(1)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, (step, errs)
+        if first is None and gpu.get_counts().sum() > 0:
+            first = step
+    assert first is not None and first > 0, first  # no contact at the start; contacts form later
+    assert gpu.get_counts().sum() > 5
 
 
 def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
