@@ -1805,7 +1805,11 @@ extern "C" int sdem_internal_set_marks(sdem_ctx* c, int32_t n, const int32_t* id
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(marks)");
   for (const auto& q : idx)
     if (auto p = find(q.first)) mk[q.second] = p->second;
+  // both buffers: the force kernel carries marks over only after re-sorts
+  // (after one, the other buffer is rewritten from this one anyway)
   e = hipMemcpy(A.mark, mk.data(), mk.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->buf[1 - c->cur].mark, mk.data(), mk.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(marks)");
   return SDEM_OK;
 }

@@ -136,6 +136,9 @@ struct BoxState {
   int num_cntcts[4];       // numCntcts of the last step
   int rebuild;             // this step re-sorts the box and rebuilds its lists
   int n_act;               // particles with candidates (packed by the list kernel)
+  int prev_rebuild;        // the step before this one re-sorted the box
+  int wall_prev;           // the step before refreshed the near-wall lists
+  int pad;
   double disp_acc;         // sum over steps since the box's last rebuild of the
                            // largest move of one of its particles: bounds every
                            // displacement since then

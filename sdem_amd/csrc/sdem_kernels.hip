@@ -351,12 +351,14 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
   // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
   const double disp = B.disp_acc + step;
   const bool rebuild = forced || ndel > 0 || !(disp <= P.rebuild_disp);
+  B.prev_rebuild = B.rebuild;
   B.rebuild = rebuild ? 1 : 0;
   if (rebuild) B.n_act = 0;  // the list kernel packs the box's particles with candidates anew
   B.disp_acc = rebuild ? 0.0 : disp;
   const int next = B.next_wall_update;
   B.n_stored = B.n_alive;
   B.next_wall_at_begin = next;
+  B.wall_prev = iter >= next ? 1 : 0;  // the last force kernel wrote fresh near-wall lists
   if (iter >= next) {
     double max_v = mv + P.max_wall_vel;
     max_v = max_v > 0.3 ? max_v : 0.3;
@@ -1037,6 +1039,15 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
   const bool rebuilt = BS[box].rebuild != 0;  // this step re-sorted the box: A is read through perm
   // k_ctl has advanced iterNumber: is the next step a near-wall update?
   const bool upd_next = S->iter >= BS[box].next_wall_update;
+  // id, type, mark and the near-wall list change only where the box is
+  // re-sorted, a particle is removed or the lists are refreshed.  They are
+  // written when the buffer written here may hold other values: this step
+  // or the one before re-sorted the box (the other buffer is in an older
+  // order), this step or the one before refreshed the lists.  Otherwise
+  // both buffers hold the same values in the same order and the stores are
+  // left out.
+  const bool put_ids = rebuilt || BS[box].prev_rebuild != 0;
+  const bool put_wall = put_ids || upd_next || BS[box].wall_prev != 0;
   Acc acc;
   acc.F = D3(0.0, 0.0, 0.0);
   acc.T = D3(0.0, 0.0, 0.0);
@@ -1081,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
         al2 = ld3n(A.ra, cap, s);
         al3 = ld3n(A.ra + 3 * cap, cap, s);
       }
-      mark = A.mark[s];
+      if (put_ids) mark = A.mark[s];
     }
 
     // ---------------- every contact, one call site ------------------------
@@ -1285,12 +1296,15 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       }
     }
     B.pos[k] = make_double4(pv[0], pv[1], pv[2], ri);
-    B.id[k] = idi;
-    B.type[k] = dead ? -ti : ti;
-    B.mark[k] = PIM ? mark : A.mark[s];
+    if (put_ids) {
+      B.id[k] = idi;
+      B.mark[k] = PIM ? mark : A.mark[s];
+    }
+    if (put_ids || dead) B.type[k] = dead ? -ti : ti;
     // the next step's near-wall list is built here from the new state
     // (UpdateNearPrtclsPW runs at the start of that step, after deletion)
-    B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : (PIM ? mask0 : A.wmask[s]);
+    if (put_wall)
+      B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : (PIM ? mask0 : A.wmask[s]);
     if (upd_next && !dead) {
       vnext = norm3(lin1);
       anext = norm3(a1);  // linAcc(1) at the start of the next step
