@@ -42,8 +42,8 @@ SKIN = 0.05
 # HBM bytes per k_force launch on the default 1M bed, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by what the same
 # counters read for a plain 512 MiB copy (x2.00 and x1.00):
-# 97,996 KiB x 2 read + 195,802 KiB written (profiles/k_force_settling_1M.md)
-KFORCE_TRAFFIC_1M = (97_996 * 2 + 195_802) * 1024
+# 96,058 KiB x 2 read + 176,272 KiB written (profiles/k_force_settling_1M.md)
+KFORCE_TRAFFIC_1M = (96_058 * 2 + 176_272) * 1024
 
 
 def parse():
