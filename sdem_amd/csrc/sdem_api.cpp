@@ -245,7 +245,7 @@ struct sdem_ctx {
   int *perm_tmp = nullptr, *perm = nullptr;
   double4* pk = nullptr;
   int* pid = nullptr;
-  int *vcnt = nullptr, *vlist = nullptr, *vkey = nullptr, *cand = nullptr;
+  int *vlist = nullptr, *vkey = nullptr, *cand = nullptr;
   int4* act = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
@@ -492,7 +492,7 @@ static int upload_state(sdem_ctx* c) {
   auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
   M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSLOT);
   M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
-  M(&c->vcnt, cap); M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
+  M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
   M(&c->act, cap); M(&c->cand, cap * c->lcap);
   M(&c->ccnt, cap); M(&c->clist, cap * c->hcap);
   M(&c->d_blk_box, cap / BOX_ALIGN);
@@ -761,7 +761,7 @@ static int upload_params(sdem_ctx* c) {
   D.S = c->S; D.BS = c->BS; D.bslots = c->bslots;
   D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
   D.tile_box = d_tile_box;
-  D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid; D.vcnt = c->vcnt;
+  D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid;
   D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
   D.act = c->act; D.cand = c->cand;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;

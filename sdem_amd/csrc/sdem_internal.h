@@ -179,7 +179,7 @@ struct Dev {
   const int2* tile_box;  // first and last box whose keys a scan tile covers
   double4* pk;           // positions in cell order (rebuild steps)
   int* pid;              // ids in cell order (rebuild steps)
-  int *vcnt, *vlist, *vkey;
+  int *vlist, *vkey;     // candidate lists by particle (lists longer than the LDS stage)
   int4* act;             // (particle, candidate count, smallest gap as a double) per packed slot
   int* cand;             // candidate entries by packed slot: entry q of slot t at q*cap+t
   unsigned* clist;

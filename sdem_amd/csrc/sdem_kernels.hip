@@ -710,7 +710,7 @@ __device__ __forceinline__ void vlist_add(VlStage& L, int tid, int* __restrict__
 __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                                Scalars* S, BoxState* BS, const double4* __restrict__ pk,
                                                const int* __restrict__ pid, const int* __restrict__ cstart,
-                                               int* __restrict__ vcnt, int* __restrict__ vlist,
+                                               int* __restrict__ vlist,
                                                int* __restrict__ vkey, int4* __restrict__ act,
                                                int* __restrict__ cand, int* __restrict__ ccnt,
                                                const double4* __restrict__ fpk,
@@ -829,7 +829,6 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
       }
     }
   }
-  vcnt[k] = c;
   if (c > lcap) {
     atomicMax(&S->overflow, 2);
     atomicMax(&S->list_need, c);
@@ -1404,7 +1403,7 @@ hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st) {
 
 hipError_t launch_vlist(const Dev& D, hipStream_t st) {
   hipLaunchKernelGGL(k_vlist, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box,
-                     D.S, D.BS, D.pk, D.pid, D.start, D.vcnt, D.vlist, D.vkey, D.act, D.cand, D.ccnt, D.fpk,
+                     D.S, D.BS, D.pk, D.pid, D.start, D.vlist, D.vkey, D.act, D.cand, D.ccnt, D.fpk,
                      D.fid, D.fstart);
   return hipGetLastError();
 }
