@@ -20,9 +20,13 @@
 //                             cell order
 //   k_vlist                 : candidate lists, pairs within ri+rj+skin
 //                             (fixed particles included), sorted by
-//                             partner id
+//                             partner id; the particles that have any are
+//                             packed per box with their smallest gap
 //   -- every step --
-//   k_contact               : exact contact test over the candidate lists
+//   k_contact               : exact contact test over the packed lists
+//                             (a particle whose smallest gap exceeds twice
+//                             the box's summed largest moves since the
+//                             build touches nothing and is skipped)
 //   k_force                 : history match, contact forces, accelerations,
 //                             integration, wrap/delete, the largest move of
 //                             the step, and the next step's near-wall masks
