@@ -63,6 +63,8 @@ def parse():
                          "independent sand boxes stepped together; surrogate: the learned surrogate "
                          "trained on DEM trajectories, then against the DEM on the 1M-sphere bed")
     ap.add_argument("--train-snapshots", type=int, default=240)
+    ap.add_argument("--train-spacing", type=int, default=10,
+                    help="DEM steps between training snapshots (surrogate workload)")
     ap.add_argument("--boxes", type=int, default=256)
     return ap.parse_args()
 
@@ -156,7 +158,7 @@ def surrogate_bench(args):
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
     losses = surrogate.train(model, n=50_000, seeds=(101, 102, 103), warm=200, snapshots=args.train_snapshots,
-                             log=log, lib=lib)
+                             spacing=args.train_spacing, log=log, lib=lib)
     t_train = time.perf_counter() - t0
     speed = surrogate.rollout(model, n=args.n, seed=args.seed, warm=args.warmup, steps=200, log=log, lib=lib)
     err = surrogate.rollout(model, n=200_000, seed=args.seed, warm=args.warmup, steps=args.steps, log=log, lib=lib)
@@ -176,7 +178,8 @@ def surrogate_bench(args):
         "config": {"workload": f"settling bed, {args.n} spheres; graph network 8 layers x 2x128 SiLU MLPs, "
                                f"10 DEM steps per call, trained on {args.train_snapshots} snapshots of "
                                "3 x 50k-sphere boxes", "train_s": t_train,
-                   "train_loss_ratio_last": float(np.mean(losses[-20:]))},
+                   "train_loss_ratio_last": float(np.mean(losses[-20:])),
+                   "train_snapshot_spacing": args.train_spacing},
         "timing_s_1M_200_steps": {"dem": speed["dem_s"], "surrogate": speed["surrogate_s"],
                                   "ballistic": speed["ballistic_s"]},
         "errors_vs_dem_200k": {k: v for k, v in err.items()
