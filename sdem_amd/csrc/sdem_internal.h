@@ -135,10 +135,11 @@ struct BoxState {
   double host_max_a;
   int num_cntcts[4];       // numCntcts of the last step
   int rebuild;             // this step re-sorts the box and rebuilds its lists
-  int n_act;               // particles with candidates (packed by the list kernel)
+  int n_act;               // particles with candidates and a gap under half the skin (packed)
   int prev_rebuild;        // the step before this one re-sorted the box
   int wall_prev;           // the step before refreshed the near-wall lists
-  int pad;
+  int n_far;               // ... of them with all gaps over half the skin (packed
+                           // from the end of the box's slots down)
   double disp_acc;         // sum over steps since the box's last rebuild of the
                            // largest move of one of its particles: bounds every
                            // displacement since then

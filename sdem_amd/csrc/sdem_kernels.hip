@@ -357,7 +357,7 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
   const bool rebuild = forced || ndel > 0 || !(disp <= P.rebuild_disp);
   B.prev_rebuild = B.rebuild;
   B.rebuild = rebuild ? 1 : 0;
-  if (rebuild) B.n_act = 0;  // the list kernel packs the box's particles with candidates anew
+  if (rebuild) B.n_act = B.n_far = 0;  // the list kernel packs the box's particles with candidates anew
   B.disp_acc = rebuild ? 0.0 : disp;
   const int next = B.next_wall_update;
   B.n_stored = B.n_alive;
@@ -840,24 +840,32 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
     atomicMax(&S->list_need, c);
   }
   }  // live
-  // The particles with candidates, packed per box from slot P.off on (one
-  // atomic per wave; the order does not enter any result): the contact test
-  // runs over this list only, with (particle, count) and the entries
-  // indexed by the packed slot t.  The others have no contact until the
-  // next rebuild.
+  // The particles with candidates, packed per box (one atomic per wave; the
+  // order does not enter any result): the contact test runs over these
+  // only, with (particle, count, smallest gap) and the entries indexed by
+  // the packed slot t.  Those whose smallest gap is under half the skin go
+  // from slot P.off up, the others from the box's last slot down, so that
+  // the waves of the latter skip as a whole until the box has moved a
+  // quarter of the skin.  The particles without candidates have no contact
+  // until the next rebuild.
   const bool has = live && c > 0;
-  const unsigned long long bal = __ballot(has);
-  if (live && !has) ccnt[k] = 0;
-  if (!bal) return;
-  const int lane = threadIdx.x & 63, leader = __ffsll((long long)bal) - 1;
-  int t0 = 0;
-  if (lane == leader) t0 = atomicAdd(&BS[box].n_act, __popcll(bal));
-  t0 = __shfl(t0, leader, 64);
-  if (!has) return;
-  const int t = P.off + t0 + __popcll(bal & ((1ull << lane) - 1ull));
   // the gap less a margin for the roundings of this and the contact test
   // (both are within a few units in the last place of ri + rj)
   const double gstore = gmin - 1.0e-9 * ri_live;
+  const bool far = has && gstore > 0.5 * P.skin, near = has && !far;
+  const unsigned long long bn = __ballot(near), bfar = __ballot(far);
+  if (live && !has) ccnt[k] = 0;
+  if (!(bn | bfar)) return;
+  const int lane = threadIdx.x & 63;
+  const int ln = bn ? __ffsll((long long)bn) - 1 : 0, lf = bfar ? __ffsll((long long)bfar) - 1 : 0;
+  int tn = 0, tf = 0;
+  if (bn && lane == ln) tn = atomicAdd(&BS[box].n_act, __popcll(bn));
+  if (bfar && lane == lf) tf = atomicAdd(&BS[box].n_far, __popcll(bfar));
+  tn = __shfl(tn, ln, 64);
+  tf = __shfl(tf, lf, 64);
+  if (!has) return;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int t = near ? P.off + tn + __popcll(bn & below) : P.off + P.bcap - 1 - (tf + __popcll(bfar & below));
   act[t] = make_int4(k, c, __double2loint(gstore), __double2hiint(gstore));
   const int nq = c < lcap ? c : lcap;
   if (c <= VL_LDS || lcap <= VL_LDS)  // the list never left LDS
@@ -945,9 +953,10 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   if (S->halt || S->overflow == 2) return;
   const int box = block_box(blk_box);
   const Params& P = PB[box];
-  // packed slot t: the (t - off)-th particle of the box with candidates
+  // packed slot t: a particle of the box with candidates, from either end of
+  // the box's slots
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t - P.off >= BS[box].n_act) return;
+  if (t - P.off >= BS[box].n_act && t - P.off < P.bcap - BS[box].n_far) return;
   const size_t cap = P.cap;
   // the packed entry and the first CPF list entries, all in flight at once
   // (their addresses depend on t only), then every partner position of
