@@ -223,6 +223,31 @@ def test_surrogate_advances_a_dem_context(hip_lib):
     assert np.all(np.isfinite(eng.get_state()["posr"]))
 
 
+def test_surrogate_keeps_rotvel2_of_the_last_dem_step(hip_lib):
+    """With AB2 rotation the force kernel leaves rotVel(2) out of the state
+    it writes (it equals that state's linVel(1)).  The surrogate replaces
+    linVel(1) and leaves the rotation alone, so before its first call the
+    library stores rotVel(2): afterwards it is still the linVel(1) of the last
+    DEM step, as if every step had stored it."""
+    print("surrogate: rotVel(2) across surrogate calls", flush=True)
+    from sdem_amd import surrogate
+    m = surrogate.make_model(hidden=32, layers=2, seed=9, lib=hip_lib)
+    sc = scenarios.settling(n=20000, seed=5)
+    eng = native.build_engine(hip_lib, sc)
+    eng.iterate(7)
+    s0 = eng.get_state()
+    assert s0["rot_vel"].shape[1] == 2
+    assert np.array_equal(s0["rot_vel"][:, 1], s0["lin_vel"][:, 0])  # AB2 quirk, read back
+    m.advance(eng, 2)
+    s1 = eng.get_state()
+    v0 = dict(zip(s0["id"].tolist(), s0["lin_vel"][:, 0].tolist()))
+    assert not np.array_equal(s1["lin_vel"][:, 0], s0["lin_vel"][:, 0])  # the surrogate moved it
+    assert [v0[i] for i in s1["id"].tolist()] == s1["rot_vel"][:, 1].tolist()
+    eng.iterate(3)
+    s2 = eng.get_state()
+    assert np.array_equal(s2["rot_vel"][:, 1], s2["lin_vel"][:, 0])
+
+
 def test_fused_inference_matches_training_path(hip_lib):
     """Inference runs each message-passing MLP as one fused kernel (input
     gather, three matrix-core products, activations on chip); it gives the
