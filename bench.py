@@ -42,8 +42,11 @@ SKIN = 0.05
 # HBM bytes per k_force launch on the default 1M bed, measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by what the same
 # counters read for a plain 512 MiB copy (x2.00 and x1.00):
-# 96,058 KiB x 2 read + 176,272 KiB written (profiles/k_force_settling_1M.md)
-KFORCE_TRAFFIC_1M = (96_058 * 2 + 176_272) * 1024
+# 96,044 KiB x 2 read + 176,272 KiB written (profiles/k_force_settling_1M.md)
+KFORCE_TRAFFIC_1M = (96_044 * 2 + 176_272) * 1024
+# steps per library call in a timed region (a few seconds of work at most,
+# so that a long run still prints progress)
+CHUNK = 20_000
 
 
 def parse():
@@ -115,7 +118,7 @@ def ensemble_bench(args):
           f"{eng.get_tuning()}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     done = 0
-    chunk = max(2, (args.steps // 10) // 2 * 2)
+    chunk = CHUNK
     while done < args.steps:
         k = min(chunk, args.steps - done)
         eng.iterate(k)
@@ -238,10 +241,12 @@ def main():
     print(f"# warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
     # Timed region: steps queued as HIP graphs; Engine.iterate returns after
     # synchronising the library's stream, so the region is bracketed by
-    # device synchronisation on both sides.
+    # device synchronisation on both sides.  One call per CHUNK steps (a
+    # progress line and a contact-count sample after each): every call ends
+    # in a host synchronisation, so the default run is one call.
     t0 = time.perf_counter()
     done, counts_sum, nsamp = 0, np.zeros(4, np.int64), 0
-    chunk = max(2, (args.steps // 10) // 2 * 2)
+    chunk = CHUNK
     while done < args.steps:
         k = min(chunk, args.steps - done)
         eng.iterate(k)

@@ -253,7 +253,7 @@ struct sdem_ctx {
   bool use_graph = true;
   // graphs of one and of two steps, without / with the speculative fast
   // path (single box: no sort and list kernels), by starting buffer
-  hipGraphExec_t gexec[2][2][2] = {};  // [fast][nsteps - 1][buffer]
+  hipGraphExec_t gexec[2][4][2] = {};  // [fast][1, 2, 8 or 32 steps][buffer]
   bool spec_ready = false;  // the next step may try the fast path
   int spec_chunk = 8;       // steps queued per fast batch (adapted)
   int spec_full = 1;        // steps queued per full batch between fast ones: grows
@@ -1129,11 +1129,13 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev, bool fast = false) {
   return e;
 }
 
-// A graph of `n` (1 or 2) consecutive steps starting from buffer c->cur
-// (the kernels read their control flags from device memory, so the graph
-// is fixed); c->cur is left as it was.
+// A graph of `n` (1, 2, 8 or 32) consecutive steps starting from buffer
+// c->cur (the kernels read their control flags from device memory, so the
+// graph is fixed); c->cur is left as it was.  Long runs of fast steps go in
+// the larger graphs: each graph launch costs several microseconds more than
+// a kernel inside a graph.
 static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out) {
-  hipGraphExec_t& slot = c->gexec[fast ? 1 : 0][n - 1][c->cur];
+  hipGraphExec_t& slot = c->gexec[fast ? 1 : 0][n == 1 ? 0 : (n == 2 ? 1 : (n == 8 ? 2 : 3))][c->cur];
   if (slot) {
     *out = slot;
     return hipSuccess;
@@ -1185,6 +1187,12 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     int it = 0;
     if (!c->prof && c->use_graph) {
+      for (int big : {32, 8})
+        for (; fast && it + big <= batch && e == hipSuccess; it += big) {
+          hipGraphExec_t g = nullptr;
+          e = step_graph(c, big, fast, &g);
+          if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
+        }
       for (; it + 2 <= batch && e == hipSuccess; it += 2) {
         hipGraphExec_t g = nullptr;
         e = step_graph(c, 2, fast, &g);
