@@ -1131,9 +1131,9 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev, bool fast = false) {
 
 // A graph of `n` (1, 2, 8 or 32) consecutive steps starting from buffer
 // c->cur (the kernels read their control flags from device memory, so the
-// graph is fixed); c->cur is left as it was.  Long runs of fast steps go in
-// the larger graphs: each graph launch costs several microseconds more than
-// a kernel inside a graph.
+// graph is fixed); c->cur is left as it was.  Long batches go in the larger
+// graphs: each graph launch costs several microseconds more than a kernel
+// inside a graph.
 static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out) {
   hipGraphExec_t& slot = c->gexec[fast ? 1 : 0][n == 1 ? 0 : (n == 2 ? 1 : (n == 8 ? 2 : 3))][c->cur];
   if (slot) {
@@ -1188,7 +1188,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     int it = 0;
     if (!c->prof && c->use_graph) {
       for (int big : {32, 8})
-        for (; fast && it + big <= batch && e == hipSuccess; it += big) {
+        for (; it + big <= batch && e == hipSuccess; it += big) {
           hipGraphExec_t g = nullptr;
           e = step_graph(c, big, fast, &g);
           if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
