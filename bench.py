@@ -271,6 +271,10 @@ def main():
     achieved = kernel_bytes / k_avg_s / 1e9
     step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * P
     metric, workload = WORKLOADS[args.workload]
+    if args.workload == "settling" and n != 1_000_000:
+        side = 0.2 * (n / 1.0e6) ** 0.5  # scenarios.settling scales the xy area with n
+        metric = f"particle-steps/s, settling bed of {n} spheres"
+        workload = workload.replace("1M", f"{n}").replace("0.2x0.2 m", f"{side:.4g}x{side:.4g} m")
     headline = args.workload == "settling" and n == 1_000_000 and args.seed == 20261203
     out = {
         "metric": metric,
