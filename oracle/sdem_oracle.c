@@ -1552,8 +1552,12 @@ static void update_near_prtcls_pw(oracle_ctx *c, int iter) {
   double max_a = fmax_f(maxreal, norm3(R3(c->o.gravity[0], c->o.gravity[1], c->o.gravity[2])));
   max_a = max_a * 1.5;
   double t = (sqrt(max_v * max_v + 2.0 * max_a * c->dxmax1) - max_v) / max_a;
-  int num_iter = (int)(t / c->o.dt);
-  if (num_iter < 1) num_iter = 1;
+  /* int(t/dt) (:225): t is NaN (0/0) when nothing accelerates and there is
+     no gravity; int() of NaN or of 2^31 or more gives the most negative
+     integer on x86 (cvttsd2si), which max(., 1) makes 1.  Spelled out here,
+     since the C cast of such values is undefined. */
+  double q = t / c->o.dt;
+  int num_iter = (q >= 1.0 && q < 2147483648.0) ? (int)q : 1;
   if (num_iter > c->o.wall_max_update_iter) num_iter = c->o.wall_max_update_iter;
   c->next_iter_update = iter + num_iter;
 }

@@ -21,11 +21,11 @@
 
 namespace sdem {
 hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st);
-hipError_t launch_ctl(const Dev& D, hipStream_t st, bool fast);
-hipError_t launch_finish(const Dev& D, hipStream_t st);
+hipError_t launch_ctl(const Dev& D, const Buf& A, hipStream_t st);
+hipError_t launch_finish(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_vlist(const Dev& D, hipStream_t st);
-hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid);
+hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid, bool fast);
 int64_t scan_tiles(int64_t n);
 int scan_tile_keys();
 }  // namespace sdem
@@ -490,7 +490,7 @@ static int upload_state(sdem_ctx* c) {
   }
   hipError_t e = dalloc(c, c->allocs_state, &c->S, 1);
   auto M = [&](auto** p, size_t n) { if (e == hipSuccess) e = dalloc(c, c->allocs_state, p, n); };
-  M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSLOT);
+  M(&c->BS, nbox); M(&c->bslots, (size_t)nbox * NSET * NSLOT);
   M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
   M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
   M(&c->act, cap); M(&c->cand, cap * c->lcap);
@@ -573,7 +573,7 @@ static int upload_state(sdem_ctx* c) {
   s.need_rebuild = 1;
   s.rv2_stored[0] = s.rv2_stored[1] = 1;
   H(c->S, &s, sizeof(s));
-  if (e == hipSuccess) e = hipMemset(c->bslots, 0, sizeof(BoxSlot) * NSLOT * nbox);
+  if (e == hipSuccess) e = hipMemset(c->bslots, 0, sizeof(BoxSlot) * NSET * NSLOT * nbox);
   if (e != hipSuccess) return hip_fail(c, e, "upload state");
   c->cur = 0;
   c->dev_ready = true;
@@ -1111,19 +1111,20 @@ static int grow_vlist(sdem_ctx* c, int list_need) {
 
 // Queue one step on the context's stream; ev (SDEM_NUM_PHASES+1 events) may
 // be null.  The buffers swap roles afterwards.  A fast step (single box)
-// leaves out the sort and list kernels; its control kernel halts the call
-// if the box must rebuild.
+// leaves out the sort and list kernels, and its control runs inside the
+// contact-test kernel; it halts the call if the box must rebuild.
 static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev, bool fast = false) {
   Buf& A = c->buf[c->cur];
   Buf& B = c->buf[1 - c->cur];
+  fast = fast && c->D.nbox == 1;
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  hipError_t e = launch_ctl(c->D, c->stream, fast);
+  hipError_t e = fast ? hipSuccess : launch_ctl(c->D, A, c->stream);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (e == hipSuccess && !fast) e = launch_sort(c->D, A, c->stream);
   if (ev) (void)hipEventRecord(ev[2], c->stream);
   if (e == hipSuccess && !fast) e = launch_vlist(c->D, c->stream);
   if (ev) (void)hipEventRecord(ev[3], c->stream);
-  if (e == hipSuccess) e = launch_force(c->D, A, B, c->stream, ev ? ev[4] : nullptr);
+  if (e == hipSuccess) e = launch_force(c->D, A, B, c->stream, ev ? ev[4] : nullptr, fast);
   if (ev) (void)hipEventRecord(ev[5], c->stream);
   c->cur = 1 - c->cur;
   return e;
@@ -1208,7 +1209,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     for (; it < batch && e == hipSuccess; it++)
       e = enqueue_step(c, c->prof ? &evs[(size_t)it * NE] : nullptr);
-    if (e == hipSuccess) e = launch_finish(c->D, c->stream);
+    if (e == hipSuccess) e = launch_finish(c->D, c->buf[c->cur], c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
       for (auto& x : evs) (void)hipEventDestroy(x);
@@ -1245,6 +1246,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     remaining -= ok_steps;
     s.halt = 0;
     s.halted_steps = 0;
+    s.cnt_over = 0;
     if (s.spec_fail) {
       // the step after the clean ones must rebuild: replay it on the full
       // path (nothing of it ran), then speculate again in batches of about
@@ -1278,7 +1280,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     if (e == hipSuccess) e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
     // the slots hold what the failing and the halted steps folded in; the
     // replayed step rebuilds its lists, so only its own maxima are needed
-    if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSLOT * sizeof(BoxSlot));
+    if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSET * NSLOT * sizeof(BoxSlot));
     if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
     c->need_pre = true;
     c->spec_ready = false;
@@ -1752,7 +1754,7 @@ extern "C" int sdem_internal_after_update(sdem_ctx* c, int32_t steps) {
   for (int k = 0; k < 4; k++) B.num_cntcts[k] = 0;
   e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->BS, &B, sizeof(B), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(c->bslots, 0, NSLOT * sizeof(BoxSlot));
+  if (e == hipSuccess) e = hipMemset(c->bslots, 0, c->boxes.size() * NSET * NSLOT * sizeof(BoxSlot));
   if (e != hipSuccess) return hip_fail(c, e, "surrogate update");
   c->need_pre = true;
   c->spec_ready = false;  // the next step rebuilds

@@ -105,11 +105,16 @@ struct Params {
 };
 
 // Partial results of the per-particle kernels, accumulated with atomics
-// spread over NSLOT addresses per box (block b uses slot b % NSLOT) so that
+// spread over NSLOT addresses per box (wave w uses slot w % NSLOT) so that
 // no single address takes thousands of atomics; reduced by the next step's
-// control kernels and by k_finish.
-constexpr int NSLOT = 64;
-struct BoxSlot {                 // one box (NSLOT of them per box)
+// control and by k_finish.  There are two sets of NSLOT slots per box,
+// numbered like the two state buffers: a step's force kernel adds into the
+// set of the buffer it writes, and the next step's control reduces that
+// set (the one of its input buffer) and clears the other for its own force
+// kernel, so that the reads and the clearing never meet in one kernel.
+// Set s of every box: bslots[s * nbox * NSLOT + box * NSLOT + slot].
+constexpr int NSLOT = 64, NSET = 2;
+struct BoxSlot {                 // one slot (NSLOT per box and set)
   unsigned long long max_v_bits; // near-wall list: max |v|, max |a| (as bits of
   unsigned long long max_a_bits; // non-negative doubles)
   unsigned long long step_bits;  // largest distance a particle moved in the step
@@ -160,6 +165,8 @@ struct Scalars {
   int list_need;           // largest candidate count seen when a list overflowed
   int spec_fail;           // a step queued without the rebuild kernels needed them:
                            // it halted the call before doing anything
+  int cnt_over;            // a contact test found more contacts than history
+                           // slots (the force kernel raises overflow)
   int rv2_stored[2];       // with AB2 rotation, rotVel(2) sits in buffer b's rv
                            // array; 0 once the force kernel wrote the buffer:
                            // rotVel(2) is then linVel(1) of the same buffer
@@ -175,7 +182,7 @@ struct Dev {
   int64_t nkeys;         // sort keys of all boxes (cells + one last key each)
   Scalars* S;
   BoxState* BS;
-  BoxSlot* bslots;       // NSLOT per box
+  BoxSlot* bslots;       // NSET sets of NSLOT per box, set-major
   int *keys, *rank, *counts, *start, *tile_sum, *perm_tmp, *perm;
   const int2* tile_box;  // first and last box whose keys a scan tile covers
   double4* pk;           // positions in cell order (rebuild steps)

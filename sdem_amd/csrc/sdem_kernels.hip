@@ -12,6 +12,8 @@
 //                             only; later steps get them from k_force)
 //   k_ctl                   : one wave: decides the rebuild, runs the
 //                             near-wall cadence and advances iterNumber
+//                             (on steps queued without the rebuild
+//                             kernels, the first wave of k_contact does it)
 //   -- rebuild steps only (the kernels return at once otherwise) --
 //   k_bin                   : (level, cell) key per particle, per-key counts
 //   k_scan_tiles/_sums/_add : exclusive scan of the counts
@@ -327,17 +329,18 @@ __global__ __launch_bounds__(256) void k_pre(const Params* __restrict__ PB, cons
   }
 }
 
-// Per-box part of the step control, one wave: reduces the box's slots (the
-// last step's largest move, deletions, max|v|, max|a|), clears them for the
-// step, decides whether the box is re-sorted and its candidate lists rebuilt
+// Per-box part of the step control, one wave: reduces the box's slots `rd`
+// (the last step's largest move, deletions, max|v|, max|a|), clears the
+// other set `wr` for this step's force kernel, decides whether the box is re-sorted and its candidate lists rebuilt
 // (forced, a particle was removed, or the summed largest moves since the
 // last rebuild may exceed half the skin: the exact contact test runs every
 // step, so the decision never changes results), and runs the near-wall
 // cadence (Prtcl_ContactSearchPW.f90:201-229) at iterNumber `iter`.
 // Returns the decision on lane 0.
-__device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl, bool forced) {
-  const int lane = threadIdx.x;
-  BoxSlot& m = sl[lane];
+__device__ bool box_control(const Params& P, int iter, BoxState& B, const BoxSlot* rd, BoxSlot* wr,
+                            bool forced) {
+  const int lane = threadIdx.x & 63;
+  const BoxSlot& m = rd[lane];
   double mv = bitsd(m.max_v_bits), ma = bitsd(m.max_a_bits), step = bitsd(m.step_bits);
   int ndel = m.ndel;
   for (int off = 32; off > 0; off >>= 1) {
@@ -346,11 +349,12 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
     step = fmax(step, __shfl_down(step, off, 64));
     ndel += __shfl_down(ndel, off, 64);
   }
-  m.max_v_bits = 0ull;
-  m.max_a_bits = 0ull;
-  m.step_bits = 0ull;
-  m.ndel = 0;
-  for (int q = 0; q < 4; q++) m.cnt[q] = 0;
+  BoxSlot& z = wr[lane];
+  z.max_v_bits = 0ull;
+  z.max_a_bits = 0ull;
+  z.step_bits = 0ull;
+  z.ndel = 0;
+  for (int q = 0; q < 4; q++) z.cnt[q] = 0;
   if (lane != 0) return false;
   // |x_now - x_rebuild| <= sum of the steps' largest moves (triangle inequality)
   const double disp = B.disp_acc + step;
@@ -371,15 +375,12 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
     const double gn = sqrt(P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2]);
     max_a = max_a > gn ? max_a : gn;
     max_a = max_a * 1.5;
-    int num_iter;
-    if (max_a > 0.0) {
-      const double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
-      const double q = t / P.dt;
-      num_iter = q >= 2147483647.0 ? 2147483647 : (int)q;
-    } else {
-      num_iter = P.wall_max_update_iter;
-    }
-    if (num_iter < 1) num_iter = 1;
+    // t is 0/0 (NaN) when nothing accelerates and there is no gravity.
+    // int() of NaN, or of 2^31 or more, is the most negative integer with the
+    // reference's compilers on x86, which max(., 1) turns into one step.
+    const double t = (sqrt(max_v * max_v + 2.0 * max_a * P.dxmax1) - max_v) / max_a;
+    const double q = t / P.dt;
+    int num_iter = (q >= 1.0 && q < 2147483648.0) ? (int)q : 1;
     if (num_iter > P.wall_max_update_iter) num_iter = P.wall_max_update_iter;
     B.next_wall_update = iter + num_iter;
   }
@@ -387,30 +388,38 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, BoxSlot* sl,
   return rebuild;
 }
 
-// Step control, one wave: advances iterNumber (Prtcl_DEMSystem.f90:248) and
-// decides whether this step is forced to rebuild every box; with a single
-// box it also runs that box's control.  After a capacity overflow it only
-// marks the call halted, so every later kernel of the call does nothing and
-// the failing step's input buffer stays intact for the replay; every halted
-// step is counted so that the host knows how many steps completed.
-// `fast` (single box): the step was queued without the sort and list
-// kernels; if the box must rebuild, the step halts the call before any
-// side effect (spec_fail), and the host replays it with them.
-__global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
-                                            BoxSlot* bslots, int nbox, int fast) {
-  const int lane = threadIdx.x;
-  if (S->halt || S->overflow) {
+// Step control, one wave (lanes 0-63 of the calling block): advances
+// iterNumber (Prtcl_DEMSystem.f90:248) and decides whether this step is
+// forced to rebuild every box; with a single box it also runs that box's
+// control.  After a capacity overflow it only marks the call halted, so
+// every later kernel of the call does nothing and the failing step's input
+// buffer stays intact for the replay; every halted step is counted so that
+// the host knows how many steps completed.  `fast` (single box): the step
+// was queued without the sort and list kernels; if the box must rebuild,
+// the step halts the call before its force kernel (spec_fail), and the host
+// replays it with them.  Returns on every lane whether the step goes ahead.
+// The flags are read by lane 0 alone, which also writes them.
+__device__ bool step_control(const Params* __restrict__ PB, Scalars* S, BoxState* BS, const BoxSlot* rd,
+                             BoxSlot* wr, int nbox, int fast) {
+  const int lane = threadIdx.x & 63;
+  int flags = 0, iter = 0;
+  if (lane == 0) {
+    flags = (S->halt || S->overflow) ? 1 : 0;
+    flags |= (S->need_rebuild || !S->have_parts) ? 2 : 0;
+    iter = S->iter;
+  }
+  flags = __shfl(flags, 0, 64);
+  if (flags & 1) {
     if (lane == 0) {
       S->halt = 1;
       S->halted_steps++;
     }
-    return;
+    return false;
   }
-  const int iter = S->iter;
-  const bool forced = S->need_rebuild || !S->have_parts;
+  const bool forced = (flags & 2) != 0;
   if (fast) {
     // box_control's rebuild decision, read-only
-    const BoxSlot& m = bslots[lane];
+    const BoxSlot& m = rd[lane];
     double step = bitsd(m.step_bits);
     int ndel = m.ndel;
     for (int off = 32; off > 0; off >>= 1) {
@@ -425,10 +434,9 @@ __global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scala
         S->spec_fail = 1;
         S->halted_steps++;
       }
-      return;
+      return false;
     }
   }
-  __syncthreads();  // every lane has read the flags before lane 0 clears them
   if (lane == 0) {
     S->forced = forced ? 1 : 0;
     S->need_rebuild = 0;
@@ -438,27 +446,37 @@ __global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scala
     S->iter = iter + 1;  // iterNumber advances once per step
   }
   if (nbox == 1) {
-    const bool r = box_control(PB[0], iter, BS[0], bslots, forced);
+    const bool r = box_control(PB[0], iter, BS[0], rd, wr, forced);
     if (lane == 0 && r) {
       S->rebuild = 1;
       S->rebuilds++;
     }
   }
+  return true;
+}
+
+// The control of a step queued with every kernel (the fast steps run it in
+// k_contact).
+__global__ __launch_bounds__(64) void k_ctl(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
+                                            const BoxSlot* rd, BoxSlot* wr, int nbox) {
+  (void)step_control(PB, S, BS, rd, wr, nbox, 0);
 }
 
 // Per-box step control of an ensemble, one wave per box, after k_ctl.
 __global__ __launch_bounds__(64) void k_box(const Params* __restrict__ PB, Scalars* S, BoxState* BS,
-                                            BoxSlot* bslots) {
+                                            const BoxSlot* rd, BoxSlot* wr) {
   if (S->halt) return;
   const int box = blockIdx.x;
-  const bool r = box_control(PB[box], S->iter_at_begin, BS[box], bslots + (size_t)box * NSLOT, S->forced != 0);
+  const bool r = box_control(PB[box], S->iter_at_begin, BS[box], rd + (size_t)box * NSLOT,
+                             wr + (size_t)box * NSLOT, S->forced != 0);
   if (threadIdx.x == 0 && r) {
     atomicOr(&S->rebuild, 1);
     atomicAdd(&S->rebuilds, 1);
   }
 }
 
-// counters of the last step for the host (once per call), one wave per box
+// counters of the last step for the host (once per call), one wave per box,
+// from the slot set of the buffer the last step wrote
 __global__ __launch_bounds__(64) void k_finish(const Scalars* S, BoxState* BS, const BoxSlot* bslots) {
   if (S->halt || !S->have_parts) return;
   const int box = blockIdx.x;
@@ -939,42 +957,77 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // the partner's index in buffer A (moving) or in the fixed arrays, with the
 // fixed flag, and a flag for a periodic image.  A light kernel (few
 // registers), so that many waves hide the gather latency.
+//
+// `fast` (single box, a step queued without the sort and list kernels):
+// the kernel also runs the step's control in the first wave of block 0,
+// which saves a dependent launch per step.  No other wave reads what that
+// wave writes: the lists are those of the last rebuild (a fast step never
+// rebuilds: if the box must, the control halts the call and the host
+// replays the step; what the other waves wrote meanwhile is rewritten by
+// the replay), and each wave adds the last step's largest move (slot set
+// rd, which the control leaves alone) to the summed moves itself.  It
+// may read the summed moves before or after the control wave has added
+// the same move to them; after, it adds it twice, which only makes the gap
+// test below skip less.
 constexpr unsigned CL_SHIFT = 0x80000000u;
 
 __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ blk_box, Buf A, Scalars* S,
-                                                 const BoxState* BS,
+                                                 BoxState* BS,
                                                  const int* __restrict__ perm,
                                                  const int4* __restrict__ act,
                                                  const int* __restrict__ cand,
                                                  unsigned* __restrict__ clist,
                                                  int* __restrict__ ccnt,
-                                                 const double4* __restrict__ fpk) {
-  if (S->halt || S->overflow == 2) return;
+                                                 const double4* __restrict__ fpk,
+                                                 const BoxSlot* rd, BoxSlot* wr, int fast) {
   const int box = block_box(blk_box);
+  // summed largest moves since the last rebuild, before the control (if it
+  // runs below) adds the last step's
+  double disp = BS[box].disp_acc;
+  if (fast && blockIdx.x == 0 && threadIdx.x < 64)
+    if (!step_control(PB, S, BS, rd, wr, 1, 1)) return;
+  int stop = (S->halt || S->overflow == 2) ? 1 : 0;
+  if (fast) stop = __shfl(stop, 0, 64);  // one decision per wave: the whole wave reduces below
+  if (stop) return;
   const Params& P = PB[box];
   // packed slot t: a particle of the box with candidates, from either end of
   // the box's slots
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t - P.off >= BS[box].n_act && t - P.off < P.bcap - BS[box].n_far) return;
+  const bool listed = !(t - P.off >= BS[box].n_act && t - P.off < P.bcap - BS[box].n_far);
+  // a fast step's waves stay whole until they have reduced the move below
+  if (fast ? !__ballot(listed) : !listed) return;
   const size_t cap = P.cap;
   // the packed entry and the first CPF list entries, all in flight at once
   // (their addresses depend on t only), then every partner position of
   // those: two latencies for most particles instead of a chain
   constexpr int CPF = 4;
-  unsigned ent[CPF];
-  for (int q = 0; q < CPF; q++)
-    if (q < P.lcap) ent[q] = (unsigned)cand[(size_t)q * cap + t];
-  const int4 a = act[t];
+  unsigned ent[CPF] = {0u, 0u, 0u, 0u};
+  int4 a = make_int4(0, 0, 0, 0);
+  if (listed) {
+    for (int q = 0; q < CPF; q++)
+      if (q < P.lcap) ent[q] = (unsigned)cand[(size_t)q * cap + t];
+    a = act[t];
+  }
+  if (fast) {
+    // the last step's largest move, reduced by the whole wave while the
+    // loads above are in flight
+    double step = bitsd(rd[(size_t)box * NSLOT + (threadIdx.x & 63)].step_bits);
+    for (int off = 32; off > 0; off >>= 1) step = fmax(step, __shfl_xor(step, off, 64));
+    disp = disp + step;
+    if (!listed) return;
+  }
   const int k = a.x, nq = min(a.y, P.lcap);
-  const bool rebuilt = BS[box].rebuild != 0;
-  // No particle of the box has moved more than disp_acc since the lists
-  // were built, so no listed pair has closed its gap by more than twice
-  // that: while the smallest gap of the list (less a rounding margin) is
-  // larger, no candidate touches and nothing more is loaded.  disp_acc
-  // grows until the next rebuild, so the count stays the 0 written then.
+  const bool rebuilt = !fast && BS[box].rebuild != 0;
+  // No particle of the box has moved more than disp (the summed moves, or
+  // more) since the lists were built, so no listed pair has closed its gap
+  // by more than twice that: while the smallest gap of the list (less a
+  // rounding margin) is larger, no candidate touches and nothing more is
+  // loaded.  The summed moves only grow until the next rebuild, so a
+  // particle skipped now touched nothing at any step since the rebuild at
+  // which it was tested: its count is the 0 written then.
   const double gap = __hiloint2double(a.w, a.z);
-  if (gap > 2.0 * BS[box].disp_acc * (1.0 + 1.0e-9)) {
+  if (gap > 2.0 * disp * (1.0 + 1.0e-9)) {
     if (rebuilt) ccnt[k] = 0;
     return;
   }
@@ -1026,7 +1079,11 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     c++;
   }
   ccnt[k] = c;
-  if (c > P.hcap) atomicMax(&S->overflow, 1);
+  // More contacts than history slots: the force kernel of the step turns
+  // this into an overflow (the host grows the slots and replays the step).
+  // A fast step that its control halts runs no force kernel, so what its
+  // other waves found here never halts a call.
+  if (c > P.hcap) atomicMax(&S->cnt_over, 1);
 }
 
 // PIM / PRIM: translational / rotational integration method fixed at
@@ -1051,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
   const Params& P = PB[box];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool rebuilt = BS[box].rebuild != 0;  // this step re-sorted the box: A is read through perm
-  // k_ctl has advanced iterNumber: is the next step a near-wall update?
+  // the control has advanced iterNumber: is the next step a near-wall update?
   const bool upd_next = S->iter >= BS[box].next_wall_update;
   // id, type, mark and the near-wall list change only where the box is
   // re-sorted, a particle is removed or the lists are refreshed.  They are
@@ -1117,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     // perspective the reference evaluates it from (ContactForce_PP :732-756,
     // _PPG :761-798, _PPFix :803-827, _PW :832-856), so the routine is
     // inlined once.
-    const int nc = min(ccnt[k], P.hcap);
+    const int nc = min(ccnt[k], P.hcap);  // more: cnt_over is set
     const uint64_t mask0 = A.wmask[s];
     uint64_t mask = mask0;
     const int ntot = nc + __popcll(mask);
@@ -1366,6 +1423,8 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       S->have_parts = 1;
       S->rv2_stored[B.idx] = 0;
+      // this step's contact test found more contacts than history slots
+      if (S->cnt_over) atomicMax(&S->overflow, 1);
     }
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
@@ -1381,20 +1440,26 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 int scan_tile_keys() { return SCAN_TILE; }
 
+// slot set `set` (numbered like the state buffers) of every box
+static inline BoxSlot* slot_set(const Dev& D, int set) { return D.bslots + (size_t)set * D.nbox * NSLOT; }
+
 hipError_t launch_pre(const Dev& D, const Buf& A, hipStream_t st) {
   hipLaunchKernelGGL(k_pre, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box, A, D.S, D.BS,
-                     D.bslots);
+                     slot_set(D, A.idx));
   return hipGetLastError();
 }
 
-hipError_t launch_ctl(const Dev& D, hipStream_t st, bool fast) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots, D.nbox, (fast && D.nbox == 1) ? 1 : 0);
-  if (D.nbox > 1) hipLaunchKernelGGL(k_box, dim3(D.nbox), dim3(64), 0, st, D.PB, D.S, D.BS, D.bslots);
+hipError_t launch_ctl(const Dev& D, const Buf& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, D.PB, D.S, D.BS, slot_set(D, A.idx),
+                     slot_set(D, 1 - A.idx), D.nbox);
+  if (D.nbox > 1)
+    hipLaunchKernelGGL(k_box, dim3(D.nbox), dim3(64), 0, st, D.PB, D.S, D.BS, slot_set(D, A.idx),
+                       slot_set(D, 1 - A.idx));
   return hipGetLastError();
 }
 
-hipError_t launch_finish(const Dev& D, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish, dim3(D.nbox), dim3(64), 0, st, D.S, D.BS, D.bslots);
+hipError_t launch_finish(const Dev& D, const Buf& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish, dim3(D.nbox), dim3(64), 0, st, D.S, D.BS, slot_set(D, A.idx));
   return hipGetLastError();
 }
 
@@ -1423,16 +1488,18 @@ hipError_t launch_vlist(const Dev& D, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid) {
+hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid, bool fast) {
   hipLaunchKernelGGL(k_contact, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                     D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk);
+                     D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk, slot_set(D, A.idx),
+                     slot_set(D, B.idx),
+                     (fast && D.nbox == 1) ? 1 : 0);
   if (mid) (void)hipEventRecord(mid, st);
   if (D.pim == 2 && D.prim == 2)
     hipLaunchKernelGGL((k_force<2, 2>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
-                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
+                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
   else
     hipLaunchKernelGGL((k_force<0, 0>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
-                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, D.bslots);
+                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
   return hipGetLastError();
 }
 

@@ -115,6 +115,50 @@ def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
     assert not errs, errs
 
 
+def test_history_overflow_on_a_fast_step(hip_lib, oracle_lib):
+    """A third sphere closes on a pair in contact while the candidate lists
+    stay valid (skin 0.4 d, moves of 5e-8 m per step), so the middle
+    sphere's second contact exceeds its one history slot on a step queued
+    without the rebuild kernels, whose control runs in the contact test:
+    the step halts, the slots grow and it is replayed from its input.  The
+    state is the oracle's every 10 steps."""
+    print("history overflow on a fast step: 3 spheres, hist_slots=1", flush=True)
+    d = 2.0e-3
+    sc = scenarios.two_spheres(d=d, v=0.01, gap=-0.5e-6, dt=1.0e-6)
+    st = sc["state"]
+    x2 = st["posr"][1, 0]
+    posr = np.vstack([st["posr"], [[x2 + d + 3.0e-6, st["posr"][1, 1], st["posr"][1, 2], d / 2]]])
+    lv1 = np.array([[0.01, 0, 0], [-0.01, 0, 0], [-0.05, 0, 0]])
+    sc["state"] = scenarios.zero_state([1, 2, 3], [1, 1, 1], posr, lin_vel1=lv1)
+    sc["options"]["num_prtcl"] = 3
+    gpu = native.build_engine(hip_lib, sc, tuning=dict(skin=0.4), capacity=dict(hist_slots=1))
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    bytes0 = gpu.device_bytes()
+    for _ in range(10):
+        gpu.iterate(10)
+        ref.iterate(10)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-10)
+        assert not errs, (gpu.get_iter(), errs)
+    ct = ref.get_contacts()
+    assert np.sum(ct["kind"] != native.KIND_PW) >= 2, "the third sphere never touched"
+    assert gpu.device_bytes() > bytes0, "no history slot was added"
+
+
+def test_wall_cadence_without_acceleration(hip_lib, oracle_lib):
+    """Two spheres apart and at rest, no gravity: the cadence formula is
+    0/0 and the near-wall lists are refreshed at every step, as the
+    oracle does (tests/test_oracle_pinning.py pins why)."""
+    print("near-wall cadence with nothing accelerating: 2 spheres, 12 steps", flush=True)
+    sc = scenarios.two_spheres(v=0.0, gap=1.0e-3)
+    gpu, ref = engines(sc, hip_lib, oracle_lib)
+    for k in (1, 3, 8):
+        gpu.iterate(k)
+        ref.iterate(k)
+        assert gpu.get_iter() == ref.get_iter(), (gpu.get_iter(), ref.get_iter())
+        errs = compare(gpu, ref)
+        assert not errs, errs
+
+
 def test_two_runs_are_bitwise_identical(hip_lib):
     print("determinism: n=2000, 25 steps twice", flush=True)
     sc = scenarios.random_cloud(n=2000, seed=8, periodic=(1, 0, 0), walls="yYzZ")

@@ -340,3 +340,18 @@ def test_open_boundary_deletes_escaped_particles(oracle_lib):
     e.iterate(1)
     ids = e.get_state()["id"]
     assert st["id"][top] not in ids and len(ids) == 49
+
+
+def test_wall_cadence_without_acceleration(oracle_lib):
+    """No gravity and nothing accelerating: the cadence formula's
+    t = (sqrt(v^2 + 2 a dx) - v) / a is 0/0 (Prtcl_ContactSearchPW.f90:
+    223-225), and int() of NaN is the most negative integer on x86 (the
+    truncating conversion numpy's cast uses as well), so numIter =
+    max(int(t/dt), 1) = 1: the near-wall lists are refreshed at every step."""
+    with np.errstate(invalid="ignore"):
+        assert np.array([np.nan, 2.0 ** 31]).astype(np.int32).tolist() == [np.iinfo(np.int32).min] * 2
+    sc = scenarios.two_spheres(v=0.0, gap=1.0e-3)  # apart and at rest, no gravity
+    e = make(oracle_lib, sc)
+    for k in range(1, 6):
+        e.iterate(1)
+        assert e.get_iter() == (k, k), (k, e.get_iter())
