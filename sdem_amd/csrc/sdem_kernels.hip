@@ -958,7 +958,8 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // fixed flag, and a flag for a periodic image.  A light kernel (few
 // registers), so that many waves hide the gather latency.
 //
-// `fast` (single box, a step queued without the sort and list kernels):
+// FAST (single box, a step queued without the sort and list kernels; an
+// instantiation of its own, so that the other one compiles as before):
 // the kernel also runs the step's control in the first wave of block 0,
 // which saves a dependent launch per step.  No other wave reads what that
 // wave writes: the lists are those of the last rebuild (a fast step never
@@ -971,6 +972,7 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // test below skip less.
 constexpr unsigned CL_SHIFT = 0x80000000u;
 
+template <int FAST>
 __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ blk_box, Buf A, Scalars* S,
                                                  BoxState* BS,
@@ -980,15 +982,15 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  unsigned* __restrict__ clist,
                                                  int* __restrict__ ccnt,
                                                  const double4* __restrict__ fpk,
-                                                 const BoxSlot* rd, BoxSlot* wr, int fast) {
+                                                 const BoxSlot* rd, BoxSlot* wr) {
   const int box = block_box(blk_box);
-  // summed largest moves since the last rebuild, before the control (if it
-  // runs below) adds the last step's
-  double disp = BS[box].disp_acc;
-  if (fast && blockIdx.x == 0 && threadIdx.x < 64)
+  // FAST: summed largest moves since the last rebuild, before the control
+  // adds the last step's
+  double disp = FAST ? BS[box].disp_acc : 0.0;
+  if (FAST && blockIdx.x == 0 && threadIdx.x < 64)
     if (!step_control(PB, S, BS, rd, wr, 1, 1)) return;
   int stop = (S->halt || S->overflow == 2) ? 1 : 0;
-  if (fast) stop = __shfl(stop, 0, 64);  // one decision per wave: the whole wave reduces below
+  if (FAST) stop = __shfl(stop, 0, 64);  // one decision per wave: the whole wave reduces below
   if (stop) return;
   const Params& P = PB[box];
   // packed slot t: a particle of the box with candidates, from either end of
@@ -996,7 +998,7 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool listed = !(t - P.off >= BS[box].n_act && t - P.off < P.bcap - BS[box].n_far);
   // a fast step's waves stay whole until they have reduced the move below
-  if (fast ? !__ballot(listed) : !listed) return;
+  if (FAST ? !__ballot(listed) : !listed) return;
   const size_t cap = P.cap;
   // the packed entry and the first CPF list entries, all in flight at once
   // (their addresses depend on t only), then every partner position of
@@ -1009,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       if (q < P.lcap) ent[q] = (unsigned)cand[(size_t)q * cap + t];
     a = act[t];
   }
-  if (fast) {
+  if (FAST) {
     // the last step's largest move, reduced by the whole wave while the
     // loads above are in flight
     double step = bitsd(rd[(size_t)box * NSLOT + (threadIdx.x & 63)].step_bits);
@@ -1018,7 +1020,8 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     if (!listed) return;
   }
   const int k = a.x, nq = min(a.y, P.lcap);
-  const bool rebuilt = !fast && BS[box].rebuild != 0;
+  const bool rebuilt = !FAST && BS[box].rebuild != 0;
+  if (!FAST) disp = BS[box].disp_acc;  // the control kernel has added the last step's move
   // No particle of the box has moved more than disp (the summed moves, or
   // more) since the lists were built, so no listed pair has closed its gap
   // by more than twice that: while the smallest gap of the list (less a
@@ -1489,10 +1492,14 @@ hipError_t launch_vlist(const Dev& D, hipStream_t st) {
 }
 
 hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid, bool fast) {
-  hipLaunchKernelGGL(k_contact, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                     D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk, slot_set(D, A.idx),
-                     slot_set(D, B.idx),
-                     (fast && D.nbox == 1) ? 1 : 0);
+  if (fast && D.nbox == 1)
+    hipLaunchKernelGGL(k_contact<1>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
+                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                       slot_set(D, A.idx), slot_set(D, B.idx));
+  else
+    hipLaunchKernelGGL(k_contact<0>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
+                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                       slot_set(D, A.idx), slot_set(D, B.idx));
   if (mid) (void)hipEventRecord(mid, st);
   if (D.pim == 2 && D.prim == 2)
     hipLaunchKernelGGL((k_force<2, 2>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
