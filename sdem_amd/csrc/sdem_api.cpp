@@ -1159,8 +1159,10 @@ static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out)
 // Steps in batches.  With graphs and a single box, steps are queued on the
 // fast path (no sort and list kernels) in batches of spec_chunk, growing
 // while they run clean; when a step finds that its box must rebuild, it
-// halts the batch before any side effect, the host replays it on the full
-// path and speculation resumes after it.  Results do not depend on the
+// halts the batch before its force kernel (the state buffers and the step
+// counters are untouched; contact lists its contact test wrote are written
+// again by the replay), the host replays it on the full path and
+// speculation resumes after it.  Results do not depend on the
 // path: the same criterion decides every rebuild.  A capacity overflow
 // halts a batch too: the capacity grows and the failing step is replayed.
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
@@ -1249,8 +1251,8 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     s.cnt_over = 0;
     if (s.spec_fail) {
       // the step after the clean ones must rebuild: replay it on the full
-      // path (nothing of it ran), then speculate again in batches of about
-      // the run just seen
+      // path (its force kernel did not run), then speculate again in
+      // batches of about the run just seen
       s.spec_fail = 0;
       e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
       if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
