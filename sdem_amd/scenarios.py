@@ -285,26 +285,40 @@ def sandbox(n=10_000, seed=DEFAULT_SEED, box=(0.05, 0.05, 0.2), ntypes=11, jitte
     return sc
 
 
+def incline_quad(L: float, W: float, angle_deg: float = 30.0):
+    """The inclined plane of the three-size box: a quad rising from the
+    floor edge x = 0 along +x at `angle_deg` over half the box length L,
+    across the full width W, facing up.  Its upper edge (x = L/2,
+    z = L/2 tan angle) is free, so spheres resting on that edge are
+    touched through the edge test of PW_isInContact
+    (Prtcl_Geometry.f90:89-104), not the plane test."""
+    xr = 0.5 * L
+    zr = xr * math.tan(math.radians(angle_deg))
+    return dict(p1=(0.0, 0.0, 0.0), p2=(xr, 0.0, zr), p3=(xr, W, zr), p4=(0.0, W, 0.0), user_id=7,
+                wall_type=1, both_side=False, is_infinite=False, trans_vel=(0.0, 0.0, 0.0))
+
+
 def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
     """Three size classes 1 / 2.5 / 5 mm (number fractions 0.85/0.12/0.03) in a
     closed box with a 30-degree plane; Hertz-Mindlin, mu=0.5, dt=2e-6 s,
     hierarchical search.  The box keeps the 1:1:2 aspect of 0.3x0.3x0.6 m,
     sized so the pack fits at fraction `phi` in its lower 80 %, and is
     raised by the incline's height: the pack starts above the incline (the
-    plane rises along +x to half the box width times tan 30 degrees) and
-    collapses onto it."""
+    plane rises at 30 degrees along +x over the first half of the floor,
+    `incline_quad`) and collapses onto it."""
     rng = np.random.default_rng(seed)
     centers = np.array([1.0e-3, 2.5e-3, 5.0e-3])
     t = rng.choice(3, size=n, p=[0.85, 0.12, 0.03])
     dq = centers[t]
     vol_solid = float(np.sum(np.pi / 6 * dq ** 3))
+    tan30 = math.tan(math.radians(30.0))
     if box is None:
         s = (vol_solid / phi / 0.8 / 2.0) ** (1 / 3)
-        zr0 = s * math.tan(math.radians(30.0)) * 0.5
-        box = (s, s, 2 * s + zr0)
+        box = (s, s, 2 * s + 0.5 * s * tan30)
     box = np.asarray(box, float)
     L = box[0]
-    zr = L * math.tan(math.radians(30.0)) * 0.5
+    ramp = incline_quad(L, box[1])
+    zr = ramp["p2"][2]
     height = vol_solid / phi / (box[0] * box[1])
     z0 = zr + centers.max()
     pos = rsa_pack(dq / 2, (0, 0, z0), (box[0], box[1], min(z0 + height, box[2] * 0.999)), seed)
@@ -313,9 +327,7 @@ def polydisperse(n=4_000_000, seed=DEFAULT_SEED, phi=0.2, box=None) -> Dict:
     ids = np.arange(1, n + 1, dtype=np.int32)
     st = zero_state(ids, t + 1, np.concatenate([pos, dq[:, None] / 2], 1))
     walls = box_walls((0, 0, 0), box)
-    # 30-degree incline rising along +x from the bottom, facing up
-    walls.append(dict(p1=(0, 0, 0), p2=(L, 0, zr), p3=(L, box[1], zr), p4=(0, box[1], 0), user_id=7,
-                      wall_type=1, both_side=False, is_infinite=False, trans_vel=(0.0, 0.0, 0.0)))
+    walls.append(ramp)
     opts = base_options(dt=2.0e-6, domain_max=tuple(box), num_prtcl=n, cf_type=CFT_NLIN,
                         cs_method=CSM_NBS_MUNJIZA_HRCHL)
     return dict(options=opts,
@@ -378,3 +390,60 @@ def ensemble(nbox: int = 256, seed: int = DEFAULT_SEED, nmin: int = 2000, nmax: 
         out.append(ensemble_box(n, seed + 7919 * (b + 1), model="linear" if b % 2 == 0 else "hm",
                                 fric=fric, rest=rest, kn=kn))
     return out
+
+
+# --------------------------------------------------------------------------
+# contact-bearing starts
+# --------------------------------------------------------------------------
+def _with_start(sc: Dict, state: Dict, contacts: Dict) -> Dict:
+    out = dict(sc)
+    out["state"] = state
+    out["contacts"] = contacts
+    out["options"] = dict(sc["options"])
+    return out
+
+
+def settle(sc, steps: int, lib=None, dt: Optional[float] = None, gravity_scale: float = 1.0,
+           chunk: int = 5000, log=None, tuning: Optional[dict] = None):
+    """A contact-bearing start for a scenario (or a list of scenarios, stepped
+    together as one ensemble): the pack is advanced `steps` steps on the
+    product's GPU path with the scenario's own particle types, materials,
+    walls and contact model, and the state and contact records (tangential
+    history included) at the end replace the scenario's start.  `dt`
+    (default: the scenario's own) and `gravity_scale` apply to these
+    steps only -- a configuration whose own time step resolves a contact in
+    two or three steps cannot come to rest under Adams-Bashforth, and a
+    stronger pull only shortens the fall.  The returned scenario keeps its
+    own options, so it is stepped with its own dt and gravity from there."""
+    from . import native
+
+    lib = lib or native.load_hip_library()
+    many = isinstance(sc, (list, tuple))
+    scs = list(sc) if many else [sc]
+    prep = []
+    for s in scs:
+        p = dict(s)
+        o = dict(s["options"])
+        if dt is not None:
+            o["dt"] = dt
+        o["gravity"] = tuple(gravity_scale * np.asarray(o.get("gravity", (0.0, 0.0, -9.81)), float))
+        p["options"] = o
+        prep.append(p)
+    tune = dict(keep_forces=0)
+    tune.update(tuning or {})
+    eng = native.build_ensemble(lib, prep, tuning=tune) if many else native.build_engine(lib, prep[0], tuning=tune)
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        eng.iterate(k)
+        done += k
+        if log:
+            log(f"settle: {done}/{steps} steps, contacts {eng.get_counts().tolist()}")
+    out = []
+    for b, s in enumerate(scs):
+        if many:
+            eng.select(b)
+        st, ct = eng.get_state(), eng.get_contacts()
+        out.append(_with_start(s, st, ct))
+    eng.close()
+    return out if many else out[0]
