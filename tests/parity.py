@@ -8,6 +8,8 @@ plus the largest size of that quantity in the system (summation order of
 contact forces differs between the two implementations, and a resting
 particle's net force is a small difference of large contact forces).
 """
+import math
+
 import numpy as np
 
 from sdem_amd import native
@@ -101,3 +103,33 @@ def aging_lists_scene(seed=31, speed=3.0):
     st["lin_acc"][:, :, :] = 0.0
     return sc
 
+
+
+# finite plane quads for the wall-geometry tests
+def _rect(origin, eu, ev):
+    o, eu, ev = (np.asarray(a, float) for a in (origin, eu, ev))
+    return [o, o + eu, o + eu + ev, o + ev]
+
+
+def _rotation(seed):
+    q = np.random.default_rng(seed).normal(size=4)
+    a, b, c, d = q / np.linalg.norm(q)
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def quads():
+    """(corners, both_side): an axis-aligned quad, a 30-degree incline, the
+    incline two-sided, and a rectangle turned by a random rotation."""
+    c30, s30 = math.cos(math.radians(30)), math.sin(math.radians(30))
+    R = _rotation(7)
+    out = [
+        (_rect((0.010, 0.020, 0.010), (0.030, 0.0, 0.0), (0.0, 0.015, 0.0)), False),
+        (_rect((0.010, 0.010, 0.010), (0.028 * c30, 0.0, 0.028 * s30), (0.0, 0.020, 0.0)), False),
+        (_rect((0.010, 0.010, 0.010), (0.028 * c30, 0.0, 0.028 * s30), (0.0, 0.020, 0.0)), True),
+    ]
+    ctr = np.array([0.025, 0.025, 0.025])
+    eu, ev = R @ np.array([0.022, 0.0, 0.0]), R @ np.array([0.0, 0.016, 0.0])
+    out.append((_rect(ctr - 0.5 * eu - 0.5 * ev, eu, ev), False))
+    return out

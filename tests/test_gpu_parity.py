@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from parity import aging_lists_scene, compare
+from parity import aging_lists_scene, compare, quads
 from sdem_amd import native, scenarios
 
 pytestmark = pytest.mark.gpu
@@ -235,3 +235,68 @@ def test_config_parity(name, hip_lib, oracle_lib):
         ref.iterate(k)
         errs = compare(gpu, ref, exact_positions=False, rtol=1e-11)
         assert not errs, errs
+
+
+def quad_edge_scene(seed=23, n=400):
+    """Spheres around the edges and corners of three finite quads (a
+    30-degree incline, the same incline two-sided, a rectangle turned by a
+    random rotation), on the front side within a radius of the plane, with
+    random velocities and spins: many of them touch a quad through the edge
+    and corner tests of PW_isInContact (Line_point_check,
+    Prtcl_Geometry.f90:226-261)."""
+    rng = np.random.default_rng(seed)
+    qs = quads()[1:]
+    pos, rad = [], []
+    for i in range(n):
+        corners, both = qs[i % len(qs)]
+        p1, p2, p3, p4 = corners
+        nrm = np.cross(p2 - p1, p3 - p1)
+        nrm /= np.linalg.norm(nrm)
+        eu, ev = p2 - p1, p4 - p1
+        r = rng.choice([0.6e-3, 0.9e-3])
+        side = rng.integers(4)
+        s = rng.uniform(-0.05, 1.05)
+        ou = rng.uniform(-1.2, 1.2) * r / np.linalg.norm(eu)  # off edge 2 / 4, in units of the side
+        ov = rng.uniform(-1.2, 1.2) * r / np.linalg.norm(ev)  # off edge 1 / 3
+        u, v = [(s, ov), (1 + ou, s), (s, 1 + ov), (ou, s)][side]
+        h = rng.uniform(0.05, 0.95) * r * (rng.choice([-1, 1]) if both else 1)
+        pos.append(p1 + u * eu + v * ev + h * nrm)
+        rad.append(r)
+    posr = np.concatenate([np.array(pos), np.array(rad)[:, None]], 1)
+    types = np.where(posr[:, 3] < 0.7e-3, 1, 2)
+    st = scenarios.zero_state(np.arange(1, n + 1), types, posr)
+    st["lin_vel"][:, 0, :] = rng.uniform(-0.2, 0.2, (n, 3))
+    st["rot_vel"][:, 0, :] = rng.uniform(-100, 100, (n, 3))
+    walls = [dict(p1=tuple(c[0]), p2=tuple(c[1]), p3=tuple(c[2]), p4=tuple(c[3]), user_id=k + 1, wall_type=1,
+                  both_side=both, is_infinite=False, trans_vel=(0.0, 0.0, 0.0)) for k, (c, both) in enumerate(qs)]
+    return dict(options=scenarios.base_options(dt=1.0e-6, domain_max=(0.05, 0.05, 0.05), num_prtcl=n),
+                ptypes=dict(diameter=[1.2e-3, 1.8e-3], density=[2500.0] * 2, youngs=[5.0e7] * 2, poisson=[0.3] * 2),
+                pp=(0.4, 0.3, 0.05, 0.8, 0.8), wtypes=dict(youngs=[5.0e7], poisson=[0.3]),
+                pw=(0.45, 0.35, 0.05, 0.7, 0.7), walls=walls, state=st), qs
+
+
+def test_finite_quad_edges_and_corners(hip_lib, oracle_lib):
+    """Wall contacts through the edge and corner tests of finite quads
+    (tilted, two-sided, rotated): the contact set and the state equal the
+    oracle's after 1 and 10 steps, and most wall records are off the quad's
+    face (their centre projects outside the quad)."""
+    print("finite quads: 400 spheres around edges and corners, 1 + 9 steps", flush=True)
+    sc, qs = quad_edge_scene()
+    gpu, ref = engines(sc, hip_lib, oracle_lib)
+    for k, exact, rtol in ((1, True, 1e-12), (9, False, 1e-11)):
+        gpu.iterate(k)
+        ref.iterate(k)
+        errs = compare(gpu, ref, exact_positions=exact, rtol=rtol)
+        assert not errs, (k, errs)
+    ct, st = ref.get_contacts(), ref.get_state()
+    off_face = 0
+    for (kind, owner, partner) in zip(ct["kind"], ct["owner"], ct["partner"]):
+        if kind != native.KIND_PW:
+            continue
+        p1, p2, p3, p4 = qs[partner - 10000001][0]
+        p = st["posr"][np.searchsorted(st["id"], owner), :3]
+        eu, ev = p2 - p1, p4 - p1
+        u, v = (p - p1) @ eu / (eu @ eu), (p - p1) @ ev / (ev @ ev)
+        off_face += not (0 <= u <= 1 and 0 <= v <= 1)
+    print(f"  wall records {int(np.sum(ct['kind'] == native.KIND_PW))}, off the face {off_face}", flush=True)
+    assert off_face > 50
