@@ -9,8 +9,12 @@ dt=5e-6 s, AB2.  The state is uploaded once; the timed region runs K steps
 with everything resident in HBM.
 
 Prints one JSON line: particle-steps/s, the roofline of the dominant kernel
-(the fused step kernel) and a CPU baseline (the C restatement of the
-reference step on one host core, on a bounded sample of the same workload).
+(the fused step kernel) and of the whole step, and a CPU baseline (the C
+restatement of the reference step on every host core at once, one thread
+per bounded sample of the same workload).  Under "contact_bed": the same bed
+settled by the GPU path into a resting pack (about 2.3 contact records per
+sphere) and stepped with the same options, with its own roofline and CPU
+baseline.
 """
 from __future__ import annotations
 
@@ -59,9 +63,15 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick ~15 s of CPU work")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("settling", "sandbox", "polydisperse", "ensemble", "surrogate"),
+    ap.add_argument("--no-contact", action="store_true", help="leave out the settled bed in contact")
+    ap.add_argument("--contact-steps", type=int, default=200, help="timed steps of the bed in contact")
+    ap.add_argument("--contact-state", default=None,
+                    help=".npz of the settled bed: read if it exists, else written after the settle")
+    ap.add_argument("--workload", choices=("settling", "contact", "sandbox", "polydisperse", "ensemble",
+                                           "surrogate"),
                     default="settling",
-                    help="settling: the 1M-sphere bed (the headline line); sandbox: the 10k-sphere sand "
+                    help="settling: the 1M-sphere bed (the headline line); contact: the same bed settled "
+                         "into contact, alone; sandbox: the 10k-sphere sand "
                          "box; polydisperse: 4M spheres of three sizes on an incline; ensemble: 256 "
                          "independent sand boxes stepped together; surrogate: the learned surrogate "
                          "trained on DEM trajectories, then against the DEM on the 1M-sphere bed")
@@ -72,28 +82,192 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(args):
-    """The oracle (C restatement of the reference step) on one host core, on
-    a seeded 20k-particle box of the same workload (same pack fraction,
-    size distribution and physics; xy area scaled to the particle count)."""
+def host_threads() -> int:
+    """Host threads the CPU baselines use: the cores this process may run on,
+    at most 16 (one GPU's share of a GPU box's cores)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_rate(scenarios_, steps_for, target_s=15.0):
+    """The oracle (C restatement of the reference step) on one thread per
+    scenario, all at once: each thread steps an independent oracle context
+    (the library has no global state and ctypes releases the interpreter
+    lock during a call).  Returns (particle-steps/s summed over the threads,
+    steps per thread)."""
+    import threading
+
     import oracle
-    from sdem_amd import native, scenarios
+    from sdem_amd import native
+
+    lib = oracle.load()
+    engs = [native.build_engine(lib, sc, prefix=oracle.PREFIX) for sc in scenarios_]
+    for e in engs:
+        e.iterate(2)
+    t0 = time.perf_counter()
+    engs[0].iterate(3)
+    per_step = (time.perf_counter() - t0) / 3
+    steps = steps_for or max(5, int(target_s / max(per_step, 1e-6)))
+    ns = [e.num_particles() for e in engs]
+    barrier = threading.Barrier(len(engs) + 1)
+
+    def run(e):
+        barrier.wait()
+        e.iterate(steps)
+
+    th = [threading.Thread(target=run, args=(e,)) for e in engs]
+    for t in th:
+        t.start()
+    barrier.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    return sum(ns) * steps / wall, steps
+
+
+def cpu_baseline(args):
+    """The oracle on every host core at once (one thread each, at most 16),
+    each on its own seeded 20k-sphere box of the headline workload (same
+    pack fraction, size distribution and physics; xy area scaled to the
+    particle count)."""
+    from sdem_amd import scenarios
+
+    nth = host_threads()
+    n = args.cpu_sample
+    scs = [scenarios.settling(n=n, seed=args.seed + 101 * k) for k in range(nth)]
+    rate, steps = oracle_rate(scs, args.cpu_steps)
+    return {"value": rate, "unit": "particle-steps/s", "cores": nth, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{nth} independent settling beds of {n} spheres (seeds {args.seed} + 101 k; same "
+                      f"distribution, phi=0.3, xy area scaled), {steps} steps each after 5 warm-up steps, "
+                      f"oracle, one thread per bed"}
+
+
+# ---------------------------------------------------------------------------
+# the settling bed in contact
+# ---------------------------------------------------------------------------
+# the settle: 60,000 steps at twice gravity (the fall from the loose pack),
+# then 12,000 at gravity, all with the configuration's own physics and dt
+CONTACT_SETTLE = ((60_000, 2.0), (12_000, 1.0))
+
+
+def settled_bed(n, seed, log, state_path=None):
+    """The 1M-sphere settling bed (configuration 2's physics, time step and
+    walls) settled into a resting pack by the GPU path itself
+    (`scenarios.settle`, CONTACT_SETTLE); with `state_path` the settled
+    state and contact records are read from / written to that .npz file."""
+    from sdem_amd import scenarios
+
+    sc = scenarios.settling(n=n, seed=seed)
+    if state_path and os.path.exists(state_path):
+        z = np.load(state_path)
+        sc["state"] = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+        sc["contacts"] = {k[3:]: z[k] for k in z.files if k.startswith("ct_")}
+        return sc, 0.0
+    t0 = time.perf_counter()
+    for steps, g in CONTACT_SETTLE:
+        sc = scenarios.settle(sc, steps, gravity_scale=g, chunk=steps // 4, log=log,
+                              tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN))
+    t_settle = time.perf_counter() - t0
+    if state_path:
+        np.savez(state_path, **{"st_" + k: v for k, v in sc["state"].items()},
+                 **{"ct_" + k: v for k, v in sc["contacts"].items()})
+    return sc, t_settle
+
+
+def cpu_baseline_contact(args, log):
+    """The oracle on every host core at once, each thread on its own copy of
+    a 20k-sphere settling bed in contact (configuration 2's physics, settled
+    on the GPU path as the 1M bed is, scaled to 20k spheres)."""
+    from sdem_amd import scenarios
 
     n = args.cpu_sample
     sc = scenarios.settling(n=n, seed=args.seed)
-    eng = native.build_engine(oracle.load(), sc, prefix=oracle.PREFIX)
-    eng.iterate(2)
+    for steps, g in CONTACT_SETTLE:
+        sc = scenarios.settle(sc, steps, gravity_scale=g, chunk=steps, log=log)
+    nth = host_threads()
+    rate, steps = oracle_rate([sc] * nth, args.cpu_steps)
+    P = len(sc["contacts"]["kind"])
+    return {"value": rate, "unit": "particle-steps/s", "cores": nth, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{nth} copies of a {n}-sphere settling bed in contact ({P} contact records), "
+                      f"{steps} steps each after 5 warm-up steps, oracle, one thread per copy"}
+
+
+def timed_region(eng, steps, log):
+    """K steps bracketed by stream synchronisation (Engine.iterate returns
+    after synchronising the library's stream); one library call per CHUNK
+    steps, with a progress line and a contact-count sample after each.
+    Returns (wall seconds, mean contact records per step sampled)."""
     t0 = time.perf_counter()
-    eng.iterate(5)
-    per_step = (time.perf_counter() - t0) / 5
-    steps = args.cpu_steps or max(5, int(15.0 / max(per_step, 1e-6)))
-    t0 = time.perf_counter()
+    done, counts_sum, nsamp = 0, np.zeros(4, np.int64), 0
+    while done < steps:
+        k = min(CHUNK, steps - done)
+        eng.iterate(k)
+        counts_sum += eng.get_counts()
+        nsamp += 1
+        done += k
+        log(f"# {done}/{steps} steps")
+    return time.perf_counter() - t0, float(counts_sum.sum()) / max(1, nsamp), counts_sum // max(1, nsamp)
+
+
+def kernel_roofline(eng, n, P, steps, ms_step):
+    """Per-kernel durations (HIP events around every kernel on the library's
+    stream over `steps` more steps, queued one by one) and the roofline of
+    the force kernel and of the whole step on the algorithmic bytes."""
+    eng.set_profiling(True)
     eng.iterate(steps)
-    dt = time.perf_counter() - t0
+    prof = eng.get_profile()
+    eng.set_profiling(False)
+    k_ms, k_n = prof["force"]
+    k_avg_s = k_ms / max(1, k_n) / 1e3
+    kernel_bytes = BYTES_PER_PARTICLE_KERNEL * n + BYTES_PER_CONTACT * P
+    achieved = kernel_bytes / k_avg_s / 1e9
+    step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * P
+    step_achieved = step_bytes / (ms_step / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_force", "kernel_ms_avg": k_avg_s * 1e3, "algorithmic_bytes_per_launch": kernel_bytes,
+            "whole_step": {"achieved": step_achieved, "frac": step_achieved / HBM_PEAK_GBS,
+                           "algorithmic_bytes_per_step": step_bytes,
+                           "definition": "B_step = 428 N + 80 P per step over the timed wall time"}}
+    return roof, {k: v[0] / max(1, v[1]) for k, v in prof.items()}
+
+
+def contact_bed_bench(args, log):
+    """The settled 1M bed stepped with configuration 2's options: the
+    contact regime of the headline workload (P ~ 2.3 contact records per
+    sphere)."""
+    from sdem_amd import native
+
+    n = args.n
+    sc, t_settle = settled_bed(n, args.seed, log, args.contact_state)
+    lib = native.load_hip_library()
+    eng = native.build_engine(lib, sc, tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN, keep_forces=0))
+    eng.iterate(args.warmup)
+    steps = args.contact_steps
+    wall, P, by_kind = timed_region(eng, steps, log)
+    ms_step = wall / steps * 1e3
+    roof, breakdown = kernel_roofline(eng, n, P, min(steps, 200), ms_step)
+    out = {"value": n * steps / wall, "unit": "particle-steps/s", "steps": steps, "warmup": args.warmup,
+           "ms_per_step": ms_step, "contact_records_per_step": P,
+           "contact_records_by_kind": {"PP": int(by_kind[0]), "PPG": int(by_kind[1]), "PPFix": int(by_kind[2]),
+                                       "PW": int(by_kind[3])},
+           "settle": {"schedule_steps_gravity_scale": [list(x) for x in CONTACT_SETTLE], "seconds": t_settle},
+           "roofline": roof, "step_breakdown_ms": breakdown, "particles_at_end": eng.num_particles()}
     eng.close()
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"settling bed, {n} spheres (same distribution, phi=0.3, xy area scaled), "
-                      f"{steps} steps after 7 warm-up steps, oracle on 1 core"}
+    return out
 
 
 def ensemble_bench(args):
@@ -223,6 +397,20 @@ def main():
         return
     from sdem_amd import native, scenarios
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "contact":
+        out = contact_bed_bench(args, log)
+        out.update({"metric": "particle-steps/s at 1M polydisperse spheres, settled bed in contact", "n_gpus": 1,
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                    "data": "synthetic (seeded loose pack settled by the GPU path)",
+                    "config": {"workload": WORKLOADS["settling"][1] + ", settled into contact", "n": args.n,
+                               "seed": args.seed}})
+        print(json.dumps(out), flush=True)
+        return
+    cpu = None
+    if not args.no_cpu and args.workload == "settling":
+        log("# cpu baseline (before any GPU work)")
+        cpu = cpu_baseline(args)
     t0 = time.perf_counter()
     if args.workload == "sandbox":
         n = 10_000
@@ -233,49 +421,31 @@ def main():
     else:
         n = args.n
         sc = scenarios.settling(n=n, seed=args.seed)
-    t_gen = time.perf_counter() - t0
-    print(f"# generated {n} spheres in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    log(f"# generated {n} spheres in {time.perf_counter() - t0:.1f}s")
     lib = native.load_hip_library()
     eng = native.build_engine(lib, sc, tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN, keep_forces=0))
     eng.iterate(args.warmup)
-    print(f"# warm-up {args.warmup} steps done", file=sys.stderr, flush=True)
-    # Timed region: steps queued as HIP graphs; Engine.iterate returns after
-    # synchronising the library's stream, so the region is bracketed by
-    # device synchronisation on both sides.  One call per CHUNK steps (a
-    # progress line and a contact-count sample after each): every call ends
-    # in a host synchronisation, so the default run is one call.
-    t0 = time.perf_counter()
-    done, counts_sum, nsamp = 0, np.zeros(4, np.int64), 0
-    chunk = CHUNK
-    while done < args.steps:
-        k = min(chunk, args.steps - done)
-        eng.iterate(k)
-        counts_sum += eng.get_counts()
-        nsamp += 1
-        done += k
-        print(f"# {done}/{args.steps} steps", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
+    log(f"# warm-up {args.warmup} steps done")
+    # Timed region: steps queued as HIP graphs, bracketed by device
+    # synchronisation on both sides (timed_region).
+    wall, P, _ = timed_region(eng, args.steps, log)
     ms_step = wall / args.steps * 1e3
     value = n * args.steps / wall
-    P = float(counts_sum.sum()) / max(1, nsamp)  # contact records per step (sampled)
-    # Kernel durations: HIP events around every kernel on the library's
-    # stream, over the same number of further steps queued one by one.
-    eng.set_profiling(True)
-    eng.iterate(min(args.steps, 1000))
-    prof = eng.get_profile()
-    eng.set_profiling(False)
+    roof, breakdown = kernel_roofline(eng, n, P, min(args.steps, 1000), ms_step)
     n_now = eng.num_particles()
-    k_ms, k_n = prof["force"]
-    k_avg_s = k_ms / max(1, k_n) / 1e3
-    kernel_bytes = BYTES_PER_PARTICLE_KERNEL * n + BYTES_PER_CONTACT * P
-    achieved = kernel_bytes / k_avg_s / 1e9
-    step_bytes = BYTES_PER_PARTICLE_STEP * n + BYTES_PER_CONTACT * P
+    eng.close()
     metric, workload = WORKLOADS[args.workload]
     if args.workload == "settling" and n != 1_000_000:
         side = 0.2 * (n / 1.0e6) ** 0.5  # scenarios.settling scales the xy area with n
         metric = f"particle-steps/s, settling bed of {n} spheres"
         workload = workload.replace("1M", f"{n}").replace("0.2x0.2 m", f"{side:.4g}x{side:.4g} m")
     headline = args.workload == "settling" and n == 1_000_000 and args.seed == 20261203
+    if headline:
+        # HBM bytes of one k_force launch on this bed, from rocprofv3 --pmc
+        # FETCH_SIZE and WRITE_SIZE scaled by a plain copy's counts: a
+        # constant of that measurement, not measured in this run
+        roof["traffic"] = KFORCE_TRAFFIC_1M
+        roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this bed (constant of that run)"
     out = {
         "metric": metric,
         "value": value,
@@ -291,18 +461,18 @@ def main():
         "data": "synthetic (seeded initial packing)",
         "config": {"workload": workload, "n": n, "seed": args.seed, "contact_records_per_step": P,
                    "particles_at_end": n_now},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": KFORCE_TRAFFIC_1M if headline else None,
-                     "kernel": "k_force", "kernel_ms_avg": k_avg_s * 1e3,
-                     "algorithmic_bytes_per_launch": kernel_bytes},
-        "step_breakdown_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},
-        "whole_step_hbm_frac": step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": roof,
+        "step_breakdown_ms": breakdown,
+        "whole_step_hbm_frac": roof["whole_step"]["frac"],
     }
-    eng.close()
-    if not args.no_cpu and args.workload == "settling":
-        print("# cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    if args.workload == "settling" and not args.no_contact:
+        log("# the same bed settled into contact")
+        out["contact_bed"] = contact_bed_bench(args, log)
+        if not args.no_cpu:
+            log("# cpu baseline in contact")
+            out["contact_bed"]["cpu_baseline"] = cpu_baseline_contact(args, log)
     print(json.dumps(out), flush=True)
 
 

@@ -1038,8 +1038,14 @@ int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind, const int32_t
   if (int rc = restage(c)) return rc;
   std::vector<int> ids(b.id);
   std::sort(ids.begin(), ids.end());
-  for (int64_t k = 0; k < m; k++)
+  for (int64_t k = 0; k < m; k++) {
     if (!std::binary_search(ids.begin(), ids.end(), owner_id[k])) return c->fail(SDEM_E_ARG, "owner id not present");
+    // a PP record sits in the lower id's bucket (AddContactPP,
+    // Prtcl_CL_and_CF.f90:142-160), with its partner among the particles
+    if (kind[k] == SDEM_KIND_PP &&
+        (owner_id[k] >= partner_id[k] || !std::binary_search(ids.begin(), ids.end(), partner_id[k])))
+      return c->fail(SDEM_E_ARG, "a PP record belongs to the lower id of two particles of the state");
+  }
   b.ckind.assign(kind, kind + m);
   b.cowner.assign(owner_id, owner_id + m);
   b.cpartner.assign(partner_id, partner_id + m);
@@ -1262,6 +1268,9 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       c->spec_full = ok_steps == 0 ? std::min(2 * c->spec_full + 1, 256) : 1;
       continue;
     }
+    if (s.overflow == 3)
+      return c->fail(SDEM_E_STATE, "a particle is more than one period outside a periodic dimension, or a "
+                                   "coordinate is not finite: the state has blown up");
     // a capacity ran out in step `ok_steps` of this batch: its input buffer
     // is intact; grow the capacity and replay from there
     if (++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");

@@ -154,7 +154,8 @@ struct BoxState {
 struct Scalars {
   int iter;                // iterNumber (all boxes step together)
   int iter_at_begin;       // iterNumber at the start of the last step that ran
-  int overflow;            // 1: history slots, 2: candidate-list entries
+  int overflow;            // 1: history slots, 2: candidate-list entries, 3: a
+                           // coordinate outside the periodic domain (error)
   int halt;                // set by the step after an overflow: no more work
   int halted_steps;        // steps of the call skipped after an overflow
   int rebuild;             // some box rebuilds this step (the sort kernels run)

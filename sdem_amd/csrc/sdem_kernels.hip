@@ -128,16 +128,24 @@ __device__ __forceinline__ CF contact_force(const Params& P, const Bnry& B, d3 n
     fn = -B.kn * ovrlp + -B.dn * vrn;
     k_t = -B.kt;
   } else if (P.cf_type == 1) {  // CFT_LSD
+    // Vel_in**0.4 as (Vel_in**0.2)**2: one pow instead of three, within
+    // an ulp or two of the reference's pow results
     double vel_in = vri > 0.01 ? vri : 0.01;
-    double k_n = -(B.kn * pow(vel_in, 0.4));
-    double d_n = -(B.dn * pow(vel_in, 0.2));
+    const double v02 = pow(vel_in, 0.2);
+    const double v04 = v02 * v02;
+    double k_n = -(B.kn * v04);
+    double d_n = -(B.dn * v02);
     fn = k_n * ovrlp + d_n * vrn;
-    k_t = -(B.kt * pow(vel_in, 0.4));
+    k_t = -(B.kt * v04);
   } else {               // CFT_nLin
+    // ovrlp**1.5 and ovrlp**0.25 from square roots (a few instructions
+    // each, where FP64 pow takes well over a hundred), within an ulp or
+    // two of the reference's pow results
     double k_n = -B.kn;
     double d_n = -B.dn;
-    fn = k_n * pow(ovrlp, 1.5) + d_n * pow(ovrlp, 0.25) * vrn;
-    k_t = -(B.kt * sqrt(ovrlp));
+    const double sq = sqrt(ovrlp);
+    fn = k_n * (ovrlp * sq) + d_n * sqrt(sq) * vrn;
+    k_t = -(B.kt * sq);
   }
   ftij = add3(mul3(ot, k_t), mul3(vij_t, d_t));
   d3 fnij = mul3(norm_v, fn);
@@ -758,7 +766,18 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const double xv[3] = {pi.x, pi.y, pi.z};
   const double ri = pi.w;
   ri_live = ri;
-  for (int L = 0; L < P.nlvl; L++) {
+  // The cell walk wraps a periodic dimension by one period at most: a
+  // coordinate that is not finite, or that lies more than a period outside
+  // the domain (a particle that crossed it in one step), is refused here
+  // with an error instead of being read past the grid.
+  bool bad = false;
+  for (int d = 0; d < 3; d++)
+    if (!isfinite(xv[d]) || (P.periodic[d] && !(xv[d] >= P.dmin[d] - P.simlen[d] &&
+                                                xv[d] <= P.dmax[d] + P.simlen[d])))
+      bad = true;
+  const int nlvl = bad ? 0 : P.nlvl;
+  if (bad) atomicMax(&S->overflow, 3);
+  for (int L = 0; L < nlvl; L++) {
     const Grid& G = P.lvl[L];
     const double reach = ri + G.rmax + P.skin;
     int lo[3], hi[3];
@@ -769,6 +788,9 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
       if (!P.periodic[d]) {
         lo[d] = lo[d] < 0 ? 0 : lo[d];
         hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      } else {
+        lo[d] = lo[d] < -G.n[d] ? -G.n[d] : lo[d];
+        hi[d] = hi[d] > 2 * G.n[d] - 1 ? 2 * G.n[d] - 1 : hi[d];
       }
     }
     for (int cz = lo[2]; cz <= hi[2]; cz++) {
@@ -809,7 +831,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
       }
     }
   }
-  if (P.search_fixed && P.nfix > 0) {
+  if (!bad && P.search_fixed && P.nfix > 0) {
     const Grid& G = P.fgrid;
     const double reach = ri + G.rmax + P.skin;
     int lo[3], hi[3];
@@ -820,6 +842,9 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
       if (!P.periodic[d]) {
         lo[d] = lo[d] < 0 ? 0 : lo[d];
         hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
+      } else {
+        lo[d] = lo[d] < -G.n[d] ? -G.n[d] : lo[d];
+        hi[d] = hi[d] > 2 * G.n[d] - 1 ? 2 * G.n[d] - 1 : hi[d];
       }
     }
     for (int cz = lo[2]; cz <= hi[2]; cz++) {
@@ -983,19 +1008,20 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  int* __restrict__ ccnt,
                                                  const double4* __restrict__ fpk,
                                                  const BoxSlot* rd, BoxSlot* wr) {
+  const int tile = blockIdx.x;
   const int box = block_box(blk_box);
   // FAST: summed largest moves since the last rebuild, before the control
   // adds the last step's
   double disp = FAST ? BS[box].disp_acc : 0.0;
-  if (FAST && blockIdx.x == 0 && threadIdx.x < 64)
+  if (FAST && tile == 0 && threadIdx.x < 64)
     if (!step_control(PB, S, BS, rd, wr, 1, 1)) return;
-  int stop = (S->halt || S->overflow == 2) ? 1 : 0;
+  int stop = (S->halt || S->overflow >= 2) ? 1 : 0;
   if (FAST) stop = __shfl(stop, 0, 64);  // one decision per wave: the whole wave reduces below
   if (stop) return;
   const Params& P = PB[box];
   // packed slot t: a particle of the box with candidates, from either end of
   // the box's slots
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = tile * blockDim.x + threadIdx.x;
   const bool listed = !(t - P.off >= BS[box].n_act && t - P.off < P.bcap - BS[box].n_far);
   // a fast step's waves stay whole until they have reduced the move below
   if (FAST ? !__ballot(listed) : !listed) return;
@@ -1089,195 +1115,205 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
   if (c > P.hcap) atomicMax(&S->cnt_over, 1);
 }
 
-// PIM / PRIM: translational / rotational integration method fixed at
-// compile time (0: read from Params).  With fixed methods the integration
-// inputs are loaded before the contact loop, so that their latency overlaps
-// it; otherwise after the loop, where the registers are free.
+// ---------------------------------------------------------------------------
+// The force step, in the parts the kernel k_force is made of:
+//   contact_loop  : every contact of a particle -> F, T, the new history
+//   Integ / integrate : accelerations, integration, wrap / delete, the
+//                   new state and next step's near-wall mask
+//   wave_flush    : per-wave reductions, one atomic per wave and value
+// ---------------------------------------------------------------------------
+
+// the start-of-step state of one particle (buffer A, slot s) the contact
+// loop reads
+struct Own {
+  double4 pi;
+  d3 vi, wi;
+  int idi, ti;
+  uint64_t mask0;  // near-wall list
+};
+
+__device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
+  Own o;
+  o.pi = A.pos[s];
+  o.idi = A.id[s];
+  o.ti = A.type[s];
+  o.vi = ld3(A.lv, P.cap, s);
+  o.wi = ld3(A.rv, P.cap, s);
+  o.mask0 = A.wmask[s];
+  return o;
+}
+
+// Every contact of the particle in slot s of A (new slot k), one call site
+// of the force routine: the contacts k_contact found, in ascending partner
+// id, then the walls of the near-wall list, newest first (FindContactsPW
+// :298-319 walks the list from the highest index).  Each contact is reduced
+// to the arguments of the force routine from the perspective the reference
+// evaluates it from (ContactForce_PP :732-756, _PPG :761-798, _PPFix
+// :803-827, _PW :832-856), so the routine is inlined once.  The records the
+// particle owns go to slot k of B.
+__device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, const Buf& B, int k, int s,
+                                             const Own& o, const unsigned* __restrict__ clist,
+                                             const int* __restrict__ ccnt, const double4* __restrict__ fpk,
+                                             const int* __restrict__ fid, const int* __restrict__ ftype,
+                                             Acc& acc) {
+  const int cap = P.cap;
+  const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z);
+  const double ri = o.pi.w;
+  const int idi = o.idi, ti = o.ti;
+  const int nc = min(ccnt[k], P.hcap);  // more: cnt_over is set
+  uint64_t mask = o.mask0;
+  const int ntot = nc + __popcll(mask);
+  for (int q = 0; q < ntot; q++) {
+    d3 nvec, va, vb, wa, wb;
+    double ra, rb, ovrlp;
+    Hist h;
+    const Bnry* Bp;
+    int kind, partner;
+    bool flip = false;
+    if (q < nc) {
+      const unsigned e = clist[(size_t)q * cap + k];
+      const int j = (int)(e & CL_INDEX);
+      double4 pj;
+      d3 vj, wj;
+      int idj, tj;
+      if (e & CL_FIXED) {
+        pj = fpk[j];
+        vj = D3(0.0, 0.0, 0.0);
+        wj = vj;
+        idj = fid[j];
+        tj = ftype[j];
+      } else {
+        pj = A.pos[j];
+        vj = ld3(A.lv, cap, j);
+        wj = ld3(A.rv, cap, j);
+        idj = A.id[j];
+        tj = A.type[j];
+      }
+      int shifted = 0;
+      const double xj = image(o.pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
+      const double yj = image(o.pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
+      const double zj = image(o.pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
+      const double dr = ri + pj.w;
+      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
+      double d2 = ddx * ddx;
+      d2 = ddy * ddy + d2;
+      d2 = ddz * ddz + d2;
+      ovrlp = dr - sqrt(d2);
+      const d3 xjv = D3(xj, yj, zj);
+      partner = idj;
+      if (e & CL_FIXED) {
+        // PPFix: the moving particle's record, its own perspective
+        kind = 3;
+        h = find_hist(A, cap, s, idj);
+        Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+      } else if (!shifted) {
+        // PP: the record sits in the lower id's bucket and the force is
+        // taken from that side (FineSearch2 :525-529)
+        const bool ilo = idi < idj;
+        kind = ilo ? 1 : 0;
+        h = ilo ? find_hist(A, cap, s, idj) : find_hist(A, cap, j, idi);
+        const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
+        Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
+        flip = !ilo;
+      } else {
+        // PPG: each side keeps its own record against the other's image,
+        // the force from the lower id's side; row type = the local particle
+        kind = 2;
+        h = find_hist(A, cap, s, idj);
+        Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+        flip = idi > idj;
+      }
+      if (!flip) {
+        nvec = nv3(xjv, xi);
+        ra = ri; rb = pj.w; va = o.vi; vb = vj; wa = o.wi; wb = wj;
+      } else {
+        nvec = nv3(xi, xjv);
+        ra = pj.w; rb = ri; va = vj; vb = o.vi; wa = wj; wb = o.wi;
+      }
+    } else {
+      const int w = 63 - __clzll((long long)mask);
+      mask &= ~(1ull << w);
+      const Wall& W = P.walls[w];
+      d3 nv;
+      if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
+      kind = 4;
+      partner = W.wall_id;
+      h = find_hist(A, cap, s, W.wall_id);
+      Bp = &P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
+      nvec = mul3(nv, -1.0);  // Norm_v = -nv
+      ra = ri; rb = 1.00e20 * ri; va = o.vi; vb = wp(W.trans_vel); wa = o.wi; wb = D3(0.0, 0.0, 0.0);
+    }
+    const CF cf = contact_force(P, *Bp, nvec, ra, rb, va, vb, wa, wb, ovrlp, h.td, h.vri);
+    if (!flip) {
+      acc.F = add3(acc.F, cf.fsum);
+      acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+    } else {
+      acc.F = sub3(acc.F, cf.fsum);
+      acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
+    }
+    if (kind) push_hist(P, B, k, acc, kind, partner, cf);
+  }
+}
+
+// The integration inputs beyond the contact loop's (history levels, theta,
+// mark).  PIM / PRIM: translational / rotational integration method fixed
+// at compile time (0: read from Params); with fixed methods they are loaded
+// by `preload` ahead of their use, so that their latency overlaps other
+// work, otherwise where they are used.
 template <int PIM, int PRIM>
-__global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
-                                               const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
-                                               BoxState* BS,
-                                               const int* __restrict__ perm,
-                                               const unsigned* __restrict__ clist,
-                                               const int* __restrict__ ccnt,
-                                               const double4* __restrict__ fpk,
-                                               const int* __restrict__ fid,
-                                               const int* __restrict__ ftype,
-                                               BoxSlot* __restrict__ bslots) {
-  // a candidate-list overflow in this step's rebuild leaves the lists
-  // incomplete: do nothing, the host regrows them and replays the step
-  if (S->halt || S->overflow == 2) return;
-  const int box = block_box(blk_box);
-  const Params& P = PB[box];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool rebuilt = BS[box].rebuild != 0;  // this step re-sorted the box: A is read through perm
-  // the control has advanced iterNumber: is the next step a near-wall update?
-  const bool upd_next = S->iter >= BS[box].next_wall_update;
-  // id, type, mark and the near-wall list change only where the box is
-  // re-sorted, a particle is removed or the lists are refreshed.  They are
-  // written when the buffer written here may hold other values: this step
-  // or the one before re-sorted the box (the other buffer is in an older
-  // order), this step or the one before refreshed the lists.  Otherwise
-  // both buffers hold the same values in the same order and the stores are
-  // left out.
-  const bool put_ids = rebuilt || BS[box].prev_rebuild != 0;
-  const bool put_wall = put_ids || upd_next || BS[box].wall_prev != 0;
-  Acc acc;
-  acc.F = D3(0.0, 0.0, 0.0);
-  acc.T = D3(0.0, 0.0, 0.0);
-  acc.hc = 0;
-  acc.nkind = 0ull;
-  acc.overflow = 0;
-  double move2 = 0.0, vnext = 0.0, anext = 0.0;
-  bool dead = false;
-  if (k - P.off < BS[box].n_alive) {
+struct Integ {
+  d3 v2, v3, a2, a3, th, w2, w3, al2, al3;
+  int mark;
+
+  __device__ __forceinline__ void preload(const Params& P, const Scalars* S, const Buf& A, int s, d3 vi,
+                                          bool put_ids) {
     const int cap = P.cap;
-    const int s = rebuilt ? perm[k] : k;
-    const double4 pi = A.pos[s];
-    const d3 xi = D3(pi.x, pi.y, pi.z);
-    const double ri = pi.w;
-    const int idi = A.id[s];
-    const int ti = A.type[s];
-    const d3 vi = ld3(A.lv, cap, s);
-    const d3 wi = ld3(A.rv, cap, s);
     const int pim = PIM ? PIM : P.pi_method, prim = PRIM ? PRIM : P.pri_method;
     const d3 z3 = D3(0.0, 0.0, 0.0);
-    d3 v2 = z3, v3 = z3, a2 = z3, a3 = z3, th = z3, w2 = z3, w3 = z3, al2 = z3, al3 = z3;
-    int mark = 0;
-    // rotVel(2) of a state this kernel wrote is its linVel(1) (below)
+    v2 = v3 = a2 = a3 = th = w2 = w3 = al2 = al3 = z3;
+    mark = 0;
+    if (!PIM) return;
+    // rotVel(2) of a state the force step wrote is its linVel(1) (below)
     const bool rv2 = S->rv2_stored[A.idx] != 0;
-    if (PIM) {
-      if (pim >= 2) {
-        v2 = ld3n(A.lv + 3 * cap, cap, s);
-        a2 = ld3n(A.la, cap, s);
-      }
-      if (pim == 3) {
-        v3 = ld3n(A.lv + 6 * cap, cap, s);
-        a3 = ld3n(A.la + 3 * cap, cap, s);
-      }
-      th = ld3n(A.th, cap, s);
-      if (prim == 2) {
-        if (rv2) w2 = ld3n(A.rv + 3 * cap, cap, s);
-        else w2 = vi;
-        al2 = ld3n(A.ra, cap, s);
-      } else if (prim == 3) {
-        w2 = ld3n(A.rv + 3 * cap, cap, s);
-        w3 = ld3n(A.rv + 6 * cap, cap, s);
-        al2 = ld3n(A.ra, cap, s);
-        al3 = ld3n(A.ra + 3 * cap, cap, s);
-      }
-      if (put_ids) mark = A.mark[s];
+    if (pim >= 2) {
+      v2 = ld3n(A.lv + 3 * cap, cap, s);
+      a2 = ld3n(A.la, cap, s);
     }
-
-    // ---------------- every contact, one call site ------------------------
-    // The contacts k_contact found, in ascending partner id, then the walls
-    // of the near-wall list, newest first
-    // (FindContactsPW :298-319 walks the list from the highest index).  Each
-    // contact is reduced to the arguments of the force routine from the
-    // perspective the reference evaluates it from (ContactForce_PP :732-756,
-    // _PPG :761-798, _PPFix :803-827, _PW :832-856), so the routine is
-    // inlined once.
-    const int nc = min(ccnt[k], P.hcap);  // more: cnt_over is set
-    const uint64_t mask0 = A.wmask[s];
-    uint64_t mask = mask0;
-    const int ntot = nc + __popcll(mask);
-    for (int q = 0; q < ntot; q++) {
-      d3 nvec, va, vb, wa, wb;
-      double ra, rb, ovrlp;
-      Hist h;
-      const Bnry* Bp;
-      int kind, partner;
-      bool flip = false;
-      if (q < nc) {
-        const unsigned e = clist[(size_t)q * cap + k];
-        const int j = (int)(e & CL_INDEX);
-        double4 pj;
-        d3 vj, wj;
-        int idj, tj;
-        if (e & CL_FIXED) {
-          pj = fpk[j];
-          vj = D3(0.0, 0.0, 0.0);
-          wj = vj;
-          idj = fid[j];
-          tj = ftype[j];
-        } else {
-          pj = A.pos[j];
-          vj = ld3(A.lv, cap, j);
-          wj = ld3(A.rv, cap, j);
-          idj = A.id[j];
-          tj = A.type[j];
-        }
-        int shifted = 0;
-        const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
-        const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
-        const double zj = image(pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
-        const double dr = ri + pj.w;
-        const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
-        double d2 = ddx * ddx;
-        d2 = ddy * ddy + d2;
-        d2 = ddz * ddz + d2;
-        ovrlp = dr - sqrt(d2);
-        const d3 xjv = D3(xj, yj, zj);
-        partner = idj;
-        if (e & CL_FIXED) {
-          // PPFix: the moving particle's record, its own perspective
-          kind = 3;
-          h = find_hist(A, cap, s, idj);
-          Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
-        } else if (!shifted) {
-          // PP: the record sits in the lower id's bucket and the force is
-          // taken from that side (FineSearch2 :525-529)
-          const bool ilo = idi < idj;
-          kind = ilo ? 1 : 0;
-          h = ilo ? find_hist(A, cap, s, idj) : find_hist(A, cap, j, idi);
-          const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
-          Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
-          flip = !ilo;
-        } else {
-          // PPG: each side keeps its own record against the other's image,
-          // the force from the lower id's side; row type = the local particle
-          kind = 2;
-          h = find_hist(A, cap, s, idj);
-          Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
-          flip = idi > idj;
-        }
-        if (!flip) {
-          nvec = nv3(xjv, xi);
-          ra = ri; rb = pj.w; va = vi; vb = vj; wa = wi; wb = wj;
-        } else {
-          nvec = nv3(xi, xjv);
-          ra = pj.w; rb = ri; va = vj; vb = vi; wa = wj; wb = wi;
-        }
-      } else {
-        const int w = 63 - __clzll((long long)mask);
-        mask &= ~(1ull << w);
-        const Wall& W = P.walls[w];
-        d3 nv;
-        if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
-        kind = 4;
-        partner = W.wall_id;
-        h = find_hist(A, cap, s, W.wall_id);
-        Bp = &P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
-        nvec = mul3(nv, -1.0);  // Norm_v = -nv
-        ra = ri; rb = 1.00e20 * ri; va = vi; vb = wp(W.trans_vel); wa = wi; wb = D3(0.0, 0.0, 0.0);
-      }
-      const CF cf = contact_force(P, *Bp, nvec, ra, rb, va, vb, wa, wb, ovrlp, h.td, h.vri);
-      if (!flip) {
-        acc.F = add3(acc.F, cf.fsum);
-        acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-      } else {
-        acc.F = sub3(acc.F, cf.fsum);
-        acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
-      }
-      if (kind) push_hist(P, B, k, acc, kind, partner, cf);
+    if (pim == 3) {
+      v3 = ld3n(A.lv + 6 * cap, cap, s);
+      a3 = ld3n(A.la + 3 * cap, cap, s);
     }
+    th = ld3n(A.th, cap, s);
+    if (prim == 2) {
+      if (rv2) w2 = ld3n(A.rv + 3 * cap, cap, s);
+      else w2 = vi;
+      al2 = ld3n(A.ra, cap, s);
+    } else if (prim == 3) {
+      w2 = ld3n(A.rv + 3 * cap, cap, s);
+      w3 = ld3n(A.rv + 6 * cap, cap, s);
+      al2 = ld3n(A.ra, cap, s);
+      al3 = ld3n(A.ra + 3 * cap, cap, s);
+    }
+    if (put_ids) mark = A.mark[s];
+  }
 
-    // ---------------- acceleration (DEMS_clc_Acceleration :360-375) -------
+  // Acceleration (DEMS_clc_Acceleration :360-375), integration
+  // (Prtcl_Integrate :15-88), periodic wrap / deletion (Comm_For_Exchange),
+  // the new state in slot k of B.  Returns whether the particle left
+  // through an open side; move2 = its squared move, vnext / anext = |v|,
+  // |a| of the new state when the next step refreshes the near-wall lists.
+  __device__ __forceinline__ bool integrate(const Params& P, const Scalars* S, const Buf& A, const Buf& B, int k,
+                                            int s, const Own& o, d3 F, d3 T, bool put_ids, bool put_wall,
+                                            bool upd_next, double& move2, double& vnext, double& anext) {
+    const int cap = P.cap;
+    const int pim = PIM ? PIM : P.pi_method, prim = PRIM ? PRIM : P.pri_method;
+    const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z), vi = o.vi, wi = o.wi;
+    const int ti = o.ti;
+    const bool rv2 = PIM ? false : S->rv2_stored[A.idx] != 0;
     const double mass = P.mass[ti - 1], inertia = P.inertia[ti - 1];
-    const d3 a1 = add3(div3(acc.F, mass), D3(P.g[0], P.g[1], P.g[2]));
-    const d3 al1 = div3(acc.T, inertia);
+    const d3 a1 = add3(div3(F, mass), D3(P.g[0], P.g[1], P.g[2]));
+    const d3 al1 = div3(T, inertia);
 
-    // ---------------- integration (Prtcl_Integrate :15-88) ----------------
     const double dt = P.dt;
     d3 pos = xi;
     d3 lin1;  // linVel(1) after the translational update
@@ -1325,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       // Prtcl_Integration.f90:64: rotVel1 = GPrtcl_linVel(1,i), and
       // rotVel(2) <- rotVel1 (:69), so rotVel(2) of the new state is its
       // linVel(1): it is not stored, and read back as linVel(1) (vi) from a
-      // buffer this kernel wrote
+      // buffer the force step wrote
       const double c1 = 3.0 / 2.0, c2 = -1.0 / 2.0;
       d3 w1 = lin1;
       if (!PIM) {
@@ -1359,7 +1395,8 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
       move2 = mx * mx + my * my + mz * mz;
     }
 
-    // ---------------- periodic wrap / deletion (Comm_For_Exchange) --------
+    // periodic wrap / deletion (Comm_For_Exchange)
+    bool dead = false;
     double pv[3] = {pos.x, pos.y, pos.z};
     for (int d = 0; d < 3; d++) {
       if (P.periodic[d]) {
@@ -1369,66 +1406,149 @@ __global__ __launch_bounds__(256) void k_force(const Params* __restrict__ PB,
         dead = true;
       }
     }
-    B.pos[k] = make_double4(pv[0], pv[1], pv[2], ri);
+    B.pos[k] = make_double4(pv[0], pv[1], pv[2], o.pi.w);
     if (put_ids) {
-      B.id[k] = idi;
+      B.id[k] = o.idi;
       B.mark[k] = PIM ? mark : A.mark[s];
     }
     if (put_ids || dead) B.type[k] = dead ? -ti : ti;
     // the next step's near-wall list is built here from the new state
     // (UpdateNearPrtclsPW runs at the start of that step, after deletion)
-    if (put_wall)
-      B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : (PIM ? mask0 : A.wmask[s]);
+    if (put_wall) B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : o.mask0;
     if (upd_next && !dead) {
       vnext = norm3(lin1);
       anext = norm3(a1);  // linAcc(1) at the start of the next step
     }
-    B.hcnt[k] = dead ? 0 : acc.hc;
     if (P.keep_forces) {
-      st3n(B.force, cap, k, acc.F);
-      st3n(B.force + 3 * cap, cap, k, acc.T);
+      st3n(B.force, cap, k, F);
+      st3n(B.force + 3 * cap, cap, k, T);
     }
+    return dead;
   }
-  // wave reductions, then one atomic per wave and value on slot
-  // (wave % NSLOT) of the box: no barrier in this kernel.  The counts are
-  // exact integer sums and the maxima exact, whatever the order.
+};
+
+// What the force step decides from the box's control for one block: A is
+// read through the sort permutation on a step that re-sorted the box; id,
+// type, mark and the near-wall list change only where the box is re-sorted,
+// a particle is removed or the lists are refreshed, so they are written when
+// the buffer written here may hold other values (this step or the one
+// before re-sorted the box, or refreshed the lists); otherwise both buffers
+// hold the same values in the same order and the stores are left out.
+struct StepFlags {
+  bool rebuilt, upd_next, put_ids, put_wall;
+};
+__device__ __forceinline__ StepFlags step_flags(const Scalars* S, const BoxState& bs) {
+  StepFlags f;
+  f.rebuilt = bs.rebuild != 0;
+  // the control has advanced iterNumber: is the next step a near-wall update?
+  f.upd_next = S->iter >= bs.next_wall_update;
+  f.put_ids = f.rebuilt || bs.prev_rebuild != 0;
+  f.put_wall = f.put_ids || f.upd_next || bs.wall_prev != 0;
+  return f;
+}
+
+// Wave reductions, then one atomic per wave and value on slot
+// (wave % NSLOT) of the box: no barrier.  The counts are exact integer sums
+// and the maxima exact, whatever the order.  `counts`: the contact counts
+// (the kernel ran the contact loop); `motion`: the move, max|v|, max|a| and
+// deletions (the kernel integrated).
+__device__ __forceinline__ void wave_flush(BoxSlot* __restrict__ bslots, int box, bool counts, bool motion,
+                                           unsigned long long nkind, double move2, bool upd_next, double vnext,
+                                           double anext, bool dead) {
+  unsigned long long c01 = (nkind & 0xffffull) | (((nkind >> 16) & 0xffffull) << 32);
+  unsigned long long c23 = ((nkind >> 32) & 0xffffull) | (((nkind >> 48) & 0xffffull) << 32);
   double wmove = move2;
-  unsigned long long c01 = (acc.nkind & 0xffffull) | (((acc.nkind >> 16) & 0xffffull) << 32);
-  unsigned long long c23 = ((acc.nkind >> 32) & 0xffffull) | (((acc.nkind >> 48) & 0xffffull) << 32);
-  for (int off = 32; off > 0; off >>= 1) {
-    wmove = fmax(wmove, __shfl_xor(wmove, off, 64));
-    c01 += __shfl_xor(c01, off, 64);
-    c23 += __shfl_xor(c23, off, 64);
-  }
-  const int wdel = __popcll(__ballot(dead));
-  double wv = 0.0, wa = 0.0;
-  if (upd_next) {
-    wv = vnext;
-    wa = anext;
+  if (counts)
     for (int off = 32; off > 0; off >>= 1) {
-      wv = fmax(wv, __shfl_xor(wv, off, 64));
-      wa = fmax(wa, __shfl_xor(wa, off, 64));
+      c01 += __shfl_xor(c01, off, 64);
+      c23 += __shfl_xor(c23, off, 64);
+    }
+  int wdel = 0;
+  double wv = 0.0, wa = 0.0;
+  if (motion) {
+    for (int off = 32; off > 0; off >>= 1) wmove = fmax(wmove, __shfl_xor(wmove, off, 64));
+    wdel = __popcll(__ballot(dead));
+    if (upd_next) {
+      wv = vnext;
+      wa = anext;
+      for (int off = 32; off > 0; off >>= 1) {
+        wv = fmax(wv, __shfl_xor(wv, off, 64));
+        wa = fmax(wa, __shfl_xor(wa, off, 64));
+      }
     }
   }
   if ((threadIdx.x & 63) == 0) {
     const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     BoxSlot& bl = bslots[box * NSLOT + wave % NSLOT];
-    if (wmove > 0.0) atomicMax(&bl.step_bits, dbits(sqrt(wmove)));
-    if (upd_next) {
-      atomicMax(&bl.max_v_bits, dbits(wv));
-      atomicMax(&bl.max_a_bits, dbits(wa));
+    if (motion) {
+      if (wmove > 0.0) atomicMax(&bl.step_bits, dbits(sqrt(wmove)));
+      if (upd_next) {
+        atomicMax(&bl.max_v_bits, dbits(wv));
+        atomicMax(&bl.max_a_bits, dbits(wa));
+      }
+      if (wdel) atomicAdd(&bl.ndel, wdel);
     }
-    const int cnt[4] = {(int)(c01 & 0xffffffffull), (int)(c01 >> 32), (int)(c23 & 0xffffffffull),
-                        (int)(c23 >> 32)};
-    for (int q = 0; q < 4; q++)
-      if (cnt[q]) atomicAdd(&bl.cnt[q], cnt[q]);
-    if (wdel) atomicAdd(&bl.ndel, wdel);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      S->have_parts = 1;
-      S->rv2_stored[B.idx] = 0;
-      // this step's contact test found more contacts than history slots
-      if (S->cnt_over) atomicMax(&S->overflow, 1);
+    if (counts) {
+      const int cnt[4] = {(int)(c01 & 0xffffffffull), (int)(c01 >> 32), (int)(c23 & 0xffffffffull),
+                          (int)(c23 >> 32)};
+      for (int q = 0; q < 4; q++)
+        if (cnt[q]) atomicAdd(&bl.cnt[q], cnt[q]);
     }
+  }
+}
+
+__device__ __forceinline__ Acc acc_zero() {
+  Acc acc;
+  acc.F = D3(0.0, 0.0, 0.0);
+  acc.T = D3(0.0, 0.0, 0.0);
+  acc.hc = 0;
+  acc.nkind = 0ull;
+  acc.overflow = 0;
+  return acc;
+}
+
+// The fused force step: contact loop, accelerations and integration in one
+// kernel, reading and writing each particle's stream once (the choice for
+// beds with few contacts, where the step is a streaming pass).  With fixed
+// integration methods (PIM, PRIM) the integration inputs are loaded before
+// the contact loop, so that their latency overlaps it; the registers that
+// holds are capped at two waves per SIMD (256 VGPRs; a few spill).
+template <int PIM, int PRIM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_force(const Params* __restrict__ PB,
+                                               const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
+                                               BoxState* BS,
+                                               const int* __restrict__ perm,
+                                               const unsigned* __restrict__ clist,
+                                               const int* __restrict__ ccnt,
+                                               const double4* __restrict__ fpk,
+                                               const int* __restrict__ fid,
+                                               const int* __restrict__ ftype,
+                                               BoxSlot* __restrict__ bslots) {
+  // a candidate-list overflow in this step's rebuild leaves the lists
+  // incomplete: do nothing, the host regrows them and replays the step
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const StepFlags f = step_flags(S, BS[box]);
+  Acc acc = acc_zero();
+  double move2 = 0.0, vnext = 0.0, anext = 0.0;
+  bool dead = false;
+  if (k - P.off < BS[box].n_alive) {
+    const int s = f.rebuilt ? perm[k] : k;
+    const Own o = load_own(P, A, s);
+    Integ<PIM, PRIM> in;
+    in.preload(P, S, A, s, o.vi, f.put_ids);
+    contact_loop(P, A, B, k, s, o, clist, ccnt, fpk, fid, ftype, acc);
+    dead = in.integrate(P, S, A, B, k, s, o, acc.F, acc.T, f.put_ids, f.put_wall, f.upd_next, move2, vnext, anext);
+    B.hcnt[k] = dead ? 0 : acc.hc;
+  }
+  wave_flush(bslots, box, true, true, acc.nkind, move2, f.upd_next, vnext, anext, dead);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->have_parts = 1;
+    S->rv2_stored[B.idx] = 0;
+    // this step's contact test found more contacts than history slots
+    if (S->cnt_over) atomicMax(&S->overflow, 1);
   }
   if (acc.overflow) atomicMax(&S->overflow, 1);
 }
@@ -1501,12 +1621,14 @@ hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
                        D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
                        slot_set(D, A.idx), slot_set(D, B.idx));
   if (mid) (void)hipEventRecord(mid, st);
-  if (D.pim == 2 && D.prim == 2)
-    hipLaunchKernelGGL((k_force<2, 2>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
-                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
-  else
-    hipLaunchKernelGGL((k_force<0, 0>), dim3(nblk(D.nslots, D.bs_force)), dim3(D.bs_force), 0, st, D.PB,
-                       D.blk_box, A, B, D.S, D.BS, D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
+  const unsigned nf = nblk(D.nslots, D.bs_force);
+  if (D.pim == 2 && D.prim == 2) {
+    hipLaunchKernelGGL((k_force<2, 2>), dim3(nf), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS,
+                       D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
+  } else {
+    hipLaunchKernelGGL((k_force<0, 0>), dim3(nf), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS,
+                       D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
+  }
   return hipGetLastError();
 }
 

@@ -124,3 +124,21 @@ def test_long_rollout_statistics(hip_lib, oracle_lib):
         assert abs(int(cg) - int(cr)) <= 0.03 * cr + 2
         ke0 = r["ke"] if ke0 is None else ke0
     assert r["ke"] < 0.5 * ke0  # the energy decays
+
+
+def test_runaway_particle_is_an_error(hip_lib):
+    """A sphere that crosses a periodic dimension many times in one step
+    (1e4 m/s, dt 1e-4 s, a 0.05 m period) leaves a coordinate that one
+    wrap does not bring back into the domain; the next rebuild's cell walk
+    refuses it and sdem_step returns SDEM_E_STATE (a blown-up state) instead
+    of reading past the cell grid.  The reference has no such check (its
+    cell index would fall outside its arrays)."""
+    print("runaway particle: 2 steps, expecting SDEM_E_STATE", flush=True)
+    sc = scenarios.random_cloud(n=60, seed=12, spacing=1.5, periodic=(1, 1, 1), walls="", dt=1.0e-4)
+    sc["state"]["lin_vel"][5, :, 0] = 1.0e4
+    g = native.build_engine(hip_lib, sc)
+    with pytest.raises(native.SdemError) as err:
+        g.iterate(3)
+    assert err.value.code == native.SDEM_E_STATE
+    assert "blown up" in str(err.value)
+    g.close()
