@@ -91,6 +91,24 @@ int sdem_gnn_mfma_probe(const uint16_t* a_bits, const uint16_t* b_bits, float* d
 int sdem_gnn_gemm(int32_t ta, int32_t tb, int32_t m, int32_t n, int32_t k, const float* a, const float* b,
                   const float* bias, int32_t flags, int32_t splits, float* c, float* pre);
 
+/* sdem_gnn_gemm without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  a (m x k, or k x m with ta), b (k x n, or
+ * n x k with tb), bias (n) and c (m x n) are device buffers of fp32,
+ * row-major; pre (m x n, may be NULL) gets the pre-activation when splits
+ * resolve to one K chunk.  The work is queued on `stream` (a hipStream_t;
+ * NULL = the null stream) and the call returns without waiting for it.  The
+ * bf16 copies of a and b and the split-K partial sums live in one fixed
+ * array of SDEM_GNN_GEMM_SCRATCH_BYTES on each device, which the library
+ * holds until the process ends: calls whose staging (2 m k + 2 k n bytes,
+ * each rounded up to 256, plus 4 m n per K chunk when there is more than
+ * one) does not fit return SDEM_E_CAPACITY.  Every call on a device shares
+ * that device's array: calls that overlap on one device (other streams,
+ * other threads) race.  Gives the bits of sdem_gnn_gemm. */
+#define SDEM_GNN_GEMM_SCRATCH_BYTES (16 << 20)
+int sdem_gnn_gemm_noalloc(int32_t ta, int32_t tb, int32_t m, int32_t n, int32_t k, const float* a, const float* b,
+                          const float* bias, int32_t flags, int32_t splits, float* c, float* pre, void* stream);
+
 const char* sdem_gnn_last_error(const sdem_gnn* m);
 void sdem_gnn_destroy(sdem_gnn* m);
 

@@ -740,3 +740,79 @@ hipError_t advance(double4* pos, double* vel, double* acc, int n, int stride, co
 }
 
 }  // namespace gnn
+
+// ---------------------------------------------------------------------------
+// sdem_gnn_gemm_noalloc (include/sdem_gnn.h): sdem_gnn_gemm on device
+// buffers, with the bf16 copies of A and B and the split-K partial sums in
+// `scratch`, a fixed array of the library on every device it runs on, so
+// that nothing is allocated.  Host code reaches the array's device memory
+// only through the address hipGetSymbolAddress gives for the current
+// device (kept per device in gemm_scratch), never through its name.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+#include "sdem_gnn.h"
+
+static __device__ __align__(256) unsigned char scratch[SDEM_GNN_GEMM_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+// the address of this device's scratch array, looked up once per device
+hipError_t gemm_scratch(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)scratch_ptr[dev];
+  return hipSuccess;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" int sdem_gnn_gemm_noalloc(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* a,
+                                     const float* b, const float* bias, int32_t flags, int32_t splits, float* c,
+                                     float* pre, void* stream) {
+  using namespace gnn;
+  if (!a || !b || !c || M < 1 || N < 1 || K < 1) return SDEM_E_ARG;
+  if ((flags & EPI_BIAS) && !bias) return SDEM_E_ARG;
+  // the K chunks of gemm(): whole multiples of its K stage
+  int sp = splits < 1 ? 1 : splits;
+  int kchunk = (K + sp - 1) / sp;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  const int nz = (K + kchunk - 1) / kchunk;
+  const size_t na = (size_t)M * K, nbb = (size_t)K * N, nc = (size_t)M * N;
+  const size_t off_b = align256(na * 2), off_p = off_b + align256(nbb * 2);
+  const size_t need = off_p + (nz > 1 ? (size_t)nz * nc * 4 : 0);
+  if (need > (size_t)SDEM_GNN_GEMM_SCRATCH_BYTES) return SDEM_E_CAPACITY;
+  unsigned char* base = nullptr;
+  hipError_t e = gemm_scratch(&base);
+  if (e != hipSuccess) return SDEM_E_HIP;
+  hipStream_t st = (hipStream_t)stream;
+  __bf16* sa = (__bf16*)base;
+  __bf16* sb = (__bf16*)(base + off_b);
+  float* part = nz > 1 ? (float*)(base + off_p) : c;
+  // every scratch byte read below is written first in the same call: no
+  // clearing is needed, and nothing of an earlier call is read
+  e = to_bf16(a, na, sa, st);
+  if (e == hipSuccess) e = to_bf16(b, nbb, sb, st);
+  int fl = flags & (EPI_BIAS | EPI_SILU | EPI_PRE);
+  if (!pre || nz > 1) fl &= ~EPI_PRE;  // the pre-activation of one chunk alone means nothing
+  const int lda = ta ? M : K, ldb = tb ? K : N;
+  Epi ep{fl, bias, pre, nullptr, part, nullptr};
+  if (e == hipSuccess) e = gemm(ta != 0, tb != 0, sa, sb, M, N, K, lda, ldb, sp, ep, st);
+  if (e == hipSuccess && nz > 1) e = reduce_parts(part, nz, nc, c, false, st);
+  return e == hipSuccess ? SDEM_OK : SDEM_E_HIP;
+}
