@@ -71,6 +71,19 @@ extern "C" {
 #define SDEM_PHASE_FORCE      4   /* history/force/accel/integrate kernel       */
 #define SDEM_NUM_PHASES       5
 
+/* the reference's step timers (Prtcl_DEMSystem.f90:29-37, logged at
+ * :280-305), reported by sdem_get_timers while profiling is on */
+#define SDEM_TIMER_TOTAL          0  /* m_total_timer                             */
+#define SDEM_TIMER_PRE_ITER       1  /* m_pre_iter_timer      "PreItertion time"     */
+#define SDEM_TIMER_COMM_CS        2  /* m_comm_cs_timer       "Comm_For_Contact"     */
+#define SDEM_TIMER_CSCF_PP        3  /* m_CSCF_PP_timer       "CS and CF P-P time"   */
+#define SDEM_TIMER_CSCF_PW        4  /* m_CSCF_PW_timer       "CS and CF P-W time"   */
+#define SDEM_TIMER_ACCELERATION   5  /* m_Acceleration_timer  "Acceleration time"    */
+#define SDEM_TIMER_INTEGRATION    6  /* m_integration_timer   "Integration time"     */
+#define SDEM_TIMER_COMM_EXCHANGE  7  /* m_comm_exchange_timer "Comm_For_Exchange"    */
+#define SDEM_TIMER_WRITE          8  /* m_write_prtcl_timer   "Write to file time"   */
+#define SDEM_NUM_TIMERS           9
+
 /*
  * The DEMS_Options fields the step reads (Prtcl_Parameters.f90:45-95,
  * namelist &DEMOptions read at :120-124).  Doubles first, then 32-bit
@@ -183,6 +196,28 @@ int sdem_set_contacts(sdem_ctx* c, int64_t m, const int32_t* kind,
  * work is queued and checked; it synchronises before returning. */
 int sdem_step(sdem_ctx* c, int32_t nsteps);
 
+/* ---- one step phase by phase ---------------------------------------------
+ * The calls of DEMS_iterate (Prtcl_DEMSystem.f90:213-248) one at a time, so
+ * that each reference call can be stood in for on its own.  They run on the
+ * context's device state in the reference's order -- pre-iteration, ghost
+ * communication, particle contacts, wall contacts, accelerations,
+ * integration, exchange, release -- and together are one sdem_step(c, 1),
+ * with its bits (the same operations in the same order).  A call out of that
+ * order returns SDEM_E_STATE, and so does sdem_step while a phased step is
+ * open.  If sdem_find_contacts or sdem_find_contacts_pw runs out of
+ * history or candidate-list capacity it grows it, puts the step back to its
+ * start and returns SDEM_E_CAPACITY: run the step again from
+ * sdem_pre_iteration.  iterNumber reads as advanced from sdem_pre_iteration
+ * on (the step's control advances it with the near-wall cadence). */
+int sdem_pre_iteration(sdem_ctx* c);     /* DEMS_preIteration :322-331: UpdateNearPrtclsPW, F/T zeroing, PreIteration */
+int sdem_comm_for_cntct(sdem_ctx* c);    /* Comm_For_Cntct :220 (ghost images: minimum image here, nothing to do) */
+int sdem_find_contacts(sdem_ctx* c);     /* FindContacts :225: search, PP / PPG / PPFix forces and history */
+int sdem_find_contacts_pw(sdem_ctx* c);  /* FindContactsPW :230: wall forces and history */
+int sdem_clc_acceleration(sdem_ctx* c);  /* clc_Acceleration :235 */
+int sdem_integrate(sdem_ctx* c);         /* Prtcl_Integrate :240 */
+int sdem_comm_for_exchange(sdem_ctx* c); /* Comm_For_Exchange :245: periodic wrap, deletion */
+int sdem_remv_released(sdem_ctx* c);     /* RemvReleased :246 and the end of the step (iterNumber + 1 :248) */
+
 /* Number of particles in the domain (np_InDomain, Prtcl_DEMSystem.f90:252). */
 int sdem_get_num_particles(sdem_ctx* c, int32_t* n);
 
@@ -216,6 +251,14 @@ int sdem_get_iter(sdem_ctx* c, int32_t* iter_number, int32_t* next_wall_update);
 int sdem_set_profiling(sdem_ctx* c, int32_t enable);
 int sdem_get_profile(sdem_ctx* c, double ms[SDEM_NUM_PHASES],
                      int64_t launches[SDEM_NUM_PHASES]);
+/* The reference's nine step timers (SDEM_TIMER_*): total milliseconds and
+ * number of timed steps or calls since profiling was enabled.  Steps run
+ * phase by phase fill each timer from its phase entry point; sdem_step runs
+ * the contact search, every force, the accelerations, integration and
+ * exchange in fused kernels and books all of them under SDEM_TIMER_CSCF_PP
+ * (the step's control under SDEM_TIMER_PRE_ITER).  The library writes no
+ * files during a step: SDEM_TIMER_WRITE is the caller's. */
+int sdem_get_timers(sdem_ctx* c, double tot_ms[SDEM_NUM_TIMERS], int64_t calls[SDEM_NUM_TIMERS]);
 
 /* Search and output tuning (no counterpart in the reference; results do not
  * depend on it).  cell_factor[d] scales the search-cell edge (the reference's

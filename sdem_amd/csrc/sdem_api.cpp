@@ -26,6 +26,7 @@ hipError_t launch_finish(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_sort(const Dev& D, const Buf& A, hipStream_t st);
 hipError_t launch_vlist(const Dev& D, hipStream_t st);
 hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid, bool fast);
+hipError_t launch_phase(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, int phase);
 int64_t scan_tiles(int64_t n);
 int scan_tile_keys();
 }  // namespace sdem
@@ -249,6 +250,7 @@ struct sdem_ctx {
   int4* act = nullptr;
   unsigned* clist = nullptr;
   int* ccnt = nullptr;
+  double* ft = nullptr;  // phase entry points: F, T, then a, alpha (6 x cap)
   int64_t nkeys = 0;
   bool use_graph = true;
   // graphs of one and of two steps, without / with the speculative fast
@@ -262,6 +264,13 @@ struct sdem_ctx {
   bool prof = false;
   double prof_ms[SDEM_NUM_PHASES] = {};
   int64_t prof_n[SDEM_NUM_PHASES] = {};
+  // the reference's step timers (Prtcl_DEMSystem.f90:29-37), kept while
+  // profiling is on
+  double timer_ms[SDEM_NUM_TIMERS] = {};
+  int64_t timer_n[SDEM_NUM_TIMERS] = {};
+  // a step run phase by phase: the last phase entry point that ran
+  // (PH_NONE: no step open)
+  int phase = 0;
   std::vector<void*> allocs_state, allocs_grid, allocs_tab;
   int64_t dev_bytes = 0;
 
@@ -494,7 +503,7 @@ static int upload_state(sdem_ctx* c) {
   M(&c->keys, cap); M(&c->rank, cap); M(&c->perm_tmp, cap); M(&c->perm, cap); M(&c->pk, cap); M(&c->pid, cap);
   M(&c->vlist, cap * c->lcap); M(&c->vkey, cap * c->lcap);
   M(&c->act, cap); M(&c->cand, cap * c->lcap);
-  M(&c->ccnt, cap); M(&c->clist, cap * c->hcap);
+  M(&c->ccnt, cap); M(&c->clist, cap * c->hcap); M(&c->ft, cap * 6);
   M(&c->d_blk_box, cap / BOX_ALIGN);
   if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
 
@@ -762,7 +771,7 @@ static int upload_params(sdem_ctx* c) {
   D.keys = c->keys; D.rank = c->rank; D.counts = c->counts; D.start = c->start; D.tile_sum = c->tile_sum;
   D.tile_box = d_tile_box;
   D.perm_tmp = c->perm_tmp; D.perm = c->perm; D.pk = c->pk; D.pid = c->pid;
-  D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt;
+  D.vlist = c->vlist; D.vkey = c->vkey; D.clist = c->clist; D.ccnt = c->ccnt; D.ft = c->ft;
   D.act = c->act; D.cand = c->cand;
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
   D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
@@ -1115,6 +1124,43 @@ static int grow_vlist(sdem_ctx* c, int list_need) {
   return SDEM_OK;
 }
 
+// After a step that ran out of a capacity (s.overflow 1: history slots, 2:
+// candidate-list entries), read from the device into s: the step's input
+// buffer is intact, so the step counters and box states go back to the start
+// of that step and the capacity grows; the step then runs again.  A state
+// that has blown up (3) is an error.
+static int rewind_and_grow(sdem_ctx* c, Scalars& s) {
+  if (s.overflow == 3)
+    return c->fail(SDEM_E_STATE, "a particle is more than one period outside a periodic dimension, or a "
+                                 "coordinate is not finite: the state has blown up");
+  const int nbox = (int)c->boxes.size();
+  const int kind = s.overflow, list_need = s.list_need;
+  s.halt = 0;
+  s.halted_steps = 0;
+  s.cnt_over = 0;
+  s.iter = s.iter_at_begin;
+  s.need_rebuild = 1;
+  s.have_parts = 0;
+  s.list_need = 0;
+  s.overflow = 0;
+  std::vector<BoxState> bs;
+  hipError_t e = dl(bs, c->BS, (size_t)nbox);
+  for (auto& B : bs) {
+    B.n_alive = B.n_stored;
+    B.next_wall_update = B.next_wall_at_begin;
+    if (c->first_step_pending) B.use_host_max_a = 1;
+  }
+  if (e == hipSuccess) e = hipMemcpy(c->BS, bs.data(), bs.size() * sizeof(BoxState), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
+  // the slots hold what the failing and the halted steps folded in; the
+  // replayed step rebuilds its lists, so only its own maxima are needed
+  if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSET * NSLOT * sizeof(BoxSlot));
+  if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
+  c->need_pre = true;
+  c->spec_ready = false;
+  return kind == 2 ? grow_vlist(c, list_need) : grow_history(c);
+}
+
 // Queue one step on the context's stream; ev (SDEM_NUM_PHASES+1 events) may
 // be null.  The buffers swap roles afterwards.  A fast step (single box)
 // leaves out the sort and list kernels, and its control runs inside the
@@ -1174,6 +1220,7 @@ static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out)
 int sdem_step(sdem_ctx* c, int32_t nsteps) {
   if (!c) return SDEM_E_ARG;
   if (nsteps < 0) return c->fail(SDEM_E_ARG, "nsteps < 0");
+  if (c->phase != 0) return c->fail(SDEM_E_STATE, "a step run phase by phase is open: finish it with sdem_remv_released");
   if (int rc = prepare(c)) return rc;
   const int nbox = (int)c->boxes.size();
   const int NE = SDEM_NUM_PHASES + 1;
@@ -1231,14 +1278,25 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     const int ran = batch - s.halted_steps;  // steps that did work, the last one failed if overflow
     const int ok_steps = s.overflow ? ran - 1 : ran;
-    for (int k = 0; k < ok_steps && c->prof; k++)
+    for (int k = 0; k < ok_steps && c->prof; k++) {
       for (int q = 0; q < SDEM_NUM_PHASES; q++) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, evs[(size_t)k * NE + q], evs[(size_t)k * NE + q + 1]) == hipSuccess) {
           c->prof_ms[q] += ms;
           c->prof_n[q]++;
+          // the reference's timers: the control is the pre-iteration; the
+          // fused kernels do the contact search and every force, the
+          // accelerations, integration and exchange in one pass, all of it
+          // under "CS and CF P-P"
+          const int t = q == SDEM_PHASE_CONTROL ? SDEM_TIMER_PRE_ITER : SDEM_TIMER_CSCF_PP;
+          c->timer_ms[t] += ms;
+          c->timer_ms[SDEM_TIMER_TOTAL] += ms;
         }
       }
+      c->timer_n[SDEM_TIMER_TOTAL]++;
+      c->timer_n[SDEM_TIMER_PRE_ITER]++;
+      c->timer_n[SDEM_TIMER_CSCF_PP]++;
+    }
     for (auto& x : evs) (void)hipEventDestroy(x);
     if (ok_steps > 0) {
       c->stepped = true;
@@ -1268,38 +1326,115 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
       c->spec_full = ok_steps == 0 ? std::min(2 * c->spec_full + 1, 256) : 1;
       continue;
     }
-    if (s.overflow == 3)
-      return c->fail(SDEM_E_STATE, "a particle is more than one period outside a periodic dimension, or a "
-                                   "coordinate is not finite: the state has blown up");
-    // a capacity ran out in step `ok_steps` of this batch: its input buffer
-    // is intact; grow the capacity and replay from there
-    if (++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");
-    const int kind = s.overflow, list_need = s.list_need;
-    s.iter = s.iter_at_begin;
-    s.need_rebuild = 1;
-    s.have_parts = 0;
-    s.list_need = 0;
-    s.overflow = 0;
-    std::vector<BoxState> bs;
-    e = dl(bs, c->BS, (size_t)nbox);
-    for (auto& B : bs) {
-      B.n_alive = B.n_stored;
-      B.next_wall_update = B.next_wall_at_begin;
-      if (c->first_step_pending) B.use_host_max_a = 1;
-    }
-    if (e == hipSuccess) e = hipMemcpy(c->BS, bs.data(), bs.size() * sizeof(BoxState), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->S, &s, sizeof(s), hipMemcpyHostToDevice);
-    // the slots hold what the failing and the halted steps folded in; the
-    // replayed step rebuilds its lists, so only its own maxima are needed
-    if (e == hipSuccess) e = hipMemset(c->bslots, 0, (size_t)nbox * NSET * NSLOT * sizeof(BoxSlot));
-    if (e != hipSuccess) return hip_fail(c, e, "reset scalars");
-    c->need_pre = true;
-    c->spec_ready = false;
-    int rc = kind == 2 ? grow_vlist(c, list_need) : grow_history(c);
-    if (rc) return rc;
+    if (s.overflow != 3 && ++retries > 12) return c->fail(SDEM_E_CAPACITY, "capacity still exhausted after regrowing");
+    if (int rc = rewind_and_grow(c, s)) return rc;
   }
   return SDEM_OK;
 }
+
+// ===========================================================================
+// One step phase by phase, in the order of DEMS_iterate
+// (Prtcl_DEMSystem.f90:213-248); together the calls are one sdem_step(c, 1)
+// and give its bits (the phase kernels are the fused ones' parts)
+// ===========================================================================
+namespace {
+enum { PH_NONE = 0, PH_PRE, PH_COMM, PH_PP, PH_PW, PH_ACC, PH_INT, PH_EXCH, PH_REMV };
+const char* const PHASE_CALL[] = {"", "sdem_pre_iteration", "sdem_comm_for_cntct", "sdem_find_contacts",
+                                  "sdem_find_contacts_pw", "sdem_clc_acceleration", "sdem_integrate",
+                                  "sdem_comm_for_exchange", "sdem_remv_released"};
+const int PHASE_TIMER[] = {0, SDEM_TIMER_PRE_ITER, SDEM_TIMER_COMM_CS, SDEM_TIMER_CSCF_PP, SDEM_TIMER_CSCF_PW,
+                           SDEM_TIMER_ACCELERATION, SDEM_TIMER_INTEGRATION, SDEM_TIMER_COMM_EXCHANGE,
+                           SDEM_TIMER_COMM_EXCHANGE};
+}  // namespace
+
+static int run_phase(sdem_ctx* c, int ph) {
+  if (!c) return SDEM_E_ARG;
+  if (c->phase != ph - 1)
+    return c->fail(SDEM_E_STATE, std::string(PHASE_CALL[ph]) + " out of order: the calls follow DEMS_iterate (" +
+                                     (c->phase == PH_NONE ? "the step starts with sdem_pre_iteration"
+                                                          : std::string("next: ") + PHASE_CALL[c->phase + 1]) + ")");
+  if (ph == PH_PRE)
+    if (int rc = prepare(c)) return rc;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (c->prof)
+    for (auto& x : ev)
+      if (hipEventCreate(&x) != hipSuccess) return c->fail(SDEM_E_HIP, "hipEventCreate");
+  if (ev[0]) (void)hipEventRecord(ev[0], c->stream);
+  Buf& A = c->buf[c->cur];
+  Buf& B = c->buf[1 - c->cur];
+  hipError_t e = hipSuccess;
+  switch (ph) {
+    case PH_PRE:  // UpdateNearPrtclsPW's cadence, the step's control; F/T and stale marks need nothing here
+      if (c->need_pre) {
+        e = launch_pre(c->D, A, c->stream);
+        c->need_pre = false;
+      }
+      if (e == hipSuccess) e = launch_ctl(c->D, A, c->stream);
+      break;
+    case PH_COMM:  // ghost images: the kernels take minimum images
+      break;
+    case PH_PP:  // the cell sort and candidate lists (rebuild steps), the contact test, PP/PPG/PPFix forces
+      e = launch_sort(c->D, A, c->stream);
+      if (e == hipSuccess) e = launch_vlist(c->D, c->stream);
+      if (e == hipSuccess) e = launch_phase(c->D, A, B, c->stream, 1);
+      break;
+    case PH_PW: e = launch_phase(c->D, A, B, c->stream, 2); break;
+    case PH_ACC: e = launch_phase(c->D, A, B, c->stream, 3); break;
+    case PH_INT: e = launch_phase(c->D, A, B, c->stream, 4); break;
+    case PH_EXCH: e = launch_phase(c->D, A, B, c->stream, 5); break;
+    case PH_REMV:  // the new history holds this step's records only; the buffers swap
+      c->cur = 1 - c->cur;
+      e = launch_finish(c->D, c->buf[c->cur], c->stream);
+      break;
+    default: break;
+  }
+  if (ev[1]) (void)hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && ev[0]) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+      c->timer_ms[PHASE_TIMER[ph]] += ms;
+      c->timer_ms[SDEM_TIMER_TOTAL] += ms;
+      if (ph != PH_REMV) c->timer_n[PHASE_TIMER[ph]]++;
+      if (ph == PH_REMV) c->timer_n[SDEM_TIMER_TOTAL]++;
+    }
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return hip_fail(c, e, PHASE_CALL[ph]);
+  if (ph == PH_PP || ph == PH_PW) {
+    Scalars s;
+    e = hipMemcpy(&s, c->S, sizeof(s), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(scalars)");
+    if (s.overflow) {
+      // the step's input is intact: back to its start with more capacity
+      c->phase = PH_NONE;
+      const bool blown = s.overflow == 3;
+      if (int rc = rewind_and_grow(c, s)) return rc;
+      if (blown) return SDEM_E_STATE;
+      return c->fail(SDEM_E_CAPACITY, "a capacity ran out and has been grown: run the step again from "
+                                      "sdem_pre_iteration");
+    }
+  }
+  if (ph == PH_REMV) {
+    c->phase = PH_NONE;
+    c->stepped = true;
+    c->first_step_pending = false;
+    c->spec_ready = false;
+  } else {
+    c->phase = ph;
+  }
+  return SDEM_OK;
+}
+
+int sdem_pre_iteration(sdem_ctx* c) { return run_phase(c, PH_PRE); }
+int sdem_comm_for_cntct(sdem_ctx* c) { return run_phase(c, PH_COMM); }
+int sdem_find_contacts(sdem_ctx* c) { return run_phase(c, PH_PP); }
+int sdem_find_contacts_pw(sdem_ctx* c) { return run_phase(c, PH_PW); }
+int sdem_clc_acceleration(sdem_ctx* c) { return run_phase(c, PH_ACC); }
+int sdem_integrate(sdem_ctx* c) { return run_phase(c, PH_INT); }
+int sdem_comm_for_exchange(sdem_ctx* c) { return run_phase(c, PH_EXCH); }
+int sdem_remv_released(sdem_ctx* c) { return run_phase(c, PH_REMV); }
 
 // ===========================================================================
 // Readers (the selected box; ascending particle id)
@@ -1577,6 +1712,16 @@ int sdem_set_profiling(sdem_ctx* c, int32_t enable) {
   if (!c) return SDEM_E_ARG;
   c->prof = enable != 0;
   for (int q = 0; q < SDEM_NUM_PHASES; q++) { c->prof_ms[q] = 0.0; c->prof_n[q] = 0; }
+  for (int q = 0; q < SDEM_NUM_TIMERS; q++) { c->timer_ms[q] = 0.0; c->timer_n[q] = 0; }
+  return SDEM_OK;
+}
+
+int sdem_get_timers(sdem_ctx* c, double tot_ms[SDEM_NUM_TIMERS], int64_t calls[SDEM_NUM_TIMERS]) {
+  if (!c || !tot_ms || !calls) return SDEM_E_ARG;
+  for (int q = 0; q < SDEM_NUM_TIMERS; q++) {
+    tot_ms[q] = c->timer_ms[q];
+    calls[q] = c->timer_n[q];
+  }
   return SDEM_OK;
 }
 

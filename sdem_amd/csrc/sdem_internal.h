@@ -193,6 +193,7 @@ struct Dev {
   int* cand;             // candidate entries by packed slot: entry q of slot t at q*cap+t
   unsigned* clist;
   int* ccnt;
+  double* ft;            // phase entry points: F, T, then a, alpha (6 x cap)
   const double4* fpk;
   const int *fid, *ftype, *fstart;
   int bs_contact, bs_force;  // block sizes (divisors of BOX_ALIGN)

@@ -1151,19 +1151,22 @@ __device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
 // evaluates it from (ContactForce_PP :732-756, _PPG :761-798, _PPFix
 // :803-827, _PW :832-856), so the routine is inlined once.  The records the
 // particle owns go to slot k of B.
+// `parts` selects the particle contacts (1), the walls (2) or both (3), in
+// that order: the phase entry points run them in two kernels, which continue
+// the same sums.
 __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, const Buf& B, int k, int s,
                                              const Own& o, const unsigned* __restrict__ clist,
                                              const int* __restrict__ ccnt, const double4* __restrict__ fpk,
                                              const int* __restrict__ fid, const int* __restrict__ ftype,
-                                             Acc& acc) {
+                                             Acc& acc, int parts = 3) {
   const int cap = P.cap;
   const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z);
   const double ri = o.pi.w;
   const int idi = o.idi, ti = o.ti;
   const int nc = min(ccnt[k], P.hcap);  // more: cnt_over is set
   uint64_t mask = o.mask0;
-  const int ntot = nc + __popcll(mask);
-  for (int q = 0; q < ntot; q++) {
+  const int ntot = (parts & 2) ? nc + __popcll(mask) : nc;
+  for (int q = (parts & 1) ? 0 : nc; q < ntot; q++) {
     d3 nvec, va, vb, wa, wb;
     double ra, rb, ovrlp;
     Hist h;
@@ -1255,6 +1258,14 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
   }
 }
 
+// Accelerations (DEMS_clc_Acceleration, Prtcl_DEMSystem.f90:360-375):
+// a = F / m + g, alpha = T / I of the particle's type.
+__device__ __forceinline__ void accel(const Params& P, int ti, d3 F, d3 T, d3& a1, d3& al1) {
+  const double mass = P.mass[ti - 1], inertia = P.inertia[ti - 1];
+  a1 = add3(div3(F, mass), D3(P.g[0], P.g[1], P.g[2]));
+  al1 = div3(T, inertia);
+}
+
 // The integration inputs beyond the contact loop's (history levels, theta,
 // mark).  PIM / PRIM: translational / rotational integration method fixed
 // at compile time (0: read from Params); with fixed methods they are loaded
@@ -1297,26 +1308,18 @@ struct Integ {
     if (put_ids) mark = A.mark[s];
   }
 
-  // Acceleration (DEMS_clc_Acceleration :360-375), integration
-  // (Prtcl_Integrate :15-88), periodic wrap / deletion (Comm_For_Exchange),
-  // the new state in slot k of B.  Returns whether the particle left
-  // through an open side; move2 = its squared move, vnext / anext = |v|,
-  // |a| of the new state when the next step refreshes the near-wall lists.
-  __device__ __forceinline__ bool integrate(const Params& P, const Scalars* S, const Buf& A, const Buf& B, int k,
-                                            int s, const Own& o, d3 F, d3 T, bool put_ids, bool put_wall,
-                                            bool upd_next, double& move2, double& vnext, double& anext) {
+  // Integration (Prtcl_Integrate :15-88) of the particle in slot s of A
+  // with accelerations a1, al1: the new velocity levels, accelerations and
+  // rotation into slot k of B; returns the new position, before any wrap,
+  // and linVel(1) in lin1.
+  __device__ __forceinline__ d3 advance(const Params& P, const Scalars* S, const Buf& A, const Buf& B, int k, int s,
+                                        const Own& o, d3 a1, d3 al1, d3& lin1) {
     const int cap = P.cap;
     const int pim = PIM ? PIM : P.pi_method, prim = PRIM ? PRIM : P.pri_method;
     const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z), vi = o.vi, wi = o.wi;
-    const int ti = o.ti;
     const bool rv2 = PIM ? false : S->rv2_stored[A.idx] != 0;
-    const double mass = P.mass[ti - 1], inertia = P.inertia[ti - 1];
-    const d3 a1 = add3(div3(F, mass), D3(P.g[0], P.g[1], P.g[2]));
-    const d3 al1 = div3(T, inertia);
-
     const double dt = P.dt;
     d3 pos = xi;
-    d3 lin1;  // linVel(1) after the translational update
     if (pim == 1) {
       d3 sft = mul3(vi, dt);
       pos = D3(pos.x + sft.x, pos.y + sft.y, pos.z + sft.z);
@@ -1388,14 +1391,15 @@ struct Integ {
       st3n(B.ra + 3 * cap, cap, k, al2);
     }
     st3n(B.th, cap, k, th);
+    return pos;
+  }
 
-    // the move of this step, before any periodic wrap
-    {
-      const double mx = pos.x - xi.x, my = pos.y - xi.y, mz = pos.z - xi.z;
-      move2 = mx * mx + my * my + mz * mz;
-    }
-
-    // periodic wrap / deletion (Comm_For_Exchange)
+  // Periodic wrap / deletion (Comm_For_Exchange) of the new position `pos`
+  // and the stores of the new state's position, id, mark, type and next
+  // step's near-wall mask into slot k of B.  Returns whether the particle
+  // left through an open side.
+  __device__ __forceinline__ bool exchange(const Params& P, const Buf& A, const Buf& B, int k, int s, const Own& o,
+                                           d3 pos, bool put_ids, bool put_wall, bool upd_next) {
     bool dead = false;
     double pv[3] = {pos.x, pos.y, pos.z};
     for (int d = 0; d < 3; d++) {
@@ -1406,6 +1410,7 @@ struct Integ {
         dead = true;
       }
     }
+    const int ti = o.ti;
     B.pos[k] = make_double4(pv[0], pv[1], pv[2], o.pi.w);
     if (put_ids) {
       B.id[k] = o.idi;
@@ -1415,13 +1420,32 @@ struct Integ {
     // the next step's near-wall list is built here from the new state
     // (UpdateNearPrtclsPW runs at the start of that step, after deletion)
     if (put_wall) B.wmask[k] = (upd_next && !dead) ? near_wall_mask(P, pv[0], pv[1], pv[2]) : o.mask0;
+    return dead;
+  }
+
+  // Accelerations, integration, wrap / deletion and the new state in slot
+  // k of B, for the contact forces F, T.  Returns whether the particle left
+  // through an open side; move2 = its squared move, vnext / anext = |v|,
+  // |a| of the new state when the next step refreshes the near-wall lists.
+  __device__ __forceinline__ bool integrate(const Params& P, const Scalars* S, const Buf& A, const Buf& B, int k,
+                                            int s, const Own& o, d3 F, d3 T, bool put_ids, bool put_wall,
+                                            bool upd_next, double& move2, double& vnext, double& anext) {
+    d3 a1, al1, lin1;
+    accel(P, o.ti, F, T, a1, al1);
+    const d3 pos = advance(P, S, A, B, k, s, o, a1, al1, lin1);
+    // the move of this step, before any periodic wrap
+    {
+      const double mx = pos.x - o.pi.x, my = pos.y - o.pi.y, mz = pos.z - o.pi.z;
+      move2 = mx * mx + my * my + mz * mz;
+    }
+    const bool dead = exchange(P, A, B, k, s, o, pos, put_ids, put_wall, upd_next);
     if (upd_next && !dead) {
       vnext = norm3(lin1);
       anext = norm3(a1);  // linAcc(1) at the start of the next step
     }
     if (P.keep_forces) {
-      st3n(B.force, cap, k, F);
-      st3n(B.force + 3 * cap, cap, k, T);
+      st3n(B.force, P.cap, k, F);
+      st3n(B.force + 3 * P.cap, P.cap, k, T);
     }
     return dead;
   }
@@ -1554,6 +1578,157 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // ---------------------------------------------------------------------------
+// The force step in the reference's phases (the entry points
+// sdem_find_contacts ... sdem_comm_for_exchange): the parts of k_force in
+// kernels of their own, the same operations in the same order, so that a
+// step run phase by phase gives k_force's bits.  F and T, then the
+// accelerations, cross between the phases in `ft` (6 x cap doubles, slot k),
+// the new position before the wrap in B.pos.
+// ---------------------------------------------------------------------------
+// FindContacts (Prtcl_DEMSystem.f90:225): the particle contacts' forces and
+// records
+__global__ __launch_bounds__(256) void k_ph_pp(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                               Buf A, Buf B, Scalars* S, BoxState* BS, const int* __restrict__ perm,
+                                               const unsigned* __restrict__ clist, const int* __restrict__ ccnt,
+                                               const double4* __restrict__ fpk, const int* __restrict__ fid,
+                                               const int* __restrict__ ftype, double* __restrict__ ft,
+                                               BoxSlot* __restrict__ bslots) {
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  Acc acc = acc_zero();
+  if (k - P.off < BS[box].n_alive) {
+    const int s = BS[box].rebuild ? perm[k] : k;
+    const Own o = load_own(P, A, s);
+    contact_loop(P, A, B, k, s, o, clist, ccnt, fpk, fid, ftype, acc, 1);
+    st3(ft, P.cap, k, acc.F);
+    st3(ft + 3 * (size_t)P.cap, P.cap, k, acc.T);
+    B.hcnt[k] = acc.hc;
+  }
+  wave_flush(bslots, box, true, false, acc.nkind, 0.0, false, 0.0, 0.0, false);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && S->cnt_over) atomicMax(&S->overflow, 1);
+  if (acc.overflow) atomicMax(&S->overflow, 1);
+}
+
+// FindContactsPW (:230): the walls, continuing the sums and the records
+__global__ __launch_bounds__(256) void k_ph_pw(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                               Buf A, Buf B, Scalars* S, BoxState* BS, const int* __restrict__ perm,
+                                               const unsigned* __restrict__ clist, const int* __restrict__ ccnt,
+                                               const double4* __restrict__ fpk, const int* __restrict__ fid,
+                                               const int* __restrict__ ftype, double* __restrict__ ft,
+                                               BoxSlot* __restrict__ bslots) {
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  Acc acc = acc_zero();
+  if (k - P.off < BS[box].n_alive) {
+    const int s = BS[box].rebuild ? perm[k] : k;
+    const Own o = load_own(P, A, s);
+    acc.F = ld3(ft, P.cap, k);
+    acc.T = ld3(ft + 3 * (size_t)P.cap, P.cap, k);
+    acc.hc = B.hcnt[k];
+    contact_loop(P, A, B, k, s, o, clist, ccnt, fpk, fid, ftype, acc, 2);
+    st3(ft, P.cap, k, acc.F);
+    st3(ft + 3 * (size_t)P.cap, P.cap, k, acc.T);
+    B.hcnt[k] = acc.hc;
+  }
+  wave_flush(bslots, box, true, false, acc.nkind, 0.0, false, 0.0, 0.0, false);
+  if (acc.overflow) atomicMax(&S->overflow, 1);
+}
+
+// clc_Acceleration (:235): ft holds a and alpha afterwards
+__global__ __launch_bounds__(256) void k_ph_accel(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                                  Buf A, Buf B, const Scalars* S, const BoxState* BS,
+                                                  const int* __restrict__ perm, double* __restrict__ ft) {
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k - P.off >= BS[box].n_alive) return;
+  const int s = BS[box].rebuild ? perm[k] : k;
+  const d3 F = ld3(ft, P.cap, k), T = ld3(ft + 3 * (size_t)P.cap, P.cap, k);
+  if (P.keep_forces) {
+    st3n(B.force, P.cap, k, F);
+    st3n(B.force + 3 * P.cap, P.cap, k, T);
+  }
+  d3 a1, al1;
+  accel(P, A.type[s], F, T, a1, al1);
+  st3(ft, P.cap, k, a1);
+  st3(ft + 3 * (size_t)P.cap, P.cap, k, al1);
+}
+
+// Prtcl_Integrate (:240): the new velocities, accelerations and rotation,
+// the new position before any wrap in B.pos, the step's largest move
+template <int PIM, int PRIM>
+__global__ __launch_bounds__(256) void k_ph_integ(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                                  Buf A, Buf B, const Scalars* S, const BoxState* BS,
+                                                  const int* __restrict__ perm, const double* __restrict__ ft,
+                                                  BoxSlot* __restrict__ bslots) {
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const StepFlags f = step_flags(S, BS[box]);
+  double move2 = 0.0;
+  if (k - P.off < BS[box].n_alive) {
+    const int s = f.rebuilt ? perm[k] : k;
+    Own o;
+    o.pi = A.pos[s];
+    o.ti = A.type[s];
+    o.idi = 0;
+    o.vi = ld3(A.lv, P.cap, s);
+    o.wi = ld3(A.rv, P.cap, s);
+    o.mask0 = 0ull;
+    Integ<PIM, PRIM> in;
+    in.preload(P, S, A, s, o.vi, false);
+    const d3 a1 = ld3(ft, P.cap, k), al1 = ld3(ft + 3 * (size_t)P.cap, P.cap, k);
+    d3 lin1;
+    const d3 pos = in.advance(P, S, A, B, k, s, o, a1, al1, lin1);
+    const double mx = pos.x - o.pi.x, my = pos.y - o.pi.y, mz = pos.z - o.pi.z;
+    move2 = mx * mx + my * my + mz * mz;
+    B.pos[k] = make_double4(pos.x, pos.y, pos.z, o.pi.w);
+  }
+  wave_flush(bslots, box, false, true, 0ull, move2, false, 0.0, 0.0, false);
+}
+
+// Comm_For_Exchange (:245): wrap or delete the new positions; ids, marks,
+// types and the next step's near-wall masks of the new state
+__global__ __launch_bounds__(256) void k_ph_exch(const Params* __restrict__ PB, const int* __restrict__ blk_box,
+                                                 Buf A, Buf B, Scalars* S, const BoxState* BS,
+                                                 const int* __restrict__ perm, BoxSlot* __restrict__ bslots) {
+  if (S->halt || S->overflow >= 2) return;
+  const int box = block_box(blk_box);
+  const Params& P = PB[box];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const StepFlags f = step_flags(S, BS[box]);
+  double vnext = 0.0, anext = 0.0;
+  bool dead = false;
+  if (k - P.off < BS[box].n_alive) {
+    const int s = f.rebuilt ? perm[k] : k;
+    Own o;
+    o.pi = A.pos[s];
+    o.ti = A.type[s];
+    o.idi = f.put_ids ? A.id[s] : 0;
+    o.mask0 = (f.put_wall && !f.upd_next) ? A.wmask[s] : 0ull;
+    const double4 q = B.pos[k];
+    Integ<0, 0> in;
+    dead = in.exchange(P, A, B, k, s, o, D3(q.x, q.y, q.z), f.put_ids, f.put_wall, f.upd_next);
+    if (f.upd_next && !dead) {
+      vnext = norm3(ld3(B.lv, P.cap, k));
+      anext = norm3(ld3(B.la, P.cap, k));  // linAcc(1) at the start of the next step
+    }
+    if (dead) B.hcnt[k] = 0;
+  }
+  wave_flush(bslots, box, false, true, 0ull, 0.0, f.upd_next, vnext, anext, dead);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->have_parts = 1;
+    S->rv2_stored[B.idx] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers (called from sdem_api.cpp).  Particle kernels cover all
 // `nslots` storage slots (a multiple of BOX_ALIGN) with blocks of `bs`
 // threads (a divisor of BOX_ALIGN).
@@ -1608,6 +1783,44 @@ hipError_t launch_vlist(const Dev& D, hipStream_t st) {
   hipLaunchKernelGGL(k_vlist, dim3(nblk(D.nslots, 256)), dim3(256), 0, st, D.PB, D.blk_box,
                      D.S, D.BS, D.pk, D.pid, D.start, D.vlist, D.vkey, D.act, D.cand, D.ccnt, D.fpk,
                      D.fid, D.fstart);
+  return hipGetLastError();
+}
+
+// one phase of the step's force part (sdem_find_contacts ... _exchange):
+// 1 contact test + particle contacts, 2 walls, 3 accelerations,
+// 4 integration, 5 exchange
+hipError_t launch_phase(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, int phase) {
+  const unsigned nb = nblk(D.nslots, 256);
+  switch (phase) {
+    case 1:
+      hipLaunchKernelGGL(k_contact<0>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
+                         D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                         slot_set(D, A.idx), slot_set(D, B.idx));
+      hipLaunchKernelGGL(k_ph_pp, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm, D.clist,
+                         D.ccnt, D.fpk, D.fid, D.ftype, D.ft, slot_set(D, B.idx));
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_ph_pw, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm, D.clist,
+                         D.ccnt, D.fpk, D.fid, D.ftype, D.ft, slot_set(D, B.idx));
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_ph_accel, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm, D.ft);
+      break;
+    case 4:
+      if (D.pim == 2 && D.prim == 2)
+        hipLaunchKernelGGL((k_ph_integ<2, 2>), dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm,
+                           D.ft, slot_set(D, B.idx));
+      else
+        hipLaunchKernelGGL((k_ph_integ<0, 0>), dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm,
+                           D.ft, slot_set(D, B.idx));
+      break;
+    case 5:
+      hipLaunchKernelGGL(k_ph_exch, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm,
+                         slot_set(D, B.idx));
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -8,7 +8,7 @@
 !                                 before the first and after the last step
 !   sdem_main --abi-check         print the size of struct sdem_options and exit
 !   sdem_main --dry-run <file.prm>  read the input and make the particles, print
-!                                 a summary, touch no GPU
+!                                 a summary and the walls' corners, touch no GPU
 program main_DEM
   use, intrinsic :: iso_c_binding
   use sdem_capi
@@ -47,6 +47,10 @@ program main_DEM
     print '(A,3ES14.6)', "max point ", o%domain_max
     print '(A,*(1X,I0))', "bins", DEMProperty%nPrtcl_in_Bin
     print '(A,4ES14.6)', "first particle ", GPrtcl_PosR(:, 1)
+    do i = 1, DEMGeometry%nPW
+      print '(A,I0,12ES24.16)', "wall ", i, DEMGeometry%Point1(:, i), DEMGeometry%Point2(:, i), &
+            DEMGeometry%Point3(:, i), DEMGeometry%Point4(:, i)
+    end do
     stop
   end if
   chDump = ""

@@ -95,6 +95,50 @@ module sdem_capi
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
     end function
+    ! the calls of DEMS_iterate one at a time (Prtcl_DEMSystem.f90:213-248)
+    integer(c_int) function sdem_pre_iteration(ctx) bind(C, name="sdem_pre_iteration")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_comm_for_cntct(ctx) bind(C, name="sdem_comm_for_cntct")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_find_contacts(ctx) bind(C, name="sdem_find_contacts")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_find_contacts_pw(ctx) bind(C, name="sdem_find_contacts_pw")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_clc_acceleration(ctx) bind(C, name="sdem_clc_acceleration")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_integrate(ctx) bind(C, name="sdem_integrate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_comm_for_exchange(ctx) bind(C, name="sdem_comm_for_exchange")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_remv_released(ctx) bind(C, name="sdem_remv_released")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function sdem_set_profiling(ctx, enable) bind(C, name="sdem_set_profiling")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function
+    integer(c_int) function sdem_get_timers(ctx, tot_ms, calls) bind(C, name="sdem_get_timers")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tot_ms(9)
+      integer(c_int64_t), intent(out) :: calls(9)
+    end function
     integer(c_int) function sdem_get_num_particles(ctx, n) bind(C, name="sdem_get_num_particles")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -31,6 +31,12 @@ CTM_CONSTANT_TORQUE, CTM_VARIABLE_TORQUE = 1, 2
 PIM_FE, PIM_AB2, PIM_AB3 = 1, 2, 3
 KIND_PP, KIND_PPG, KIND_PPFIX, KIND_PW = 1, 2, 3, 4
 PHASE_NAMES = ("control", "bin_sort", "search", "contact", "force")
+# the calls of DEMS_iterate (Prtcl_DEMSystem.f90:213-248), one entry point each
+STEP_PHASES = ("pre_iteration", "comm_for_cntct", "find_contacts", "find_contacts_pw", "clc_acceleration",
+               "integrate", "comm_for_exchange", "remv_released")
+# the reference's step timers (Prtcl_DEMSystem.f90:29-37), sdem_get_timers order
+TIMER_NAMES = ("total", "pre_iteration", "comm_for_contact", "cs_and_cf_pp", "cs_and_cf_pw", "acceleration",
+               "integration", "comm_for_exchange", "write_to_file")
 
 
 class SdemError(RuntimeError):
@@ -110,6 +116,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
     }
     optional = {
         "set_profiling": ([V, C.c_int32], C.c_int),
+        "get_timers": ([V, _DP, _LP], C.c_int),
         "get_profile": ([V, _DP, _LP], C.c_int),
         "get_device_bytes": ([V, _LP], C.c_int),
         "set_tuning": ([V, _DP, C.c_double, C.c_int32], C.c_int),
@@ -131,6 +138,8 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
     for name, (args, res) in sig.items():
         fn = f(name)
         fn.argtypes, fn.restype = args, res
+    for name in STEP_PHASES:
+        optional[name] = ([V], C.c_int)
     for name, (args, res) in optional.items():
         if hasattr(lib, prefix + name):
             fn = f(name)
@@ -256,6 +265,31 @@ class Engine:
     def iterate(self, nsteps: int = 1) -> None:
         """nsteps x DEMS_iterate (Prtcl_DEMSystem.f90:201-248)."""
         self._check(self._fn("step")(self.h, int(nsteps)), "step")
+
+    def phase(self, name: str) -> None:
+        """One call of DEMS_iterate (STEP_PHASES; sdem_<name>)."""
+        self._check(self._fn(name)(self.h), name)
+
+    def iterate_phases(self, nsteps: int = 1) -> None:
+        """nsteps steps, each as its eight phase calls in the reference's
+        order; a step whose search or forces ran out of capacity is run again
+        from its start (the library has grown the capacity)."""
+        for _ in range(int(nsteps)):
+            for attempt in range(13):
+                try:
+                    for name in STEP_PHASES:
+                        self.phase(name)
+                    break
+                except SdemError as err:
+                    if err.code != SDEM_E_CAPACITY or attempt == 12:
+                        raise
+
+    def get_timers(self) -> Dict[str, tuple]:
+        """The reference's step timers: name -> (total ms, count)."""
+        ms = np.zeros(len(TIMER_NAMES))
+        n = np.zeros(len(TIMER_NAMES), np.int64)
+        self._check(self._fn("get_timers")(self.h, _dptr(ms), n.ctypes.data_as(_LP)), "get_timers")
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(TIMER_NAMES)}
 
     # ------------------------------------------------------------- readers
     def num_particles(self) -> int:

@@ -154,3 +154,115 @@ def scenario(state: dict, c=CASE, cf_type=2, pi=2, pri=2) -> dict:
     if cf_type == native.CFT_LINEAR:
         sc["linear"] = (2.0e4, 0.8)
     return sc
+
+
+# ---------------------------------------------------------------------------
+# an hourglass case in the format of the reference's Input/SandBox_DEM.prm:
+# the program's own geometry (GeometrySource = 0) and the sand box's
+# materials; 3 mm spheres filled into the neck from its walls (the first
+# lattice centres 1.05 r from the fill box's sides) and from just above its
+# lower edge, moving at 0.3 m/s
+# ---------------------------------------------------------------------------
+SANDBOX = dict(num=1200, steps=300, dt=1.0e-5, diam=3.0e-3, dens=2500.0, E=1.0e6, nu=0.23,
+               mk_min=(-0.015, -0.0515, -0.015), mk_max=(0.015, 0.12, 0.015))
+
+
+def sandbox_prm_text(c=SANDBOX) -> str:
+    mn, mx = c["mk_min"], c["mk_max"]
+    return f"""!
+! hourglass: the program's own walls (GeometrySource = 0), spheres in the neck
+!
+&DEMOptions
+  RestartFlag = F
+  numPrtclFix = 0
+  numPrtcl = {c['num']}
+  dtDEM = {c['dt']:.6E}
+  ifirstDEM = 1
+  ilastDEM = {c['steps']}
+  gravity = 0.0 -9.81 0.0
+  minpoint = -0.075 -0.30 -0.075
+  maxpoint = 0.075 0.30 0.075
+  isPeriodic = F,F,F
+  Prtcl_cs_ratio = 1.0
+  CS_Method = 1
+  CF_Type = 2
+  CT_Model = 1
+  PI_Method = 3
+  PRI_Method = 3
+  numPrtcl_Type = 1
+  numWall_type = 1
+  CntctList_Size = 6
+  CS_numlvls = 0
+  Base_wall_id = 10000000
+  Wall_max_update_iter = 100
+  Wall_neighbor_ratio = 2.00
+  RunName = "SandBox"
+  Res_Dir = "./"
+  RestartDir = "./"
+  SaveVisuDEM = 100000
+  BackupFreqDEM = 100000
+  Cmd_LFile_Freq = 50
+  LF_file_lvl = 5
+  LF_cmdw_lvl = 3
+  GeometrySource = 0
+  Geom_Dir = "DEMGeom.stl"
+/
+
+&ParticlePhysicalProperty
+  Bin_Divided = 1
+  Density = {c['dens']}
+  Diameter = {c['diam']}
+  YoungsModulus_P = {c['E']:.4E}
+  PoissonRatio_P = {c['nu']}
+  FrictionCoe_s_PP = 0.40
+  FrictionCoe_k_PP = 0.40
+  FrictionCoe_Roll_PP = 0.10
+  RestitutionCoe_n_PP = 0.30
+  RestitutionCoe_t_PP = 0.80
+/
+
+&WallPhysicalProperty
+  YoungsModulus_W = {c['E']:.4E}
+  PoissonRatio_W = {c['nu']}
+  FrictionCoe_s_PW = 0.40
+  FrictionCoe_k_PW = 0.40
+  FrictionCoe_Roll_PW = 0.10
+  RestitutionCoe_n_PW = 0.30
+  RestitutionCoe_t_PW = 0.80
+/
+
+&ParticleMakingOption
+  MkPrtclMinpoint = {mn[0]} {mn[1]} {mn[2]}
+  MkPrtclMaxpoint = {mx[0]} {mx[1]} {mx[2]}
+  Fill_order = "x","z","y"
+  Distance_Ratio = 1.05
+  VelMag = 0.3
+/
+"""
+
+
+def hourglass_walls():
+    """The 22 walls of GeometrySource = 0 from the fixture written out of the
+    reference's source (tests/golden/make_hourglass_fixture.py)."""
+    import json
+    import os
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "hourglass_quads.json")) as f:
+        quads = json.load(f)["quads"]
+    return [dict(p1=tuple(q["corners"][0]), p2=tuple(q["corners"][1]), p3=tuple(q["corners"][2]),
+                 p4=tuple(q["corners"][3]), user_id=q["user_id"], wall_type=q["wall_type"],
+                 both_side=q["both_side"], is_infinite=q["infinite"], trans_vel=(0.0, 0.0, 0.0))
+            for q in quads]
+
+
+def sandbox_scenario(state: dict, c=SANDBOX) -> dict:
+    """The hourglass case for native.build_engine, from a dumped state."""
+    opts = scenarios.base_options(dt=c["dt"], gravity=(0.0, -9.81, 0.0), domain_min=(-0.075, -0.30, -0.075),
+                                  domain_max=(0.075, 0.30, 0.075), num_prtcl=c["num"], cf_type=native.CFT_NLIN,
+                                  pi_method=native.PIM_AB3, pri_method=native.PIM_AB3)
+    return dict(options=opts,
+                ptypes=dict(diameter=[c["diam"]], density=[c["dens"]], youngs=[c["E"]], poisson=[c["nu"]]),
+                pp=(0.40, 0.40, 0.10, 0.30, 0.80),
+                wtypes=dict(youngs=[c["E"]], poisson=[c["nu"]]), pw=(0.40, 0.40, 0.10, 0.30, 0.80),
+                walls=hourglass_walls(), state=state)
