@@ -451,6 +451,17 @@ hipError_t dl(std::vector<T>& v, const T* src, size_t n) {
   return hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
 }
 
+// rows x len elements of a row-major device array whose rows are `cap` long,
+// from column `off` on (one box's slots of a per-particle array):
+// v[r * len + i] = src[r * cap + off + i]
+template <class T>
+hipError_t dl_box(std::vector<T>& v, const T* src, size_t cap, size_t rows, size_t off, size_t len) {
+  v.resize(rows * len);
+  if (!rows || !len) return hipSuccess;
+  return hipMemcpy2D(v.data(), len * sizeof(T), src + off, cap * sizeof(T), len * sizeof(T), rows,
+                     hipMemcpyDeviceToHost);
+}
+
 template <class T>
 hipError_t put_field(T* dst, const T& v) {
   return hipMemcpy(dst, &v, sizeof(T), hipMemcpyHostToDevice);
@@ -1539,40 +1550,41 @@ int sdem_get_state(sdem_ctx* c, int32_t capn, int32_t* nout, int32_t* id, int32_
   *nout = n;
   if (capn < n) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
   Buf& A = c->buf[c->cur];
-  const size_t cap = (size_t)c->nslots;
+  // the selected box's slots only: [off, off + bcap) of every array
+  const size_t cap = (size_t)c->nslots, off = (size_t)b.off, len = (size_t)b.bcap;
   std::vector<double> lv, la, th, rv, ra;
   std::vector<double4> p4;
   std::vector<int> ty, mk;
-  hipError_t e = dl(p4, A.pos, cap);
-  if (e == hipSuccess) e = dl(lv, A.lv, cap * 3 * ts);
-  if (e == hipSuccess) e = dl(la, A.la, cap * 3 * c->ta);
-  if (e == hipSuccess) e = dl(th, A.th, cap * 3);
-  if (e == hipSuccess) e = dl(rv, A.rv, cap * 3 * rs);
-  if (e == hipSuccess) e = dl(ra, A.ra, cap * 3 * c->tr);
-  if (e == hipSuccess) e = dl(ty, A.type, cap);
-  if (e == hipSuccess) e = dl(mk, A.mark, cap);
+  hipError_t e = dl_box(p4, A.pos, cap, 1, off, len);
+  if (e == hipSuccess) e = dl_box(lv, A.lv, cap, 3 * ts, off, len);
+  if (e == hipSuccess) e = dl_box(la, A.la, cap, 3 * c->ta, off, len);
+  if (e == hipSuccess) e = dl_box(th, A.th, cap, 3, off, len);
+  if (e == hipSuccess) e = dl_box(rv, A.rv, cap, 3 * rs, off, len);
+  if (e == hipSuccess) e = dl_box(ra, A.ra, cap, 3 * c->tr, off, len);
+  if (e == hipSuccess) e = dl_box(ty, A.type, cap, 1, off, len);
+  if (e == hipSuccess) e = dl_box(mk, A.mark, cap, 1, off, len);
   bool rv2 = true;
   if (e == hipSuccess) e = rv2_stored(c, &rv2);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(state)");
   for (int q = 0; q < n; q++) {
-    const size_t k = (size_t)idx[q].second;
+    const size_t k = (size_t)idx[q].second - off;
     if (id) id[q] = idx[q].first;
     if (ptype) ptype[q] = ty[k];
     if (usr_mark) usr_mark[q] = mk[k];
     if (posr) { posr[4 * q] = p4[k].x; posr[4 * q + 1] = p4[k].y; posr[4 * q + 2] = p4[k].z; posr[4 * q + 3] = p4[k].w; }
     for (int l = 0; l < ts; l++)
       for (int d = 0; d < 3; d++) {
-        if (lin_vel) lin_vel[((size_t)q * ts + l) * 3 + d] = lv[((size_t)l * 3 + d) * cap + k];
+        if (lin_vel) lin_vel[((size_t)q * ts + l) * 3 + d] = lv[((size_t)l * 3 + d) * len + k];
         // stored level l-1 holds linAcc(l); linAcc(1) is the stored level 0 (FE: the only one)
-        if (lin_acc) lin_acc[((size_t)q * ts + l) * 3 + d] = la[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * cap + k];
+        if (lin_acc) lin_acc[((size_t)q * ts + l) * 3 + d] = la[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * len + k];
       }
-    if (theta) for (int d = 0; d < 3; d++) theta[3 * q + d] = th[(size_t)d * cap + k];
+    if (theta) for (int d = 0; d < 3; d++) theta[3 * q + d] = th[(size_t)d * len + k];
     for (int l = 0; l < rs; l++)
       for (int d = 0; d < 3; d++) {
         if (rot_vel)
           rot_vel[((size_t)q * rs + l) * 3 + d] =
-              (l == 1 && !rv2) ? lv[(size_t)d * cap + k] : rv[((size_t)l * 3 + d) * cap + k];
-        if (rot_acc) rot_acc[((size_t)q * rs + l) * 3 + d] = ra[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * cap + k];
+              (l == 1 && !rv2) ? lv[(size_t)d * len + k] : rv[((size_t)l * 3 + d) * len + k];
+        if (rot_acc) rot_acc[((size_t)q * rs + l) * 3 + d] = ra[((size_t)(l == 0 ? 0 : l - 1) * 3 + d) * len + k];
       }
   }
   return SDEM_OK;
@@ -1599,15 +1611,15 @@ int sdem_get_forces(sdem_ctx* c, int32_t capn, int32_t* id, double* f, double* t
   int n = (int)idx.size();
   if (capn < n) return c->fail(SDEM_E_CAPACITY, "output arrays too small");
   std::vector<double> fb;
-  const size_t cap = (size_t)c->nslots;
-  hipError_t e = dl(fb, c->buf[c->cur].force, cap * 6);
+  const size_t cap = (size_t)c->nslots, off = (size_t)b.off, len = (size_t)b.bcap;
+  hipError_t e = dl_box(fb, c->buf[c->cur].force, cap, 6, off, len);
   if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(forces)");
   for (int q = 0; q < n; q++) {
-    const size_t k = (size_t)idx[q].second;
+    const size_t k = (size_t)idx[q].second - off;
     if (id) id[q] = idx[q].first;
     for (int d = 0; d < 3; d++) {
-      if (f) f[3 * q + d] = fb[(size_t)d * cap + k];
-      if (t) t[3 * q + d] = fb[(size_t)(3 + d) * cap + k];
+      if (f) f[3 * q + d] = fb[(size_t)d * len + k];
+      if (t) t[3 * q + d] = fb[(size_t)(3 + d) * len + k];
     }
   }
   return SDEM_OK;
@@ -1629,17 +1641,17 @@ static int collect(sdem_ctx* c, std::vector<Rec>& recs) {
     std::vector<std::pair<int, int>> idx;
     if (int rc = live_index(c, idx)) return rc;
     Buf& A = c->buf[c->cur];
-    const size_t cap = (size_t)c->nslots;
+    const size_t cap = (size_t)c->nslots, off = (size_t)b.off, len = (size_t)b.bcap;
     std::vector<int> hc, hk;
     std::vector<double4> hv;
-    hipError_t e = dl(hc, A.hcnt, cap);
-    if (e == hipSuccess) e = dl(hk, A.hkey, cap * c->hcap);
-    if (e == hipSuccess) e = dl(hv, A.hval, cap * c->hcap);
+    hipError_t e = dl_box(hc, A.hcnt, cap, 1, off, len);
+    if (e == hipSuccess) e = dl_box(hk, A.hkey, cap, (size_t)c->hcap, off, len);
+    if (e == hipSuccess) e = dl_box(hv, A.hval, cap, (size_t)c->hcap, off, len);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
     for (auto& pr : idx) {
-      const size_t k = (size_t)pr.second;
+      const size_t k = (size_t)pr.second - off;
       for (int q = 0; q < hc[k]; q++) {
-        const size_t slot = (size_t)q * cap + k;
+        const size_t slot = (size_t)q * len + k;
         recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), hv[slot]});
       }
     }
