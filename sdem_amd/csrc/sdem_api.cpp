@@ -248,7 +248,7 @@ struct sdem_ctx {
   int* pid = nullptr;
   int *vlist = nullptr, *vkey = nullptr, *cand = nullptr;
   int4* act = nullptr;
-  unsigned* clist = nullptr;
+  uint2* clist = nullptr;
   int* ccnt = nullptr;
   double* ft = nullptr;  // phase entry points: F, T, then a, alpha (6 x cap)
   int64_t nkeys = 0;
@@ -401,7 +401,12 @@ int make_grid(sdem_ctx* c, const Box& b, double cell_len0, double rmax, int64_t 
     }
     G->ncell *= G->n[d];
   }
-  if (G->ncell > (int64_t)1 << 30) return c->fail(SDEM_E_ARG, "search grid too large");
+  G->nkey = (int64_t)BRICK * BRICK * BRICK;
+  for (int d = 0; d < 3; d++) {
+    G->nb[d] = (G->n[d] + BRICK - 1) / BRICK;
+    G->nkey *= G->nb[d];
+  }
+  if (G->nkey > (int64_t)1 << 30) return c->fail(SDEM_E_ARG, "search grid too large");
   return SDEM_OK;
 }
 
@@ -501,7 +506,7 @@ static int upload_state(sdem_ctx* c) {
   if (off > INT32_MAX / 2) return c->fail(SDEM_E_CAPACITY, "too many particles for one context");
   c->nslots = off;
   const size_t cap = (size_t)off;
-  c->hcap = c->hist_slots_hint > 0 ? c->hist_slots_hint : hc;
+  c->hcap = std::max(KEY_HEAD, c->hist_slots_hint > 0 ? c->hist_slots_hint : hc);
   c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
   const int nbox = (int)c->boxes.size();
   for (int k = 0; k < 2; k++) {
@@ -563,8 +568,9 @@ static int upload_state(sdem_ctx* c) {
       if (it == idx.end() || it->first != b.cowner[k]) return c->fail(SDEM_E_ARG, "owner id not present");
       const size_t s = (size_t)b.off + it->second;
       if (hcn[s] >= c->hcap) return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
-      const size_t slot = (size_t)hcn[s]++ * cap + s;
-      hk[slot] = b.cpartner[k] | (b.ckind[k] << KEY_BITS);
+      const int r = hcn[s]++;
+      const size_t slot = (size_t)r * cap + s;
+      hk[key_slot(r, s, cap)] = b.cpartner[k] | (b.ckind[k] << KEY_BITS);
       hv[slot] = make_double4(b.ctan[3 * k], b.ctan[3 * k + 1], b.ctan[3 * k + 2], b.cvri[k]);
     }
     BoxState& B = bs[bi];
@@ -672,7 +678,7 @@ static int upload_params(sdem_ctx* c) {
     p.key_first = (int)key;
     for (int L = 0; L < p.nlvl; L++) {
       if (int rc = make_grid(c, b, b.cells[L], b.level_rmax[L], key, &p.lvl[L])) return rc;
-      key += p.lvl[L].ncell;
+      key += p.lvl[L].nkey;
     }
     p.dead_key = (int)key;
     key += 1;
@@ -1642,17 +1648,21 @@ static int collect(sdem_ctx* c, std::vector<Rec>& recs) {
     if (int rc = live_index(c, idx)) return rc;
     Buf& A = c->buf[c->cur];
     const size_t cap = (size_t)c->nslots, off = (size_t)b.off, len = (size_t)b.bcap;
-    std::vector<int> hc, hk;
+    std::vector<int> hc, hh, hk;
     std::vector<double4> hv;
+    // keys: the box's heads (KEY_HEAD per particle, side by side), then
+    // rows KEY_HEAD.. of the box's slots
     hipError_t e = dl_box(hc, A.hcnt, cap, 1, off, len);
-    if (e == hipSuccess) e = dl_box(hk, A.hkey, cap, (size_t)c->hcap, off, len);
+    if (e == hipSuccess) e = dl_box(hh, A.hkey, KEY_HEAD * cap, 1, KEY_HEAD * off, KEY_HEAD * len);
+    if (e == hipSuccess)
+      e = dl_box(hk, A.hkey + KEY_HEAD * cap, cap, (size_t)(c->hcap - KEY_HEAD), off, len);
     if (e == hipSuccess) e = dl_box(hv, A.hval, cap, (size_t)c->hcap, off, len);
     if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(history)");
     for (auto& pr : idx) {
       const size_t k = (size_t)pr.second - off;
       for (int q = 0; q < hc[k]; q++) {
-        const size_t slot = (size_t)q * len + k;
-        recs.push_back({pr.first, hk[slot] & KEY_MASK, (int)((uint32_t)hk[slot] >> KEY_BITS), hv[slot]});
+        const int key = q < KEY_HEAD ? hh[KEY_HEAD * k + q] : hk[(size_t)(q - KEY_HEAD) * len + k];
+        recs.push_back({pr.first, key & KEY_MASK, (int)((uint32_t)key >> KEY_BITS), hv[(size_t)q * len + k]});
       }
     }
   }

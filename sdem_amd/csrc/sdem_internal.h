@@ -19,6 +19,15 @@ namespace sdem {
 // partner key: id in the low 29 bits, contact kind (1..4) above it
 constexpr int KEY_BITS = 29;
 constexpr int KEY_MASK = (1 << KEY_BITS) - 1;
+
+// Where record r of particle s keeps its partner key: the first KEY_HEAD
+// records of a particle side by side (one 16-byte load gives all four, as
+// an int4 at s), the others in rows of cap, as the record values (hval at
+// r*cap + s).  The hcap >= KEY_HEAD rows of an hcap x cap array hold both.
+constexpr int KEY_HEAD = 4;
+__host__ __device__ inline size_t key_slot(int r, size_t s, size_t cap) {
+  return r < KEY_HEAD ? (size_t)KEY_HEAD * s + (size_t)r : (size_t)r * cap + s;
+}
 constexpr int MAX_WALLS = 64;     // walls seen by one context (near-wall bit mask)
 constexpr int MAX_TYPES = 64;
 
@@ -46,9 +55,17 @@ struct Grid {
   int n[3];
   int64_t ncell;
   int64_t key_base;   // first key of this level in the combined key space
+  // Keys of the moving particles' cells: bricks of BRICK^3 cells in
+  // row-major brick order, the cells of a brick row-major inside it, so
+  // that particles close in space sit close in slot order along all three
+  // axes (a wave's neighbours share its cache lines).  nb: bricks per axis,
+  // nkey = nb[0] nb[1] nb[2] BRICK^3 keys.
+  int nb[3];
+  int64_t nkey;
   double rmax;        // largest radius among this level's particles
 };
 constexpr int MAX_LEVELS = 6;
+constexpr int BRICK_BITS = 2, BRICK = 1 << BRICK_BITS;
 
 // one SoA particle buffer
 struct Buf {
@@ -61,7 +78,7 @@ struct Buf {
   int *id, *type, *mark;
   uint64_t *wmask;  // near-wall list as a bit mask over wall indices
   int *hcnt;        // history records owned by the particle
-  int *hkey;        // hcap x cap partner keys, record c of particle s at c*cap+s
+  int *hkey;        // hcap x cap partner keys, record c of particle s at key_slot(c, s, cap)
   double4 *hval;    // hcap x cap (TanDelta.x,.y,.z, VelRel_Init), same layout
   double *force;    // 6 x cap (F, T) when forces are kept
   int idx;          // which of the context's two buffers this is
@@ -191,7 +208,7 @@ struct Dev {
   int *vlist, *vkey;     // candidate lists by particle (lists longer than the LDS stage)
   int4* act;             // (particle, candidate count, smallest gap as a double) per packed slot
   int* cand;             // candidate entries by packed slot: entry q of slot t at q*cap+t
-  unsigned* clist;
+  uint2* clist;          // contacts of the step by particle: (entry, history row), entry q of k at q*cap+k
   int* ccnt;
   double* ft;            // phase entry points: F, T, then a, alpha (6 x cap)
   const double4* fpk;

@@ -258,9 +258,16 @@ __device__ __forceinline__ int cell_clamped(const Grid& G, int d, double x) {
   int c = cell_raw(G, d, x);
   return c < 0 ? 0 : (c >= G.n[d] ? G.n[d] - 1 : c);
 }
+// key of cell (cx, cy, cz) of a level: its brick, then its place in the
+// brick (x fastest), so that the cells of one brick row are consecutive keys
+__device__ __forceinline__ int64_t brick_key(const Grid& G, int cx, int cy, int cz) {
+  constexpr int M = BRICK - 1;
+  const int64_t brick = ((int64_t)(cz >> BRICK_BITS) * G.nb[1] + (cy >> BRICK_BITS)) * G.nb[0] + (cx >> BRICK_BITS);
+  return G.key_base + (brick << (3 * BRICK_BITS)) +
+         ((((cz & M) << BRICK_BITS) | (cy & M)) << BRICK_BITS | (cx & M));
+}
 __device__ __forceinline__ int64_t cell_key(const Grid& G, double x, double y, double z) {
-  int cx = cell_clamped(G, 0, x), cy = cell_clamped(G, 1, y), cz = cell_clamped(G, 2, z);
-  return G.key_base + ((int64_t)cz * G.n[1] + cy) * G.n[0] + cx;
+  return brick_key(G, cell_clamped(G, 0, x), cell_clamped(G, 1, y), cell_clamped(G, 2, z));
 }
 
 // ---------------------------------------------------------------------------
@@ -803,7 +810,6 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
         double shy = 0.0;
         if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
         else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
-        const int64_t rowkey = G.key_base + ((int64_t)wz * G.n[1] + wy) * G.n[0];
         for (int seg = 0; seg < 3; seg++) {
           int xa, xb;
           double shx;
@@ -814,18 +820,24 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
           if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
           if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
           const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
-          const int ja = cstart[rowkey + xa] - base, jb = cstart[rowkey + xb + 1] - base;
-          for (int kj = ja; kj < jb; kj++) {
-            const double4 pj = pk[kj];
-            const double ddx = pi.x - (pj.x + shx);
-            const double ddy = pi.y - (pj.y + shy);
-            const double ddz = pi.z - (pj.z + shz);
-            const double dr = ri + pj.w + P.skin;
-            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (d2 > dr * dr) continue;
-            if (kj == k && !shifted) continue;
-            gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
-            vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj, 2 * pid[kj]);
+          // the row's cells xa..xb, one run of consecutive keys per brick
+          for (int x0 = xa; x0 <= xb;) {
+            const int x1 = min(xb, x0 | (BRICK - 1));
+            const int ja = cstart[brick_key(G, x0, wy, wz)] - base;
+            const int jb = cstart[brick_key(G, x1, wy, wz) + 1] - base;
+            for (int kj = ja; kj < jb; kj++) {
+              const double4 pj = pk[kj];
+              const double ddx = pi.x - (pj.x + shx);
+              const double ddy = pi.y - (pj.y + shy);
+              const double ddz = pi.z - (pj.z + shz);
+              const double dr = ri + pj.w + P.skin;
+              const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+              if (d2 > dr * dr) continue;
+              if (kj == k && !shifted) continue;
+              gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
+              vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj, 2 * pid[kj]);
+            }
+            x0 = x1 + 1;
           }
         }
       }
@@ -932,12 +944,52 @@ __device__ __forceinline__ Hist find_hist(const Buf& A, int cap, int owner_s, in
   int c = A.hcnt[owner_s];
   for (int k = 0; k < c; k++) {
     const size_t slot = (size_t)k * cap + owner_s;
-    if ((A.hkey[slot] & KEY_MASK) == partner) {
+    if ((A.hkey[key_slot(k, owner_s, cap)] & KEY_MASK) == partner) {
       double4 v = A.hval[slot];
       h.td = D3(v.x, v.y, v.z);
       h.vri = v.w;
       break;
     }
+  }
+  return h;
+}
+
+// The row of particle b's history (values at b, b + cap, ...) that holds
+// the record against `key`, the first match as in find_hist, or NO_ROW:
+// the count and the first KEY_HEAD keys (one int4) are loaded by the
+// caller, in one batch (their addresses depend on b only), the rest one by
+// one.
+constexpr unsigned NO_ROW = 0xffffffffu;
+struct HeadKeys {
+  int n;
+  int4 key;
+};
+
+__device__ __forceinline__ HeadKeys head_keys(const Buf& A, int b) {
+  HeadKeys h;
+  h.n = A.hcnt[b];
+  h.key = reinterpret_cast<const int4*>(A.hkey)[b];
+  return h;
+}
+
+__device__ __forceinline__ unsigned hist_row(const Buf& A, size_t cap, int b, const HeadKeys& h, int key) {
+  if (h.n > 0 && (h.key.x & KEY_MASK) == key) return 0u;
+  if (h.n > 1 && (h.key.y & KEY_MASK) == key) return 1u;
+  if (h.n > 2 && (h.key.z & KEY_MASK) == key) return 2u;
+  if (h.n > 3 && (h.key.w & KEY_MASK) == key) return 3u;
+  for (int r = KEY_HEAD; r < h.n; r++)
+    if ((A.hkey[key_slot(r, b, cap)] & KEY_MASK) == key) return (unsigned)r;
+  return NO_ROW;
+}
+
+__device__ __forceinline__ Hist hist_at(const Buf& A, size_t cap, int b, unsigned row) {
+  Hist h;
+  h.td = D3(0.0, 0.0, 0.0);
+  h.vri = -2.0;
+  if (row != NO_ROW) {
+    const double4 v = A.hval[(size_t)row * cap + b];
+    h.td = D3(v.x, v.y, v.z);
+    h.vri = v.w;
   }
   return h;
 }
@@ -953,7 +1005,7 @@ __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, 
                                           int partner, const CF& cf) {
   if (acc.hc < P.hcap) {
     const size_t slot = (size_t)acc.hc * P.cap + k;
-    B.hkey[slot] = partner | (kind << KEY_BITS);
+    B.hkey[key_slot(acc.hc, k, P.cap)] = partner | (kind << KEY_BITS);
     B.hval[slot] = make_double4(cf.tandel.x, cf.tandel.y, cf.tandel.z, cf.vri);
     acc.hc++;
   } else {
@@ -980,8 +1032,13 @@ __device__ __forceinline__ double image(double xi, double xj, double L, int peri
 // early-outs, NBSM_FineSearch1/2/3, Prtcl_NBS_Munjiza.f90:478-557) on the
 // image coordinates, so the contact list comes out sorted as well.  Entries:
 // the partner's index in buffer A (moving) or in the fixed arrays, with the
-// fixed flag, and a flag for a periodic image.  A light kernel (few
-// registers), so that many waves hide the gather latency.
+// fixed flag, and a flag for a periodic image, each with the row of last
+// step's history that holds the pair's record (the lower id's bucket for
+// two moving particles side by side, the particle's own otherwise: the
+// buckets the force routine reads, ContactForce_PP :732-756, _PPG
+// :761-798, _PPFix :803-827), so that the force kernel loads the record
+// directly.  A light kernel (few registers), so that many waves hide the
+// gather latency, the history search's chain of loads included.
 //
 // FAST (single box, a step queued without the sort and list kernels; an
 // instantiation of its own, so that the other one compiles as before):
@@ -1004,9 +1061,10 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const int* __restrict__ perm,
                                                  const int4* __restrict__ act,
                                                  const int* __restrict__ cand,
-                                                 unsigned* __restrict__ clist,
+                                                 uint2* __restrict__ clist,
                                                  int* __restrict__ ccnt,
                                                  const double4* __restrict__ fpk,
+                                                 const int* __restrict__ fid,
                                                  const BoxSlot* rd, BoxSlot* wr) {
   const int tile = blockIdx.x;
   const int box = block_box(blk_box);
@@ -1071,24 +1129,12 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       ent[q] = fixed ? e : (unsigned)j;
     }
   const double4 pi = A.pos[s];
+  const int idi = A.id[s];
+  const HeadKeys own = head_keys(A, s);
   int c = 0;
-  for (int q = 0; q < nq; q++) {
-    unsigned e;
-    double4 pj;
-    bool fixed;
-    int j;
-    if (q < CPF) {
-      e = ent[q];
-      fixed = (e & CL_FIXED) != 0;
-      j = (int)(e & CL_INDEX);
-      pj = pjs[q];
-    } else {
-      e = (unsigned)cand[(size_t)q * cap + t];
-      fixed = (e & CL_FIXED) != 0;
-      j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
-      pj = fixed ? fpk[j] : A.pos[j];
-      if (fixed) j = (int)(e & CL_INDEX);
-    }
+  // one candidate: the reference's predicate, then the entry and its
+  // history row
+  auto consider = [&](unsigned e, double4 pj, bool fixed, int j) {
     int shifted = 0;
     const double xj = image(pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
     const double yj = image(pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
@@ -1097,15 +1143,45 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     const double dr2 = dr * dr;
     const double ddx = pi.x - xj;
     double d2 = ddx * ddx;
-    if (d2 > dr2) continue;
+    if (d2 > dr2) return;
     const double ddy = pi.y - yj;
     d2 = ddy * ddy + d2;
-    if (d2 > dr2) continue;
+    if (d2 > dr2) return;
     const double ddz = pi.z - zj;
     d2 = ddz * ddz + d2;
-    if (d2 > dr2) continue;
-    if (c < P.hcap) clist[(size_t)c * cap + k] = fixed ? e : ((unsigned)j | (shifted ? CL_SHIFT : 0u));
+    if (d2 > dr2) return;
+    if (c < P.hcap) {
+      // a moving partner's id, count and first keys in one batch; the
+      // record sits in the partner's bucket if its id is the lower
+      int idj;
+      unsigned row;
+      if (fixed) {
+        idj = fid[j];
+        row = hist_row(A, cap, s, own, idj);
+      } else {
+        idj = A.id[j];
+        const HeadKeys pk = head_keys(A, j);
+        const unsigned r_own = hist_row(A, cap, s, own, idj);
+        const unsigned r_other = hist_row(A, cap, j, pk, idi);
+        row = (!shifted && idj < idi) ? r_other : r_own;
+      }
+      clist[(size_t)c * cap + k] = make_uint2(fixed ? e : ((unsigned)j | (shifted ? CL_SHIFT : 0u)), row);
+    }
     c++;
+  };
+  // the prefetched candidates (unrolled: the staged values stay in
+  // registers), then the rest of the list
+#pragma unroll
+  for (int q = 0; q < CPF; q++)
+    if (q < nq) {
+      const bool fixed = (ent[q] & CL_FIXED) != 0;
+      consider(ent[q], pjs[q], fixed, (int)(ent[q] & CL_INDEX));
+    }
+  for (int q = CPF; q < nq; q++) {
+    const unsigned e = (unsigned)cand[(size_t)q * cap + t];
+    const bool fixed = (e & CL_FIXED) != 0;
+    const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+    consider(e, fixed ? fpk[j] : A.pos[j], fixed, j);
   }
   ccnt[k] = c;
   // More contacts than history slots: the force kernel of the step turns
@@ -1155,7 +1231,7 @@ __device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
 // that order: the phase entry points run them in two kernels, which continue
 // the same sums.
 __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, const Buf& B, int k, int s,
-                                             const Own& o, const unsigned* __restrict__ clist,
+                                             const Own& o, const uint2* __restrict__ clist,
                                              const int* __restrict__ ccnt, const double4* __restrict__ fpk,
                                              const int* __restrict__ fid, const int* __restrict__ ftype,
                                              Acc& acc, int parts = 3) {
@@ -1174,7 +1250,8 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
     int kind, partner;
     bool flip = false;
     if (q < nc) {
-      const unsigned e = clist[(size_t)q * cap + k];
+      const uint2 ce = clist[(size_t)q * cap + k];
+      const unsigned e = ce.x;
       const int j = (int)(e & CL_INDEX);
       double4 pj;
       d3 vj, wj;
@@ -1207,14 +1284,14 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
       if (e & CL_FIXED) {
         // PPFix: the moving particle's record, its own perspective
         kind = 3;
-        h = find_hist(A, cap, s, idj);
+        h = hist_at(A, cap, s, ce.y);
         Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
       } else if (!shifted) {
         // PP: the record sits in the lower id's bucket and the force is
         // taken from that side (FineSearch2 :525-529)
         const bool ilo = idi < idj;
         kind = ilo ? 1 : 0;
-        h = ilo ? find_hist(A, cap, s, idj) : find_hist(A, cap, j, idi);
+        h = hist_at(A, cap, ilo ? s : j, ce.y);
         const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
         Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
         flip = !ilo;
@@ -1222,7 +1299,7 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
         // PPG: each side keeps its own record against the other's image,
         // the force from the lower id's side; row type = the local particle
         kind = 2;
-        h = find_hist(A, cap, s, idj);
+        h = hist_at(A, cap, s, ce.y);
         Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
         flip = idi > idj;
       }
@@ -1269,8 +1346,8 @@ __device__ __forceinline__ void accel(const Params& P, int ti, d3 F, d3 T, d3& a
 // The integration inputs beyond the contact loop's (history levels, theta,
 // mark).  PIM / PRIM: translational / rotational integration method fixed
 // at compile time (0: read from Params); with fixed methods they are loaded
-// by `preload` ahead of their use, so that their latency overlaps other
-// work, otherwise where they are used.
+// by `preload` in one batch ahead of their use, otherwise where they are
+// used.
 template <int PIM, int PRIM>
 struct Integ {
   d3 v2, v3, a2, a3, th, w2, w3, al2, al3;
@@ -1534,15 +1611,17 @@ __device__ __forceinline__ Acc acc_zero() {
 // The fused force step: contact loop, accelerations and integration in one
 // kernel, reading and writing each particle's stream once (the choice for
 // beds with few contacts, where the step is a streaming pass).  With fixed
-// integration methods (PIM, PRIM) the integration inputs are loaded before
-// the contact loop, so that their latency overlaps it; the registers that
-// holds are capped at two waves per SIMD (256 VGPRs; a few spill).
+// integration methods (PIM, PRIM) the integration inputs are loaded in one
+// batch after the contact loop: loaded before it, they stay live across
+// the loop and spill at the cap of two waves per SIMD (256 VGPRs), and the
+// spill traffic of every particle costs a streaming step more than the
+// latency they hide.
 template <int PIM, int PRIM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_force(const Params* __restrict__ PB,
                                                const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
                                                BoxState* BS,
                                                const int* __restrict__ perm,
-                                               const unsigned* __restrict__ clist,
+                                               const uint2* __restrict__ clist,
                                                const int* __restrict__ ccnt,
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid,
@@ -1562,8 +1641,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int s = f.rebuilt ? perm[k] : k;
     const Own o = load_own(P, A, s);
     Integ<PIM, PRIM> in;
-    in.preload(P, S, A, s, o.vi, f.put_ids);
     contact_loop(P, A, B, k, s, o, clist, ccnt, fpk, fid, ftype, acc);
+    in.preload(P, S, A, s, o.vi, f.put_ids);
     dead = in.integrate(P, S, A, B, k, s, o, acc.F, acc.T, f.put_ids, f.put_wall, f.upd_next, move2, vnext, anext);
     B.hcnt[k] = dead ? 0 : acc.hc;
   }
@@ -1589,7 +1668,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // records
 __global__ __launch_bounds__(256) void k_ph_pp(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                                Buf A, Buf B, Scalars* S, BoxState* BS, const int* __restrict__ perm,
-                                               const unsigned* __restrict__ clist, const int* __restrict__ ccnt,
+                                               const uint2* __restrict__ clist, const int* __restrict__ ccnt,
                                                const double4* __restrict__ fpk, const int* __restrict__ fid,
                                                const int* __restrict__ ftype, double* __restrict__ ft,
                                                BoxSlot* __restrict__ bslots) {
@@ -1614,7 +1693,7 @@ __global__ __launch_bounds__(256) void k_ph_pp(const Params* __restrict__ PB, co
 // FindContactsPW (:230): the walls, continuing the sums and the records
 __global__ __launch_bounds__(256) void k_ph_pw(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                                Buf A, Buf B, Scalars* S, BoxState* BS, const int* __restrict__ perm,
-                                               const unsigned* __restrict__ clist, const int* __restrict__ ccnt,
+                                               const uint2* __restrict__ clist, const int* __restrict__ ccnt,
                                                const double4* __restrict__ fpk, const int* __restrict__ fid,
                                                const int* __restrict__ ftype, double* __restrict__ ft,
                                                BoxSlot* __restrict__ bslots) {
@@ -1794,7 +1873,7 @@ hipError_t launch_phase(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
   switch (phase) {
     case 1:
       hipLaunchKernelGGL(k_contact<0>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                         D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                         D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk, D.fid,
                          slot_set(D, A.idx), slot_set(D, B.idx));
       hipLaunchKernelGGL(k_ph_pp, dim3(nb), dim3(256), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, D.perm, D.clist,
                          D.ccnt, D.fpk, D.fid, D.ftype, D.ft, slot_set(D, B.idx));
@@ -1827,11 +1906,11 @@ hipError_t launch_phase(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
 hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st, hipEvent_t mid, bool fast) {
   if (fast && D.nbox == 1)
     hipLaunchKernelGGL(k_contact<1>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk, D.fid,
                        slot_set(D, A.idx), slot_set(D, B.idx));
   else
     hipLaunchKernelGGL(k_contact<0>, dim3(nblk(D.nslots, D.bs_contact)), dim3(D.bs_contact), 0, st, D.PB,
-                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk,
+                       D.blk_box, A, D.S, D.BS, D.perm, D.act, D.cand, D.clist, D.ccnt, D.fpk, D.fid,
                        slot_set(D, A.idx), slot_set(D, B.idx));
   if (mid) (void)hipEventRecord(mid, st);
   const unsigned nf = nblk(D.nslots, D.bs_force);
