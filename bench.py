@@ -43,11 +43,14 @@ BYTES_PER_CONTACT = 80
 # integrated, but only sdem_get_forces reads them)
 CELL_FACTOR = (0.5, 0.5, 0.5)
 SKIN = 0.05
-# HBM bytes per k_force launch on the default 1M bed, measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by what the same
-# counters read for a plain 512 MiB copy (x2.00 and x1.00):
-# 96,044 KiB x 2 read + 176,272 KiB written (profiles/k_force_settling_1M.md)
-KFORCE_TRAFFIC_1M = (96_044 * 2 + 176_272) * 1024
+# HBM bytes per k_force launch on the default 1M bed and on that bed
+# settled into contact, measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE and scaled by what the same counters read for a plain 512 MiB
+# copy (x2.00 and x1.00): 84,463 KiB x 2 read + 176,270 KiB written
+# (profiles/k_force_settling_1M.md); 736,022 KiB x 2 + 324,913 KiB
+# (profiles/k_force_contact_bed_1M.md)
+KFORCE_TRAFFIC_1M = (84_463 * 2 + 176_270) * 1024
+KFORCE_TRAFFIC_CONTACT_1M = (736_022 * 2 + 324_913) * 1024
 # steps per library call in a timed region (a few seconds of work at most,
 # so that a long run still prints progress)
 CHUNK = 20_000
@@ -81,6 +84,8 @@ def parse():
     ap.add_argument("--boxes", type=int, default=256)
     ap.add_argument("--ensemble-settle", type=int, default=0,
                     help="ensemble workload: settle the boxes this many steps first (dt 1e-6 s)")
+    ap.add_argument("--ensemble-state", default=None,
+                    help=".npz of the settled boxes: read if it exists, else written after the settle")
     return ap.parse_args()
 
 
@@ -262,6 +267,9 @@ def contact_bed_bench(args, log):
     wall, P, by_kind = timed_region(eng, steps, log)
     ms_step = wall / steps * 1e3
     roof, breakdown = kernel_roofline(eng, n, P, min(steps, 200), ms_step)
+    if n == 1_000_000 and args.seed == 20261203:
+        roof["traffic"] = KFORCE_TRAFFIC_CONTACT_1M
+        roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this bed (constant of that run)"
     out = {"value": n * steps / wall, "unit": "particle-steps/s", "steps": steps, "warmup": args.warmup,
            "ms_per_step": ms_step, "contact_records_per_step": P,
            "contact_records_by_kind": {"PP": int(by_kind[0]), "PPG": int(by_kind[1]), "PPFix": int(by_kind[2]),
@@ -301,13 +309,25 @@ def ensemble_bench(args):
           file=sys.stderr, flush=True)
     settle_s = None
     if args.ensemble_settle:
-        t0 = time.perf_counter()
-        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-        scs = scenarios.settle(scs, args.ensemble_settle, dt=1.0e-6, gravity_scale=2.0,
-                               chunk=max(1, args.ensemble_settle // 10), log=log)
+        path = args.ensemble_state
+        if path and os.path.exists(path):
+            # the settled states and contact records of every box, by box
+            z = np.load(path)
+            for b, sc in enumerate(scs):
+                sc["state"] = {k.split("_", 2)[2]: z[k] for k in z.files if k.startswith(f"st_{b}_")}
+                sc["contacts"] = {k.split("_", 2)[2]: z[k] for k in z.files if k.startswith(f"ct_{b}_")}
+            settle_s = 0.0
+        else:
+            t0 = time.perf_counter()
+            log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+            scs = scenarios.settle(scs, args.ensemble_settle, dt=1.0e-6, gravity_scale=2.0,
+                                   chunk=max(1, args.ensemble_settle // 10), log=log)
+            settle_s = time.perf_counter() - t0
+            if path:
+                np.savez(path, **{f"st_{b}_{k}": v for b, sc in enumerate(scs) for k, v in sc["state"].items()},
+                         **{f"ct_{b}_{k}": v for b, sc in enumerate(scs) for k, v in sc["contacts"].items()})
         for sc in scs:
             sc["options"]["dt"] = 1.0e-6
-        settle_s = time.perf_counter() - t0
     lib = native.load_hip_library()
     eng = native.build_ensemble(lib, scs, tuning=dict(keep_forces=0))
     # warm up, then let the autotuner pick the shapes on the moving packs
