@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sdem.h"
@@ -410,6 +411,20 @@ int make_grid(sdem_ctx* c, const Box& b, double cell_len0, double rmax, int64_t 
   return SDEM_OK;
 }
 
+// The most history records one particle of the box holds when its contact
+// records are installed: its own, and a copy of each PP record of a lower id
+// partner (KIND_COPY).
+static int installed_records(const Box& b) {
+  std::unordered_map<int, int> n;
+  for (size_t k = 0; k < b.ckind.size(); k++) {
+    n[b.cowner[k]]++;
+    if (b.ckind[k] == SDEM_KIND_PP) n[b.cpartner[k]]++;
+  }
+  int m = 0;
+  for (auto& kv : n) m = std::max(m, kv.second);
+  return m;
+}
+
 // history slots per particle: a kissing-number bound for the largest type
 // touched by the smallest, with room for soft overlaps and walls, at most 48
 // to start with (an overflow doubles it and replays the step)
@@ -495,18 +510,19 @@ static int upload_state(sdem_ctx* c) {
   free_list(c, c->allocs_state);
   // placement: every box owns a BOX_ALIGN-multiple of slots
   int64_t off = 0;
-  int hc = 0;
+  int hc = 0, hc_inst = 0;
   for (auto& b : c->boxes) {
     if (!b.have_state) return c->fail(SDEM_E_STATE, "every box needs a state before stepping");
     b.off = (int)off;
     b.bcap = round_up(std::max(b.n, 1), BOX_ALIGN);
     off += b.bcap;
     hc = std::max(hc, history_capacity(b));
+    hc_inst = std::max(hc_inst, installed_records(b));
   }
   if (off > INT32_MAX / 2) return c->fail(SDEM_E_CAPACITY, "too many particles for one context");
   c->nslots = off;
   const size_t cap = (size_t)off;
-  c->hcap = std::max(KEY_HEAD, c->hist_slots_hint > 0 ? c->hist_slots_hint : hc);
+  c->hcap = std::max({KEY_HEAD, c->hist_slots_hint > 0 ? c->hist_slots_hint : hc, hc_inst});
   c->lcap = c->list_slots_hint > 0 ? c->list_slots_hint : std::max(48, 2 * c->hcap);
   const int nbox = (int)c->boxes.size();
   for (int k = 0; k < 2; k++) {
@@ -563,15 +579,27 @@ static int upload_state(sdem_ctx* c) {
     idx.reserve(b.n);
     for (int i = 0; i < b.n; i++) idx.push_back({b.id[i], i});
     std::sort(idx.begin(), idx.end());
+    auto put = [&](size_t s, int partner, int kind, size_t k) -> bool {
+      if (hcn[s] >= c->hcap) return false;
+      const int r = hcn[s]++;
+      hk[key_slot(r, s, cap)] = partner | (kind << KEY_BITS);
+      hv[(size_t)r * cap + s] = make_double4(b.ctan[3 * k], b.ctan[3 * k + 1], b.ctan[3 * k + 2], b.cvri[k]);
+      return true;
+    };
     for (size_t k = 0; k < b.ckind.size(); k++) {
       auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(b.cowner[k], -1));
       if (it == idx.end() || it->first != b.cowner[k]) return c->fail(SDEM_E_ARG, "owner id not present");
-      const size_t s = (size_t)b.off + it->second;
-      if (hcn[s] >= c->hcap) return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
-      const int r = hcn[s]++;
-      const size_t slot = (size_t)r * cap + s;
-      hk[key_slot(r, s, cap)] = b.cpartner[k] | (b.ckind[k] << KEY_BITS);
-      hv[slot] = make_double4(b.ctan[3 * k], b.ctan[3 * k + 1], b.ctan[3 * k + 2], b.cvri[k]);
+      if (!put((size_t)b.off + it->second, b.cpartner[k], b.ckind[k], k))
+        return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
+    }
+    // the higher id's copy of each PP record (KIND_COPY), after the
+    // records proper
+    for (size_t k = 0; k < b.ckind.size(); k++) {
+      if (b.ckind[k] != SDEM_KIND_PP) continue;
+      auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(b.cpartner[k], -1));
+      if (it == idx.end() || it->first != b.cpartner[k]) continue;
+      if (!put((size_t)b.off + it->second, b.cowner[k], KIND_COPY, k))
+        return c->fail(SDEM_E_CAPACITY, "too many history records for one particle");
     }
     BoxState& B = bs[bi];
     std::memset(&B, 0, sizeof(B));
@@ -1662,6 +1690,7 @@ static int collect(sdem_ctx* c, std::vector<Rec>& recs) {
       const size_t k = (size_t)pr.second - off;
       for (int q = 0; q < hc[k]; q++) {
         const int key = q < KEY_HEAD ? hh[KEY_HEAD * k + q] : hk[(size_t)(q - KEY_HEAD) * len + k];
+        if ((int)((uint32_t)key >> KEY_BITS) == KIND_COPY) continue;
         recs.push_back({pr.first, key & KEY_MASK, (int)((uint32_t)key >> KEY_BITS), hv[(size_t)q * len + k]});
       }
     }

@@ -19,6 +19,14 @@ namespace sdem {
 // partner key: id in the low 29 bits, contact kind (1..4) above it
 constexpr int KEY_BITS = 29;
 constexpr int KEY_MASK = (1 << KEY_BITS) - 1;
+// A PP record lives in the lower id's list (as the reference's buckets);
+// the higher id's list holds a copy of it under this kind, written by the
+// same step with the same bits, so that each side of a pair reads the
+// pair's history from its own rows (which line up with its contact list
+// from step to step).  Copies are not contact records: the readers leave
+// them out, and the reference's lookups in a particle's own bucket (PPG,
+// PPFix, PW) pass over them.
+constexpr int KIND_COPY = 5;
 
 // Where record r of particle s keeps its partner key: the first KEY_HEAD
 // records of a particle side by side (one 16-byte load gives all four, as

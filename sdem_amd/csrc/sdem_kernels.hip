@@ -160,13 +160,20 @@ __device__ __forceinline__ CF contact_force(const Params& P, const Bnry& B, d3 n
 
   d3 w_hat = sub3(rvei, rvej);
   double wm = norm3(w_hat);
-  if (wm > 1.0e-10) w_hat = div3(w_hat, wm);
-  else w_hat = D3(0.0, 0.0, 0.0);
+  const double mr = P.ct_model == 1 ? -(B.fric_roll * fabs(fn) * B.rad_eff)
+                                    : -(B.fric_roll * fabs(fn) * B.rad_eff * norm3(vel_w));
   d3 mrij;
-  if (P.ct_model == 1)
-    mrij = mul3(w_hat, -(B.fric_roll * fabs(fn) * B.rad_eff));
-  else
-    mrij = mul3(w_hat, -(B.fric_roll * fabs(fn) * B.rad_eff * norm3(vel_w)));
+  if (mr == 0.0 && wm <= 1.0e308) {
+    // no rolling resistance (a zero coefficient): the products are zeros,
+    // signed as w_hat times mr, and w_hat = w / wm has the signs of w
+    // (+0 below 1e-10), so the three divisions are not needed
+    if (wm > 1.0e-10) mrij = D3(copysign(0.0, w_hat.x) * mr, copysign(0.0, w_hat.y) * mr, copysign(0.0, w_hat.z) * mr);
+    else mrij = D3(0.0 * mr, 0.0 * mr, 0.0 * mr);
+  } else {
+    if (wm > 1.0e-10) w_hat = div3(w_hat, wm);
+    else w_hat = D3(0.0, 0.0, 0.0);
+    mrij = mul3(w_hat, mr);
+  }
 
   CF o;
   o.tandel = ot;
@@ -937,25 +944,8 @@ struct Hist {
   double vri;
 };
 
-__device__ __forceinline__ Hist find_hist(const Buf& A, int cap, int owner_s, int partner) {
-  Hist h;
-  h.td = D3(0.0, 0.0, 0.0);
-  h.vri = -2.0;
-  int c = A.hcnt[owner_s];
-  for (int k = 0; k < c; k++) {
-    const size_t slot = (size_t)k * cap + owner_s;
-    if ((A.hkey[key_slot(k, owner_s, cap)] & KEY_MASK) == partner) {
-      double4 v = A.hval[slot];
-      h.td = D3(v.x, v.y, v.z);
-      h.vri = v.w;
-      break;
-    }
-  }
-  return h;
-}
-
 // The row of particle b's history (values at b, b + cap, ...) that holds
-// the record against `key`, the first match as in find_hist, or NO_ROW:
+// the record against `key`, the first match in record order, or NO_ROW:
 // the count and the first KEY_HEAD keys (one int4) are loaded by the
 // caller, in one batch (their addresses depend on b only), the rest one by
 // one.
@@ -982,6 +972,36 @@ __device__ __forceinline__ unsigned hist_row(const Buf& A, size_t cap, int b, co
   return NO_ROW;
 }
 
+// Lookups in the particle's own list as the reference's bucket holds it
+// (PP copies, KIND_COPY, aside).  A particle's records are written in the
+// order of its contacts, so the record of its contact number `hint` is
+// usually in row `hint`: that row is tried first, then the whole list.
+__device__ __forceinline__ bool key_is(int k, int key) { return (k & KEY_MASK) == key; }
+__device__ __forceinline__ bool is_copy(int k) { return ((unsigned)k >> KEY_BITS) == (unsigned)KIND_COPY; }
+__device__ __forceinline__ int key_at(const Buf& A, size_t cap, int b, const HeadKeys& h, int r) {
+  if (r < KEY_HEAD) return r == 0 ? h.key.x : (r == 1 ? h.key.y : (r == 2 ? h.key.z : h.key.w));
+  return A.hkey[key_slot(r, b, cap)];
+}
+// The first record against `key` that is not a copy, or NO_ROW (with
+// `copy`: the first record against `key`, if it is a copy, else NO_ROW).
+template <bool COPY>
+__device__ __forceinline__ unsigned own_row(const Buf& A, size_t cap, int b, const HeadKeys& h, int key, int hint) {
+  if (hint < h.n) {
+    const int k = key_at(A, cap, b, h, hint);
+    if (key_is(k, key) && is_copy(k) == COPY) return (unsigned)hint;
+  }
+#pragma unroll
+  for (int r = 0; r < KEY_HEAD; r++) {
+    const int k = key_at(A, cap, b, h, r);
+    if (r < h.n && key_is(k, key) && (COPY || !is_copy(k))) return is_copy(k) == COPY ? (unsigned)r : NO_ROW;
+  }
+  for (int r = KEY_HEAD; r < h.n; r++) {
+    const int k = A.hkey[key_slot(r, b, cap)];
+    if (key_is(k, key) && (COPY || !is_copy(k))) return is_copy(k) == COPY ? (unsigned)r : NO_ROW;
+  }
+  return NO_ROW;
+}
+
 __device__ __forceinline__ Hist hist_at(const Buf& A, size_t cap, int b, unsigned row) {
   Hist h;
   h.td = D3(0.0, 0.0, 0.0);
@@ -992,6 +1012,16 @@ __device__ __forceinline__ Hist hist_at(const Buf& A, size_t cap, int b, unsigne
     h.vri = v.w;
   }
   return h;
+}
+
+// A contact-list row names its bucket as well: ROW_PARTNER set = the record
+// sits in the partner's bucket (a PP pair whose partner has the lower id),
+// so the force kernel loads it together with the partner's state instead
+// of after the partner's id.  NO_ROW (all bits) stays NO_ROW.
+constexpr unsigned ROW_PARTNER = 0x80000000u;
+__device__ __forceinline__ Hist hist_of_entry(const Buf& A, size_t cap, int s, int j, unsigned row) {
+  if (row == NO_ROW) return hist_at(A, cap, s, NO_ROW);
+  return hist_at(A, cap, (row & ROW_PARTNER) ? j : s, row & ~ROW_PARTNER);
 }
 
 struct Acc {
@@ -1011,7 +1041,7 @@ __device__ __forceinline__ void push_hist(const Params& P, const Buf& B, int k, 
   } else {
     acc.overflow = 1;
   }
-  acc.nkind += 1ull << (16 * (kind - 1));
+  if (kind != KIND_COPY) acc.nkind += 1ull << (16 * (kind - 1));
 }
 
 // image of a partner coordinate: the reference's ghost copies hold
@@ -1128,9 +1158,23 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       pjs[q] = fixed ? fpk[j] : A.pos[j];
       ent[q] = fixed ? e : (unsigned)j;
     }
+  // the rest of the list runs as a pipeline: while candidate q is tested,
+  // the position of q + 1 and the entry of q + 2 are in flight
+  auto resolve = [&](unsigned e) -> int {
+    return (e & CL_FIXED) ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+  };
+  unsigned e1 = 0u, e2 = 0u;
+  if (CPF < nq) e1 = (unsigned)cand[(size_t)CPF * cap + t];
+  if (CPF + 1 < nq) e2 = (unsigned)cand[(size_t)(CPF + 1) * cap + t];
   const double4 pi = A.pos[s];
   const int idi = A.id[s];
   const HeadKeys own = head_keys(A, s);
+  int j1 = 0;
+  double4 p1 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (CPF < nq) {
+    j1 = resolve(e1);
+    p1 = (e1 & CL_FIXED) ? fpk[j1] : A.pos[j1];
+  }
   int c = 0;
   // one candidate: the reference's predicate, then the entry and its
   // history row
@@ -1157,13 +1201,22 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       unsigned row;
       if (fixed) {
         idj = fid[j];
-        row = hist_row(A, cap, s, own, idj);
+        row = own_row<false>(A, cap, s, own, idj, c);
       } else {
         idj = A.id[j];
-        const HeadKeys pk = head_keys(A, j);
-        const unsigned r_own = hist_row(A, cap, s, own, idj);
-        const unsigned r_other = hist_row(A, cap, j, pk, idi);
-        row = (!shifted && idj < idi) ? r_other : r_own;
+        if (shifted || idi < idj) {
+          row = own_row<false>(A, cap, s, own, idj, c);
+        } else {
+          // PP from the higher id: the lower id's record, as the particle's
+          // own copy; failing that, in the lower id's list (a pair that was
+          // periodic, or a copy the host did not install)
+          row = own_row<true>(A, cap, s, own, idj, c);
+          if (row == NO_ROW) {
+            const HeadKeys pk = head_keys(A, j);
+            const unsigned r_other = hist_row(A, cap, j, pk, idi);
+            row = r_other == NO_ROW ? NO_ROW : (r_other | ROW_PARTNER);
+          }
+        }
       }
       clist[(size_t)c * cap + k] = make_uint2(fixed ? e : ((unsigned)j | (shifted ? CL_SHIFT : 0u)), row);
     }
@@ -1178,10 +1231,16 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       consider(ent[q], pjs[q], fixed, (int)(ent[q] & CL_INDEX));
     }
   for (int q = CPF; q < nq; q++) {
-    const unsigned e = (unsigned)cand[(size_t)q * cap + t];
-    const bool fixed = (e & CL_FIXED) != 0;
-    const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
-    consider(e, fixed ? fpk[j] : A.pos[j], fixed, j);
+    const unsigned e = e1;
+    const int j = j1;
+    const double4 pj = p1;
+    if (q + 1 < nq) {
+      e1 = e2;
+      j1 = resolve(e1);
+      p1 = (e1 & CL_FIXED) ? fpk[j1] : A.pos[j1];
+      if (q + 2 < nq) e2 = (unsigned)cand[(size_t)(q + 2) * cap + t];
+    }
+    consider(e, pj, (e & CL_FIXED) != 0, j);
   }
   ccnt[k] = c;
   // More contacts than history slots: the force kernel of the step turns
@@ -1219,14 +1278,16 @@ __device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
   return o;
 }
 
-// Every contact of the particle in slot s of A (new slot k), one call site
-// of the force routine: the contacts k_contact found, in ascending partner
-// id, then the walls of the near-wall list, newest first (FindContactsPW
-// :298-319 walks the list from the highest index).  Each contact is reduced
-// to the arguments of the force routine from the perspective the reference
-// evaluates it from (ContactForce_PP :732-756, _PPG :761-798, _PPFix
-// :803-827, _PW :832-856), so the routine is inlined once.  The records the
-// particle owns go to slot k of B.
+// Every contact of the particle in slot s of A (new slot k): the contacts
+// k_contact found, in ascending partner id, then the walls of the near-wall
+// list, newest first (FindContactsPW :298-319 walks the list from the
+// highest index).  Each contact is reduced to the arguments of the force
+// routine from the perspective the reference evaluates it from
+// (ContactForce_PP :732-756, _PPG :761-798, _PPFix :803-827, _PW :832-856).
+// The walls run in a loop of their own over the context's walls, one wall
+// at a time for the whole wave (its data in scalar registers), each lane
+// taking the walls of its own list in the same order.  The records go to slot k of
+// B in the same order (a PP pair's in both lists: KIND_COPY).
 // `parts` selects the particle contacts (1), the walls (2) or both (3), in
 // that order: the phase entry points run them in two kernels, which continue
 // the same sums.
@@ -1239,91 +1300,8 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
   const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z);
   const double ri = o.pi.w;
   const int idi = o.idi, ti = o.ti;
-  const int nc = min(ccnt[k], P.hcap);  // more: cnt_over is set
-  uint64_t mask = o.mask0;
-  const int ntot = (parts & 2) ? nc + __popcll(mask) : nc;
-  for (int q = (parts & 1) ? 0 : nc; q < ntot; q++) {
-    d3 nvec, va, vb, wa, wb;
-    double ra, rb, ovrlp;
-    Hist h;
-    const Bnry* Bp;
-    int kind, partner;
-    bool flip = false;
-    if (q < nc) {
-      const uint2 ce = clist[(size_t)q * cap + k];
-      const unsigned e = ce.x;
-      const int j = (int)(e & CL_INDEX);
-      double4 pj;
-      d3 vj, wj;
-      int idj, tj;
-      if (e & CL_FIXED) {
-        pj = fpk[j];
-        vj = D3(0.0, 0.0, 0.0);
-        wj = vj;
-        idj = fid[j];
-        tj = ftype[j];
-      } else {
-        pj = A.pos[j];
-        vj = ld3(A.lv, cap, j);
-        wj = ld3(A.rv, cap, j);
-        idj = A.id[j];
-        tj = A.type[j];
-      }
-      int shifted = 0;
-      const double xj = image(o.pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
-      const double yj = image(o.pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
-      const double zj = image(o.pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
-      const double dr = ri + pj.w;
-      const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
-      double d2 = ddx * ddx;
-      d2 = ddy * ddy + d2;
-      d2 = ddz * ddz + d2;
-      ovrlp = dr - sqrt(d2);
-      const d3 xjv = D3(xj, yj, zj);
-      partner = idj;
-      if (e & CL_FIXED) {
-        // PPFix: the moving particle's record, its own perspective
-        kind = 3;
-        h = hist_at(A, cap, s, ce.y);
-        Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
-      } else if (!shifted) {
-        // PP: the record sits in the lower id's bucket and the force is
-        // taken from that side (FineSearch2 :525-529)
-        const bool ilo = idi < idj;
-        kind = ilo ? 1 : 0;
-        h = hist_at(A, cap, ilo ? s : j, ce.y);
-        const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
-        Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
-        flip = !ilo;
-      } else {
-        // PPG: each side keeps its own record against the other's image,
-        // the force from the lower id's side; row type = the local particle
-        kind = 2;
-        h = hist_at(A, cap, s, ce.y);
-        Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
-        flip = idi > idj;
-      }
-      if (!flip) {
-        nvec = nv3(xjv, xi);
-        ra = ri; rb = pj.w; va = o.vi; vb = vj; wa = o.wi; wb = wj;
-      } else {
-        nvec = nv3(xi, xjv);
-        ra = pj.w; rb = ri; va = vj; vb = o.vi; wa = wj; wb = o.wi;
-      }
-    } else {
-      const int w = 63 - __clzll((long long)mask);
-      mask &= ~(1ull << w);
-      const Wall& W = P.walls[w];
-      d3 nv;
-      if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
-      kind = 4;
-      partner = W.wall_id;
-      h = find_hist(A, cap, s, W.wall_id);
-      Bp = &P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
-      nvec = mul3(nv, -1.0);  // Norm_v = -nv
-      ra = ri; rb = 1.00e20 * ri; va = o.vi; vb = wp(W.trans_vel); wa = o.wi; wb = D3(0.0, 0.0, 0.0);
-    }
-    const CF cf = contact_force(P, *Bp, nvec, ra, rb, va, vb, wa, wb, ovrlp, h.td, h.vri);
+  const int nc = (parts & 1) ? min(ccnt[k], P.hcap) : 0;  // more: cnt_over is set
+  auto apply = [&](const CF& cf, bool flip, int kind, int partner) {
     if (!flip) {
       acc.F = add3(acc.F, cf.fsum);
       acc.T = add3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
@@ -1331,7 +1309,93 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
       acc.F = sub3(acc.F, cf.fsum);
       acc.T = sub3(add3(acc.T, mul3(cf.moment, ri)), cf.mrij);
     }
-    if (kind) push_hist(P, B, k, acc, kind, partner, cf);
+    push_hist(P, B, k, acc, kind, partner, cf);
+  };
+  // each entry is loaded one contact ahead of its use, and names the bucket
+  // of its record, so that a contact costs one round of gathers (partner
+  // state and record together)
+  uint2 ce_next = make_uint2(0u, NO_ROW);
+  if (nc > 0) ce_next = clist[k];
+  for (int q = 0; q < nc; q++) {
+    const uint2 ce = ce_next;
+    if (q + 1 < nc) ce_next = clist[(size_t)(q + 1) * cap + k];
+    const unsigned e = ce.x;
+    const int j = (int)(e & CL_INDEX);
+    const Hist h = hist_of_entry(A, cap, s, j, ce.y);
+    double4 pj;
+    d3 vj, wj;
+    int idj, tj;
+    if (e & CL_FIXED) {
+      pj = fpk[j];
+      vj = D3(0.0, 0.0, 0.0);
+      wj = vj;
+      idj = fid[j];
+      tj = ftype[j];
+    } else {
+      pj = A.pos[j];
+      vj = ld3(A.lv, cap, j);
+      wj = ld3(A.rv, cap, j);
+      idj = A.id[j];
+      tj = A.type[j];
+    }
+    int shifted = 0;
+    const double xj = image(o.pi.x, pj.x, P.simlen[0], P.periodic[0], &shifted);
+    const double yj = image(o.pi.y, pj.y, P.simlen[1], P.periodic[1], &shifted);
+    const double zj = image(o.pi.z, pj.z, P.simlen[2], P.periodic[2], &shifted);
+    const double dr = ri + pj.w;
+    const double ddx = xi.x - xj, ddy = xi.y - yj, ddz = xi.z - zj;
+    double d2 = ddx * ddx;
+    d2 = ddy * ddy + d2;
+    d2 = ddz * ddz + d2;
+    // |xi - xj| as .nv. takes it too: the components are the same up to
+    // sign, and the sums the same up to order of two exact operands
+    const double dist = sqrt(d2);
+    const double ovrlp = dr - dist;
+    const d3 xjv = D3(xj, yj, zj);
+    const Bnry* Bp;
+    int kind;
+    bool flip = false;
+    if (e & CL_FIXED) {
+      // PPFix: the moving particle's record, its own perspective
+      kind = 3;
+      Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+    } else if (!shifted) {
+      // PP: the record sits in the lower id's bucket and the force is
+      // taken from that side (FineSearch2 :525-529)
+      const bool ilo = idi < idj;
+      kind = ilo ? 1 : KIND_COPY;
+      const int tlo = ilo ? ti : tj, thi = ilo ? tj : ti;
+      Bp = &P.pp[(tlo - 1) * P.ntype + (thi - 1)];
+      flip = !ilo;
+    } else {
+      // PPG: each side keeps its own record against the other's image,
+      // the force from the lower id's side; row type = the local particle
+      kind = 2;
+      Bp = &P.pp[(ti - 1) * P.ntype + (tj - 1)];
+      flip = idi > idj;
+    }
+    // .nv. of the two centres (zero within 1e-10 of each other)
+    const d3 sep = flip ? sub3(xi, xjv) : sub3(xjv, xi);
+    const d3 nvec = dist > 1.0e-10 ? div3(sep, dist) : D3(0.0, 0.0, 0.0);
+    const CF cf = contact_force(P, *Bp, nvec, flip ? pj.w : ri, flip ? ri : pj.w, flip ? vj : o.vi,
+                                flip ? o.vi : vj, flip ? wj : o.wi, flip ? o.wi : wj, ovrlp, h.td, h.vri);
+    apply(cf, flip, kind, idj);
+  }
+  if (!(parts & 2)) return;
+  // the walls, highest index first: the ones in some list of the wave
+  for (int w = P.nwall - 1; w >= 0; w--) {
+    if (!__ballot((o.mask0 >> w) & 1ull)) continue;
+    if (!((o.mask0 >> w) & 1ull)) continue;
+    const Wall& W = P.walls[w];
+    d3 nv;
+    double ovrlp;
+    if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
+    const Hist h = hist_at(A, cap, s, own_row<false>(A, cap, s, head_keys(A, s), W.wall_id, acc.hc));
+    const Bnry& Bw = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
+    const d3 nvec = mul3(nv, -1.0);  // Norm_v = -nv
+    const CF cf = contact_force(P, Bw, nvec, ri, 1.00e20 * ri, o.vi, wp(W.trans_vel), o.wi, D3(0.0, 0.0, 0.0),
+                                ovrlp, h.td, h.vri);
+    apply(cf, false, 4, W.wall_id);
   }
 }
 
