@@ -7,7 +7,6 @@
 // accumulation is fp32.  The forward pass keeps every activation the
 // backward pass needs; the backward pass is written out layer by layer.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -72,10 +71,10 @@ struct sdem_gnn {
   size_t part_len = 0;
   float* loss = nullptr;
   // graph building scratch
-  int *gkey = nullptr, *gval = nullptr, *gkey2 = nullptr, *gval2 = nullptr, *cs = nullptr, *ce = nullptr;
-  int* deg = nullptr;
-  void* cub_tmp = nullptr;
-  size_t cub_bytes = 0;
+  // graph search scratch: cell of each particle, particles in cell order,
+  // per-cell counts / fill cursors / ranges, degrees, scan tile sums
+  int *gkey = nullptr, *gsorted = nullptr, *gcnt = nullptr, *gfill = nullptr, *cs = nullptr, *ce = nullptr;
+  int *deg = nullptr, *tsum = nullptr;
   int64_t ncell_alloc = 0;
   int search_n = 0;
   int gcap_n = 0;
@@ -397,37 +396,25 @@ int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G, bool tra
   hipError_t e = hipSuccess;
   if (n > M->search_n || ncell > M->ncell_alloc || !M->gkey) {
     // (re)allocate the search scratch
-    for (void* p : {(void*)M->gkey, (void*)M->gval, (void*)M->gkey2, (void*)M->gval2, (void*)M->cs, (void*)M->ce,
-                    (void*)M->deg, M->cub_tmp}) {
+    for (void* p : {(void*)M->gkey, (void*)M->gsorted, (void*)M->gcnt, (void*)M->gfill, (void*)M->cs, (void*)M->ce,
+                    (void*)M->deg, (void*)M->tsum}) {
       auto it = std::find(A.begin(), A.end(), p);
       if (it != A.end()) { (void)hipFree(p); A.erase(it); }
     }
     const size_t N = (size_t)std::max(n, 1);
     auto Gl = [&](auto** p, size_t k) { if (e == hipSuccess) e = galloc(A, p, k); };
-    Gl(&M->gkey, N); Gl(&M->gval, N); Gl(&M->gkey2, N); Gl(&M->gval2, N);
+    Gl(&M->gkey, N); Gl(&M->gsorted, N); Gl(&M->gcnt, (size_t)ncell); Gl(&M->gfill, (size_t)ncell);
     Gl(&M->cs, (size_t)ncell); Gl(&M->ce, (size_t)ncell); Gl(&M->deg, N + 1);
-    size_t b1 = 0, b2 = 0;
-    if (e == hipSuccess)
-      e = hipcub::DeviceRadixSort::SortPairs(nullptr, b1, M->gkey, M->gkey2, M->gval, M->gval2, n, 0, 32, st);
-    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, b2, M->deg, M->deg, n + 1, st);
-    M->cub_bytes = std::max(b1, b2);
-    char* tmp = nullptr;
-    if (e == hipSuccess) e = galloc(A, &tmp, M->cub_bytes);
-    M->cub_tmp = tmp;
+    Gl(&M->tsum, (size_t)std::max<int64_t>(1, std::max(scan_tiles(ncell), scan_tiles((int64_t)N + 1))));
     M->ncell_alloc = ncell;
     M->search_n = n;
     if (e != hipSuccess) return hipf(M, e, "hipMalloc(graph search)");
   }
-  e = cell_key(pos, n, G, M->gkey, M->gval, st);
-  if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(M->cub_tmp, M->cub_bytes, M->gkey, M->gkey2, M->gval, M->gval2, n, 0, 32, st);
-  if (e == hipSuccess) e = hipMemsetAsync(M->cs, 0, (size_t)ncell * 4, st);
-  if (e == hipSuccess) e = hipMemsetAsync(M->ce, 0, (size_t)ncell * 4, st);
-  if (e == hipSuccess) e = cell_range(M->gkey2, n, M->cs, M->ce, st);
-  if (e == hipSuccess) e = edges(false, pos, n, G, M->gval2, M->cs, M->ce, M->deg, nullptr, nullptr, nullptr, st);
+  e = cell_sort(pos, n, G, ncell, M->gkey, M->gcnt, M->gfill, M->cs, M->ce, M->gsorted, M->tsum, st);
+  if (e == hipSuccess) e = edges(false, pos, n, G, M->gsorted, M->cs, M->ce, M->deg, nullptr, nullptr, nullptr, st);
   if (e == hipSuccess) e = hipMemsetAsync(M->deg + n, 0, 4, st);
-  // degrees -> receiver offsets (in place is fine for the scan)
-  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(M->cub_tmp, M->cub_bytes, M->deg, M->deg, n + 1, st);
+  // degrees -> receiver offsets (in place)
+  if (e == hipSuccess) e = excl_scan(M->deg, M->deg, n + 1, M->tsum, st);
   int total = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&total, M->deg + n, 4, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -437,7 +424,7 @@ int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G, bool tra
   M->ne = total;
   e = hipMemcpyAsync(M->rstart, M->deg, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, st);
   if (e == hipSuccess)
-    e = edges(true, pos, n, G, M->gval2, M->cs, M->ce, nullptr, M->rstart, M->esrc, M->edst, st);
+    e = edges(true, pos, n, G, M->gsorted, M->cs, M->ce, nullptr, M->rstart, M->esrc, M->edst, st);
   return e == hipSuccess ? SDEM_OK : hipf(M, e, "graph edges");
 }
 

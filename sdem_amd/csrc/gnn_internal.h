@@ -56,8 +56,14 @@ struct FeatScale {
 };
 
 hipError_t gather_add(const float* src, const int* idx, int R, int C, float* dst, hipStream_t st);
-hipError_t cell_key(const double4* pos, int n, const GGrid& G, int* key, int* val, hipStream_t st);
-hipError_t cell_range(const int* skey, int n, int* cstart, int* cend, hipStream_t st);
+// exclusive scan of n ints (in == out allowed); tsum: scan_tiles(n) ints
+int64_t scan_tiles(int64_t n);
+hipError_t excl_scan(const int* in, int* out, int64_t n, int* tsum, hipStream_t st);
+// particles sorted by cell (ascending index within a cell): sorted[cstart[c]
+// .. cend[c]) = the particles of cell c; key, cnt, fill are scratch (n,
+// ncell, ncell ints), tsum scan_tiles(ncell) ints
+hipError_t cell_sort(const double4* pos, int n, const GGrid& G, int64_t ncell, int* key, int* cnt, int* fill,
+                     int* cstart, int* cend, int* sorted, int* tsum, hipStream_t st);
 hipError_t edges(bool fill, const double4* pos, int n, const GGrid& G, const int* sorted, const int* cstart,
                  const int* cend, int* deg, const int* rstart, int* esrc, int* edst, hipStream_t st);
 hipError_t node_feat(const double4* pos, const double* vel, const double* acc, int n, int stride,

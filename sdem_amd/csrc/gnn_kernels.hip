@@ -450,21 +450,105 @@ __device__ __forceinline__ int gcell(const GGrid& G, int d, double x) {
   return c < 0 ? 0 : (c >= G.n[d] ? G.n[d] - 1 : c);
 }
 
-__global__ void k_cell_key(const double4* __restrict__ pos, int n, GGrid G, int* __restrict__ key,
-                           int* __restrict__ val) {
+// Counting sort of the particles by cell: the cell of each particle and
+// the cells' counts (cnt zeroed by the caller), then (after an exclusive
+// scan of the counts into cstart) each particle into its cell's range, and
+// each cell's range put in ascending particle order -- the order a stable
+// sort by cell gives, whatever order the atomics took.
+__global__ void k_cell_count(const double4* __restrict__ pos, int n, GGrid G, int* __restrict__ key,
+                             int* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 p = pos[i];
-  key[i] = (gcell(G, 2, p.z) * G.n[1] + gcell(G, 1, p.y)) * G.n[0] + gcell(G, 0, p.x);
-  val[i] = i;
+  const int k = (gcell(G, 2, p.z) * G.n[1] + gcell(G, 1, p.y)) * G.n[0] + gcell(G, 0, p.x);
+  key[i] = k;
+  atomicAdd(&cnt[k], 1);
 }
 
-__global__ void k_cell_range(const int* __restrict__ skey, int n, int* __restrict__ cstart, int* __restrict__ cend) {
+__global__ void k_cell_fill(const int* __restrict__ key, int n, const int* __restrict__ cstart,
+                            int* __restrict__ fill, int* __restrict__ sorted) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int k = skey[i];
-  if (i == 0 || skey[i - 1] != k) cstart[k] = i;
-  if (i == n - 1 || skey[i + 1] != k) cend[k] = i + 1;
+  const int k = key[i];
+  sorted[cstart[k] + atomicAdd(&fill[k], 1)] = i;
+}
+
+__global__ void k_cell_order(int64_t ncell, const int* __restrict__ cstart, const int* __restrict__ cnt,
+                             int* __restrict__ cend, int* __restrict__ sorted) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int a = cstart[c], b = a + cnt[c];
+  cend[c] = b;
+  for (int q = a + 1; q < b; q++) {  // insertion sort: a cell holds a few particles
+    const int v = sorted[q];
+    int r = q - 1;
+    while (r >= a && sorted[r] > v) {
+      sorted[r + 1] = sorted[r];
+      r--;
+    }
+    sorted[r + 1] = v;
+  }
+}
+
+// Exclusive scan of n ints (in place allowed): tiles of SC_TILE summed and
+// scanned per block (wave scans by shuffles), the tiles' sums scanned by
+// one block, then added.
+constexpr int SC_T = 256, SC_PER = 4, SC_TILE = SC_T * SC_PER;
+
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int q = 0; q < SC_T / 64; q++) {
+    if (q < w) before += sh[q];
+    all += sh[q];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_tile(const int* in, int* out, int64_t n, int* __restrict__ tsum) {
+  __shared__ int sh[SC_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_PER;
+  int v[SC_PER], s = 0;
+  for (int q = 0; q < SC_PER; q++) {
+    v[q] = base + q < n ? in[base + q] : 0;
+    s += v[q];
+  }
+  int total;
+  int run = block_excl_scan(s, sh, &total);
+  for (int q = 0; q < SC_PER; q++) {
+    if (base + q < n) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_top(int* __restrict__ tsum, int nt) {
+  __shared__ int sh[SC_T / 64];
+  int carry = 0;
+  for (int c0 = 0; c0 < nt; c0 += SC_T) {
+    const int i = c0 + (int)threadIdx.x;
+    const int v = i < nt ? tsum[i] : 0;
+    int total;
+    const int ex = block_excl_scan(v, sh, &total);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_add(int* __restrict__ out, int64_t n, const int* __restrict__ tsum) {
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_PER;
+  const int add = tsum[blockIdx.x];
+  for (int q = 0; q < SC_PER; q++)
+    if (base + q < n) out[base + q] += add;
 }
 
 template <bool FILL>
@@ -698,13 +782,26 @@ hipError_t gather_add(const float* src, const int* idx, int R, int C, float* dst
   return hipGetLastError();
 }
 
-hipError_t cell_key(const double4* pos, int n, const GGrid& G, int* key, int* val, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_key, dim3(nb(n, 256)), dim3(256), 0, st, pos, n, G, key, val);
+int64_t scan_tiles(int64_t n) { return (n + SC_TILE - 1) / SC_TILE; }
+
+hipError_t excl_scan(const int* in, int* out, int64_t n, int* tsum, hipStream_t st) {
+  const int64_t nt = scan_tiles(n);
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nt), dim3(SC_T), 0, st, in, out, n, tsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_T), 0, st, tsum, (int)nt);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(SC_T), 0, st, out, n, tsum);
   return hipGetLastError();
 }
 
-hipError_t cell_range(const int* skey, int n, int* cstart, int* cend, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_range, dim3(nb(n, 256)), dim3(256), 0, st, skey, n, cstart, cend);
+hipError_t cell_sort(const double4* pos, int n, const GGrid& G, int64_t ncell, int* key, int* cnt, int* fill,
+                     int* cstart, int* cend, int* sorted, int* tsum, hipStream_t st) {
+  hipError_t e = hipMemsetAsync(cnt, 0, (size_t)ncell * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(fill, 0, (size_t)ncell * 4, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cell_count, dim3(nb(n, 256)), dim3(256), 0, st, pos, n, G, key, cnt);
+  if ((e = excl_scan(cnt, cstart, ncell, tsum, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cell_fill, dim3(nb(n, 256)), dim3(256), 0, st, key, n, cstart, fill, sorted);
+  hipLaunchKernelGGL(k_cell_order, dim3(nb((size_t)ncell, 256)), dim3(256), 0, st, ncell, cstart, cnt, cend, sorted);
   return hipGetLastError();
 }
 

@@ -328,6 +328,23 @@ __device__ __forceinline__ int block_box(const int* __restrict__ blk_box) {
   return blk_box[(blockIdx.x * blockDim.x) / BOX_ALIGN];
 }
 
+// The workgroups of a launch go to the chip's NXCD compute dies in turn
+// (block b to die b % NXCD), and each die has an L2 of its own.  The force
+// kernel numbers its tiles so that each die takes runs of XCD_RUN
+// consecutive tiles: the partners a tile gathers (close in cell order) are
+// then more often in the L2 of the die that gathers them, while all dies
+// still work in one region of memory at a time (one contiguous range per
+// die was slower: 8 streams far apart).  A bijection of [0, gridDim.x); the
+// last incomplete round of runs keeps the plain numbering.
+constexpr int NXCD = 8, XCD_RUN = 16;
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int full = (nb / (NXCD * XCD_RUN)) * (NXCD * XCD_RUN);
+  if (b >= full) return b;
+  const int x = b % NXCD, i = b / NXCD;
+  return ((i / XCD_RUN) * NXCD + x) * XCD_RUN + i % XCD_RUN;
+}
+
 __global__ __launch_bounds__(256) void k_pre(const Params* __restrict__ PB, const int* __restrict__ blk_box,
                                              Buf A, Scalars* S, BoxState* BS, BoxSlot* bslots) {
   __shared__ double shd[4];
@@ -1694,9 +1711,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // a candidate-list overflow in this step's rebuild leaves the lists
   // incomplete: do nothing, the host regrows them and replays the step
   if (S->halt || S->overflow >= 2) return;
-  const int box = block_box(blk_box);
+  const int tile = xcd_tile();
+  const int box = blk_box[(tile * blockDim.x) / BOX_ALIGN];
   const Params& P = PB[box];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * blockDim.x + threadIdx.x;
   const StepFlags f = step_flags(S, BS[box]);
   Acc acc = acc_zero();
   double move2 = 0.0, vnext = 0.0, anext = 0.0;
