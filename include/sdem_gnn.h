@@ -80,7 +80,11 @@ int sdem_gnn_train_step(sdem_gnn* m, const float* target, float lr, float clip, 
 
 /* `calls` surrogate calls on a DEM context: each advances its state by nsub
  * steps on the device; the context then goes on stepping from there, with
- * iterNumber advanced and its contact history cleared */
+ * iterNumber advanced and its contact history cleared.  A graph of more than
+ * SDEM_GNN_MAX_EDGES_PER_NODE edges per particle (a state the surrogate has
+ * let collapse) is refused with SDEM_E_STATE; the context then holds the
+ * state after the calls that completed, for the plain DEM to take over. */
+#define SDEM_GNN_MAX_EDGES_PER_NODE 48
 int sdem_gnn_advance(sdem_gnn* m, sdem_ctx* dem, int32_t calls);
 
 /* checks of the matrix-core building blocks: one v_mfma_f32_32x32x16_bf16

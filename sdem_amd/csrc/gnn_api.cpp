@@ -419,6 +419,9 @@ int build_graph(sdem_gnn* M, const double4* pos, int n, const GGrid& G, bool tra
   if (e == hipSuccess) e = hipMemcpyAsync(&total, M->deg + n, 4, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hipf(M, e, "graph search");
+  if ((int64_t)total > (int64_t)SDEM_GNN_MAX_EDGES_PER_NODE * n)
+    return M->fail(SDEM_E_STATE, "surrogate graph too dense (" + std::to_string(total) + " edges for " +
+                                     std::to_string(n) + " particles): the state has collapsed");
   if (int rc = alloc_graph(M, n, total, train)) return rc;
   M->n = n;
   M->ne = total;

@@ -449,6 +449,17 @@ __device__ __forceinline__ int gcell(const GGrid& G, int d, double x) {
   int c = (int)floor((x - G.lo[d]) * G.inv_w[d]);
   return c < 0 ? 0 : (c >= G.n[d] ? G.n[d] - 1 : c);
 }
+// A particle more than a cut-off outside the domain in a closed dimension
+// (one the surrogate has thrown out: the DEM removes it at its next step)
+// is left out of the cells and has no edges, instead of joining the edge
+// cells: there such particles, in their thousands, made the search
+// quadratic in their number.
+__device__ __forceinline__ bool ginside(const GGrid& G, double x, double y, double z) {
+  const double v[3] = {x, y, z};
+  for (int d = 0; d < 3; d++)
+    if (!G.per[d] && !(v[d] >= G.lo[d] - G.rc && v[d] < G.lo[d] + G.L[d] + G.rc)) return false;
+  return true;
+}
 
 // Counting sort of the particles by cell: the cell of each particle and
 // the cells' counts (cnt zeroed by the caller), then (after an exclusive
@@ -460,6 +471,10 @@ __global__ void k_cell_count(const double4* __restrict__ pos, int n, GGrid G, in
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 p = pos[i];
+  if (!ginside(G, p.x, p.y, p.z)) {
+    key[i] = -1;
+    return;
+  }
   const int k = (gcell(G, 2, p.z) * G.n[1] + gcell(G, 1, p.y)) * G.n[0] + gcell(G, 0, p.x);
   key[i] = k;
   atomicAdd(&cnt[k], 1);
@@ -470,7 +485,7 @@ __global__ void k_cell_fill(const int* __restrict__ key, int n, const int* __res
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int k = key[i];
-  sorted[cstart[k] + atomicAdd(&fill[k], 1)] = i;
+  if (k >= 0) sorted[cstart[k] + atomicAdd(&fill[k], 1)] = i;
 }
 
 __global__ void k_cell_order(int64_t ncell, const int* __restrict__ cstart, const int* __restrict__ cnt,
@@ -558,6 +573,10 @@ __global__ void k_edges(const double4* __restrict__ pos, int n, GGrid G, const i
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 p = pos[i];
+  if (!ginside(G, p.x, p.y, p.z)) {
+    if (!FILL) deg[i] = 0;
+    return;
+  }
   const double xv[3] = {p.x, p.y, p.z};
   int c[3];
   for (int d = 0; d < 3; d++) c[d] = gcell(G, d, xv[d]);

@@ -48,19 +48,39 @@ def make_model(hidden=128, layers=8, nsub=10, dt=5.0e-6, dmax=4.0e-3, vel_scale=
     return gnn.Model(cfg, seed=seed, lib=lib)
 
 
+# A contact-bearing window of the settling bed: the loose pack is first
+# collapsed by the DEM itself for these (steps, gravity scale) stages, which
+# leave it in contact with the floor and with itself (about one contact
+# record per sphere, still moving), and the window starts there at the
+# configuration's own gravity.
+CONTACT_WINDOW = ((14_000, 2.0),)
+
+
+def start_state(n: int, seed: int, settle=(), log=None, lib=None):
+    """The settling bed of n spheres (configuration-2 physics, xy area scaled
+    to n), advanced by the DEM through the `settle` stages ((steps, gravity
+    scale) pairs) when given."""
+    sc = scenarios.settling(n=n, seed=seed)
+    for steps, gscale in settle:
+        sc = scenarios.settle(sc, steps, gravity_scale=gscale, chunk=max(1, steps // 2), log=log, lib=lib,
+                              tuning=dict(cell_factor=(0.5, 0.5, 0.5), skin=0.05))
+    return sc
+
+
 def train(model: gnn.Model, n: int = 50_000, seeds=(101, 102, 103), warm: int = 200, snapshots: int = 300,
-          lr: float = 1e-3, lr_end: float = 2e-4, spacing: int = 10, log=print, lib=None) -> List[float]:
+          lr: float = 1e-3, lr_end: float = 2e-4, spacing: int = 10, log=print, lib=None, settle=()) -> List[float]:
     """Online training on settling boxes of n spheres (configuration-2
-    physics, xy area scaled to n): after `warm` steps, every `spacing` DEM
-    steps one snapshot -- graph and features of the current state, the
-    DEM's next nsub steps as target -- and one optimisation step.  Returns
-    per snapshot the loss divided by the loss of a zero prediction (the
+    physics, xy area scaled to n), each first advanced through the `settle`
+    stages (start_state): after `warm` steps, every `spacing` DEM steps one
+    snapshot -- graph and features of the current state, the DEM's next
+    nsub steps as target -- and one optimisation step.  Returns per
+    snapshot the loss divided by the loss of a zero prediction (the
     ballistic step without the network)."""
     lib = lib or native.load_hip_library()
     nsub = model.cfg.nsub
     boxes = []
     for sd in seeds:
-        sc = scenarios.settling(n=n, seed=sd)
+        sc = start_state(n, sd, settle, log=None, lib=lib)
         eng = native.build_engine(lib, sc, tuning=dict(cell_factor=(0.5, 0.5, 0.5), skin=0.05, keep_forces=0))
         eng.iterate(warm)
         o = sc["options"]
@@ -107,60 +127,97 @@ def bed_metrics(state: Dict, area: float, slab: float = 0.02) -> Dict[str, float
 
 
 def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: int = 200, steps: int = 2000,
-            dem_every: int = 500, dem_steps: int = 10, log=print, lib=None) -> Dict:
-    """The same settling bed three ways from one state after `warm` DEM
-    steps: plain DEM for `steps` steps (the judge), the surrogate for
-    steps / nsub calls (with `dem_steps` plain DEM steps after every
-    `dem_every` calls, as judge and fallback), and the ballistic step
-    without the network.  Returns timings and errors against the DEM."""
+            dem_every: int = 500, dem_steps: int = 10, log=print, lib=None, settle=()) -> Dict:
+    """The same settling bed (advanced through the `settle` stages first,
+    start_state) three ways from one state after `warm` DEM steps: plain
+    DEM for `steps` steps (the judge), the surrogate for steps / nsub calls
+    (with `dem_steps` plain DEM steps after every `dem_every` calls, as
+    judge and fallback), and the ballistic step without the network.
+    Returns timings and errors against the DEM."""
     lib = lib or native.load_hip_library()
     nsub = model.cfg.nsub
-    sc = scenarios.settling(n=n, seed=seed)
+    sc = start_state(n, seed, settle, log=None, lib=lib)
     tune = dict(cell_factor=(0.5, 0.5, 0.5), skin=0.05, keep_forces=0)
     o = sc["options"]
     area = (o["domain_max"][0] - o["domain_min"][0]) * (o["domain_max"][1] - o["domain_min"][1])
     out = {}
+    if log:
+        log(f"# rollout: {n} spheres, start state ready")
     ref = native.build_engine(lib, sc, tuning=tune)
     ref.iterate(warm)
+    out["contact_records_per_sphere_start"] = float(ref.get_counts().sum()) / max(1, ref.num_particles())
     t = time.perf_counter()
     ref.iterate(steps)
     out["dem_s"] = time.perf_counter() - t
+    out["contact_records_per_sphere_end"] = float(ref.get_counts().sum()) / max(1, ref.num_particles())
     s_ref = ref.get_state()
     ref.close()
+    if log:
+        log(f"# rollout: DEM {steps} steps in {out['dem_s']:.2f}s")
 
-    def run(m: gnn.Model):
+    def run(m: gnn.Model, name: str):
         eng = native.build_engine(lib, sc, tuning=tune)
         eng.iterate(warm)
         calls = steps // nsub
         done = 0
         t0 = time.perf_counter()
+        fell_back = None
         while done < calls:
-            k = min(dem_every, calls - done)
-            m.advance(eng, k)
-            done += k
-            if done < calls and dem_steps > 0:
-                eng.iterate(dem_steps)  # the DEM takes over briefly
+            k = min(100, dem_every - done % dem_every, calls - done)
+            try:
+                for c in range(k):
+                    if fell_back is None:
+                        try:
+                            m.advance(eng, 1)
+                        except native.SdemError as err:
+                            # the state collapsed under the surrogate: the
+                            # plain DEM takes over for the rest of the window
+                            fell_back = done + c
+                            if log:
+                                log(f"# rollout {name}: DEM fallback after {fell_back} calls: {err}")
+                    if fell_back is not None:
+                        eng.iterate(nsub)
+                done += k
+                if done < calls and dem_steps > 0 and done % dem_every == 0:
+                    eng.iterate(dem_steps)  # the DEM takes over briefly
+            except native.SdemError as err:
+                # the DEM refuses the state the surrogate left (spheres
+                # driven into each other blow the bed apart): no result
+                out[name + "_blew_up_after_calls"] = done
+                if log:
+                    log(f"# rollout {name}: the DEM blew up after {done} calls: {err}")
+                eng.close()
+                out[name + "_fallback_after_calls"] = fell_back
+                return None, time.perf_counter() - t0, 0
+            if log:
+                log(f"# rollout {name}: {done}/{calls} calls, {time.perf_counter() - t0:.2f}s, "
+                    f"{eng.num_particles()} spheres")
         el = time.perf_counter() - t0
         extra = (calls - 1) // dem_every * dem_steps if dem_steps > 0 else 0
         st = eng.get_state()
         eng.close()
+        out[name + "_fallback_after_calls"] = fell_back
         return st, el, extra
 
-    s_sur, out["surrogate_s"], extra = run(model)
+    s_sur, out["surrogate_s"], extra = run(model, "surrogate")
     out["edges"] = int(model.ne) if hasattr(model, "ne") else 0
     zero = gnn.Model(model.cfg, seed=1, lib=lib)
     zero.set_params(np.zeros(zero.num_params(), np.float32))
-    s_bal, out["ballistic_s"], _ = run(zero)
+    s_bal, out["ballistic_s"], _ = run(zero, "ballistic")
     zero.close()
-    mr, ms, mb = bed_metrics(s_ref, area), bed_metrics(s_sur, area), bed_metrics(s_bal, area)
+    mr = bed_metrics(s_ref, area)
+    ms = bed_metrics(s_sur, area) if s_sur is not None else None
+    mb = bed_metrics(s_bal, area) if s_bal is not None else None
     for k in ("height", "phi_bottom", "ke"):
         out[k + "_dem"] = mr[k]
-        out[k + "_err_surrogate"] = abs(ms[k] - mr[k]) / mr[k]
-        out[k + "_err_ballistic"] = abs(mb[k] - mr[k]) / mr[k]
+        out[k + "_err_surrogate"] = abs(ms[k] - mr[k]) / mr[k] if ms else None
+        out[k + "_err_ballistic"] = abs(mb[k] - mr[k]) / mr[k] if mb else None
     L = np.array(o["domain_max"]) - np.array(o["domain_min"])
     per = np.array(o["is_periodic"], bool)
 
     def rms(s):
+        if s is None:
+            return None, None, None
         # particles present in both (a particle thrown out of the domain is deleted)
         ids, ia, ib = np.intersect1d(s["id"], s_ref["id"], return_indices=True)
         d = s["posr"][ia, :3] - s_ref["posr"][ib, :3]
