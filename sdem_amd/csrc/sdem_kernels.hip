@@ -1276,10 +1276,13 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
 // ---------------------------------------------------------------------------
 
 // the start-of-step state of one particle (buffer A, slot s) the contact
-// loop reads
+// loop reads (its spin rotVel(1) is loaded where it is used: by the contact
+// loop of a particle that touches something, and by the integration when it
+// reads rotVel(1), which AB2 rotation does not: a particle without contacts
+// does not load it)
 struct Own {
   double4 pi;
-  d3 vi, wi;
+  d3 vi;
   int idi, ti;
   uint64_t mask0;  // near-wall list
 };
@@ -1290,7 +1293,6 @@ __device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
   o.idi = A.id[s];
   o.ti = A.type[s];
   o.vi = ld3(A.lv, P.cap, s);
-  o.wi = ld3(A.rv, P.cap, s);
   o.mask0 = A.wmask[s];
   return o;
 }
@@ -1318,6 +1320,8 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
   const double ri = o.pi.w;
   const int idi = o.idi, ti = o.ti;
   const int nc = (parts & 1) ? min(ccnt[k], P.hcap) : 0;  // more: cnt_over is set
+  d3 wi = D3(0.0, 0.0, 0.0);
+  if (nc > 0 || ((parts & 2) && o.mask0 != 0ull)) wi = ld3(A.rv, cap, s);
   auto apply = [&](const CF& cf, bool flip, int kind, int partner) {
     if (!flip) {
       acc.F = add3(acc.F, cf.fsum);
@@ -1395,7 +1399,7 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
     const d3 sep = flip ? sub3(xi, xjv) : sub3(xjv, xi);
     const d3 nvec = dist > 1.0e-10 ? div3(sep, dist) : D3(0.0, 0.0, 0.0);
     const CF cf = contact_force(P, *Bp, nvec, flip ? pj.w : ri, flip ? ri : pj.w, flip ? vj : o.vi,
-                                flip ? o.vi : vj, flip ? wj : o.wi, flip ? o.wi : wj, ovrlp, h.td, h.vri);
+                                flip ? o.vi : vj, flip ? wj : wi, flip ? wi : wj, ovrlp, h.td, h.vri);
     apply(cf, flip, kind, idj);
   }
   if (!(parts & 2)) return;
@@ -1410,7 +1414,7 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
     const Hist h = hist_at(A, cap, s, own_row<false>(A, cap, s, head_keys(A, s), W.wall_id, acc.hc));
     const Bnry& Bw = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
     const d3 nvec = mul3(nv, -1.0);  // Norm_v = -nv
-    const CF cf = contact_force(P, Bw, nvec, ri, 1.00e20 * ri, o.vi, wp(W.trans_vel), o.wi, D3(0.0, 0.0, 0.0),
+    const CF cf = contact_force(P, Bw, nvec, ri, 1.00e20 * ri, o.vi, wp(W.trans_vel), wi, D3(0.0, 0.0, 0.0),
                                 ovrlp, h.td, h.vri);
     apply(cf, false, 4, W.wall_id);
   }
@@ -1474,7 +1478,7 @@ struct Integ {
                                         const Own& o, d3 a1, d3 al1, d3& lin1) {
     const int cap = P.cap;
     const int pim = PIM ? PIM : P.pi_method, prim = PRIM ? PRIM : P.pri_method;
-    const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z), vi = o.vi, wi = o.wi;
+    const d3 xi = D3(o.pi.x, o.pi.y, o.pi.z), vi = o.vi;
     const bool rv2 = PIM ? false : S->rv2_stored[A.idx] != 0;
     const double dt = P.dt;
     d3 pos = xi;
@@ -1514,6 +1518,7 @@ struct Integ {
       st3n(B.la + 3 * cap, cap, k, a2);
     }
     if (!PIM) th = ld3n(A.th, cap, s);
+    const d3 wi = prim == 2 ? D3(0.0, 0.0, 0.0) : ld3(A.rv, cap, s);  // rotVel(1), unused by AB2
     if (prim == 1) {
       th = add3(th, mul3(wi, dt));
       st3n(B.rv, cap, k, add3(wi, mul3(al1, dt)));
@@ -1840,7 +1845,6 @@ __global__ __launch_bounds__(256) void k_ph_integ(const Params* __restrict__ PB,
     o.ti = A.type[s];
     o.idi = 0;
     o.vi = ld3(A.lv, P.cap, s);
-    o.wi = ld3(A.rv, P.cap, s);
     o.mask0 = 0ull;
     Integ<PIM, PRIM> in;
     in.preload(P, S, A, s, o.vi, false);

@@ -85,6 +85,57 @@ def compare(gpu: native.Engine, ref: native.Engine, exact_positions=True, rtol=R
     return errs
 
 
+def close_rows(a, b, rtol=RTOL):
+    """Row by row: each row (a record's or a particle's vector) against the
+    size of its own row in b, not the system's largest."""
+    a, b = np.atleast_2d(np.asarray(a, float)), np.atleast_2d(np.asarray(b, float))
+    if a.shape != b.shape:
+        return False, f"shape {a.shape} vs {b.shape}"
+    if a.size == 0:
+        return True, ""
+    scale = np.linalg.norm(b, axis=1, keepdims=True)
+    bad = np.abs(a - b) > rtol * scale + 1e-300
+    if bad.any():
+        i = np.argwhere(bad)[0]
+        rel = np.abs(a - b).max(axis=1) / (scale[:, 0] + 1e-300)
+        return False, f"max row-relative err {rel.max():.3e} at {tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}"
+    return True, ""
+
+
+def compare_per_record(gpu: native.Engine, ref: native.Engine, rtol=RTOL):
+    """After one step from an identical state, the quantities that no
+    summation order enters, each against its own size: every contact
+    record's TanDelta (vector) and VelRel_Init, and the force and torque of
+    every particle that takes part in exactly one contact record (its sum
+    has one term).  Returns a list of failure strings."""
+    errs = []
+    gc, rc = gpu.get_contacts(), ref.get_contacts()
+    for k in ("kind", "owner", "partner"):
+        if not np.array_equal(gc[k], rc[k]):
+            return [f"contact {k} differs"]
+    ok, m = close_rows(gc["tan_delta"], rc["tan_delta"], rtol)
+    if not ok:
+        errs.append("tan_delta per record: " + m)
+    ok, m = close_rows(gc["vrel_init"][:, None], rc["vrel_init"][:, None], rtol)
+    if not ok:
+        errs.append("vrel_init per record: " + m)
+    # the records a particle's sum takes: those it owns (each side of a
+    # periodic pair owns its own), and the PP records of a lower-id partner
+    ids, cnt = np.unique(np.concatenate([rc["owner"], rc["partner"][(rc["kind"] == native.KIND_PP)]]),
+                         return_counts=True)
+    single = set(ids[cnt == 1].tolist())
+    gi, gf, gt = gpu.get_forces()
+    ri, rf, rt = ref.get_forces()
+    if not np.array_equal(gi, ri):
+        return errs + ["force ids differ"]
+    sel = np.isin(ri, np.fromiter(single, dtype=ri.dtype, count=len(single)))
+    for name, a, b in (("force", gf, rf), ("torque", gt, rt)):
+        ok, m = close_rows(a[sel], b[sel], rtol)
+        if not ok:
+            errs.append(f"{name} of single-contact particles: " + m)
+    return errs
+
+
 def aging_lists_scene(seed=31, speed=3.0):
     """400 sparse spheres (no contact at the start), every one at the same
     speed in a random direction, periodic in x, y and z, dt 1e-5 s: pairs

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from parity import aging_lists_scene, compare, quads
+from parity import aging_lists_scene, compare, compare_per_record, quads
 from sdem_amd import native, scenarios
 
 pytestmark = pytest.mark.gpu
@@ -41,6 +41,8 @@ def test_one_step_parity(name, hip_lib, oracle_lib):
     gpu.iterate(1)
     ref.iterate(1)
     errs = compare(gpu, ref)
+    assert not errs, errs
+    errs = compare_per_record(gpu, ref)
     assert not errs, errs
     assert gpu.get_counts().sum() > 50
 
