@@ -1001,20 +1001,20 @@ __device__ __forceinline__ int key_at(const Buf& A, size_t cap, int b, const Hea
 }
 // The first record against `key` that is not a copy, or NO_ROW (with
 // `copy`: the first record against `key`, if it is a copy, else NO_ROW).
-template <bool COPY>
-__device__ __forceinline__ unsigned own_row(const Buf& A, size_t cap, int b, const HeadKeys& h, int key, int hint) {
+__device__ __forceinline__ unsigned own_row(const Buf& A, size_t cap, int b, const HeadKeys& h, int key, int hint,
+                                            bool copy) {
   if (hint < h.n) {
     const int k = key_at(A, cap, b, h, hint);
-    if (key_is(k, key) && is_copy(k) == COPY) return (unsigned)hint;
+    if (key_is(k, key) && is_copy(k) == copy) return (unsigned)hint;
   }
 #pragma unroll
   for (int r = 0; r < KEY_HEAD; r++) {
     const int k = key_at(A, cap, b, h, r);
-    if (r < h.n && key_is(k, key) && (COPY || !is_copy(k))) return is_copy(k) == COPY ? (unsigned)r : NO_ROW;
+    if (r < h.n && key_is(k, key) && (copy || !is_copy(k))) return is_copy(k) == copy ? (unsigned)r : NO_ROW;
   }
   for (int r = KEY_HEAD; r < h.n; r++) {
     const int k = A.hkey[key_slot(r, b, cap)];
-    if (key_is(k, key) && (COPY || !is_copy(k))) return is_copy(k) == COPY ? (unsigned)r : NO_ROW;
+    if (key_is(k, key) && (copy || !is_copy(k))) return is_copy(k) == copy ? (unsigned)r : NO_ROW;
   }
   return NO_ROW;
 }
@@ -1175,23 +1175,9 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       pjs[q] = fixed ? fpk[j] : A.pos[j];
       ent[q] = fixed ? e : (unsigned)j;
     }
-  // the rest of the list runs as a pipeline: while candidate q is tested,
-  // the position of q + 1 and the entry of q + 2 are in flight
-  auto resolve = [&](unsigned e) -> int {
-    return (e & CL_FIXED) ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
-  };
-  unsigned e1 = 0u, e2 = 0u;
-  if (CPF < nq) e1 = (unsigned)cand[(size_t)CPF * cap + t];
-  if (CPF + 1 < nq) e2 = (unsigned)cand[(size_t)(CPF + 1) * cap + t];
   const double4 pi = A.pos[s];
   const int idi = A.id[s];
   const HeadKeys own = head_keys(A, s);
-  int j1 = 0;
-  double4 p1 = make_double4(0.0, 0.0, 0.0, 0.0);
-  if (CPF < nq) {
-    j1 = resolve(e1);
-    p1 = (e1 & CL_FIXED) ? fpk[j1] : A.pos[j1];
-  }
   int c = 0;
   // one candidate: the reference's predicate, then the entry and its
   // history row
@@ -1212,28 +1198,16 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
     d2 = ddz * ddz + d2;
     if (d2 > dr2) return;
     if (c < P.hcap) {
-      // a moving partner's id, count and first keys in one batch; the
-      // record sits in the partner's bucket if its id is the lower
-      int idj;
-      unsigned row;
-      if (fixed) {
-        idj = fid[j];
-        row = own_row<false>(A, cap, s, own, idj, c);
-      } else {
-        idj = A.id[j];
-        if (shifted || idi < idj) {
-          row = own_row<false>(A, cap, s, own, idj, c);
-        } else {
-          // PP from the higher id: the lower id's record, as the particle's
-          // own copy; failing that, in the lower id's list (a pair that was
-          // periodic, or a copy the host did not install)
-          row = own_row<true>(A, cap, s, own, idj, c);
-          if (row == NO_ROW) {
-            const HeadKeys pk = head_keys(A, j);
-            const unsigned r_other = hist_row(A, cap, j, pk, idi);
-            row = r_other == NO_ROW ? NO_ROW : (r_other | ROW_PARTNER);
-          }
-        }
+      // the record in the particle's own list; for PP from the higher id
+      // that is its copy of the lower id's record, and failing one, the
+      // record in the lower id's list (a pair that was periodic, or a copy
+      // the host did not install)
+      const int idj = fixed ? fid[j] : A.id[j];
+      const bool copy = !fixed && !shifted && idj < idi;
+      unsigned row = own_row(A, cap, s, own, idj, c, copy);
+      if (copy && row == NO_ROW) {
+        const unsigned r_other = hist_row(A, cap, j, head_keys(A, j), idi);
+        row = r_other == NO_ROW ? NO_ROW : (r_other | ROW_PARTNER);
       }
       clist[(size_t)c * cap + k] = make_uint2(fixed ? e : ((unsigned)j | (shifted ? CL_SHIFT : 0u)), row);
     }
@@ -1248,16 +1222,10 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
       consider(ent[q], pjs[q], fixed, (int)(ent[q] & CL_INDEX));
     }
   for (int q = CPF; q < nq; q++) {
-    const unsigned e = e1;
-    const int j = j1;
-    const double4 pj = p1;
-    if (q + 1 < nq) {
-      e1 = e2;
-      j1 = resolve(e1);
-      p1 = (e1 & CL_FIXED) ? fpk[j1] : A.pos[j1];
-      if (q + 2 < nq) e2 = (unsigned)cand[(size_t)(q + 2) * cap + t];
-    }
-    consider(e, pj, (e & CL_FIXED) != 0, j);
+    const unsigned e = (unsigned)cand[(size_t)q * cap + t];
+    const bool fixed = (e & CL_FIXED) != 0;
+    const int j = fixed ? (int)(e & CL_INDEX) : (rebuilt ? perm[e & CL_INDEX] : (int)(e & CL_INDEX));
+    consider(e, fixed ? fpk[j] : A.pos[j], fixed, j);
   }
   ccnt[k] = c;
   // More contacts than history slots: the force kernel of the step turns
@@ -1403,15 +1371,16 @@ __device__ __forceinline__ void contact_loop(const Params& P, const Buf& A, cons
     apply(cf, flip, kind, idj);
   }
   if (!(parts & 2)) return;
-  // the walls, highest index first: the ones in some list of the wave
-  for (int w = P.nwall - 1; w >= 0; w--) {
-    if (!__ballot((o.mask0 >> w) & 1ull)) continue;
-    if (!((o.mask0 >> w) & 1ull)) continue;
+  // the walls of the particle's list, highest index first
+  uint64_t mask = o.mask0;
+  while (mask) {
+    const int w = 63 - __clzll((long long)mask);
+    mask &= ~(1ull << w);
     const Wall& W = P.walls[w];
     d3 nv;
     double ovrlp;
     if (!pw_in_contact(W, xi, ri, &ovrlp, &nv)) continue;
-    const Hist h = hist_at(A, cap, s, own_row<false>(A, cap, s, head_keys(A, s), W.wall_id, acc.hc));
+    const Hist h = hist_at(A, cap, s, own_row(A, cap, s, head_keys(A, s), W.wall_id, acc.hc, false));
     const Bnry& Bw = P.pw[(ti - 1) * P.nwtype + (W.wall_type - 1)];
     const d3 nvec = mul3(nv, -1.0);  // Norm_v = -nv
     const CF cf = contact_force(P, Bw, nvec, ri, 1.00e20 * ri, o.vi, wp(W.trans_vel), wi, D3(0.0, 0.0, 0.0),
