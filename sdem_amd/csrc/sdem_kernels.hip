@@ -1271,10 +1271,9 @@ __device__ __forceinline__ Own load_own(const Params& P, const Buf& A, int s) {
 // highest index).  Each contact is reduced to the arguments of the force
 // routine from the perspective the reference evaluates it from
 // (ContactForce_PP :732-756, _PPG :761-798, _PPFix :803-827, _PW :832-856).
-// The walls run in a loop of their own over the context's walls, one wall
-// at a time for the whole wave (its data in scalar registers), each lane
-// taking the walls of its own list in the same order.  The records go to slot k of
-// B in the same order (a PP pair's in both lists: KIND_COPY).
+// The walls run in a loop of their own after the particle contacts.  The
+// records go to slot k of B in the same order (a PP pair's in both lists:
+// KIND_COPY).
 // `parts` selects the particle contacts (1), the walls (2) or both (3), in
 // that order: the phase entry points run them in two kernels, which continue
 // the same sums.
