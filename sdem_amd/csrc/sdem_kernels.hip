@@ -732,22 +732,50 @@ __device__ __forceinline__ void vlist_insert(int* __restrict__ vlist, int* __res
   vlist[(size_t)(b + 1) * cap + k] = (int)entry;
 }
 
-// The first VL_LDS entries of a particle's list are kept sorted in LDS
-// (entry q of thread tid at q*256 + tid: a wave's lanes hit consecutive
-// banks); a longer list moves to vlist/vkey and is sorted there.  One
-// 256-thread block holds 24 KB (12 entries: as fast as 8 or 10 on dense
-// packs, 15 % faster than 16, whose 32 KB allow fewer blocks per CU).
+// A particle's list is appended to while the cells are walked (entries
+// only) and sorted once at the end, after its keys are loaded.  Its first
+// VL_LDS entries sit in LDS (entry q of thread tid at q*256 + tid: a wave's
+// lanes hit consecutive banks), the rest in vlist/vkey.  One 256-thread
+// block holds 24 KB of them.
 constexpr int VL_LDS = 12, VL_T = 256;
 struct VlStage {
   int key[VL_LDS * VL_T];
   int ent[VL_LDS * VL_T];
 };
 
-__device__ __forceinline__ void vlist_add(VlStage& L, int tid, int* __restrict__ vlist, int* __restrict__ vkey,
-                                          size_t cap, int k, int lcap, int& c, unsigned entry, int key) {
+__device__ __forceinline__ void vlist_append(VlStage& L, int tid, int* __restrict__ vlist, size_t cap, int k,
+                                             int lcap, int& c, unsigned entry) {
   if (c < lcap) {
-    if (c < VL_LDS) {
-      int b = c - 1;
+    if (c < VL_LDS) L.ent[c * VL_T + tid] = (int)entry;
+    else vlist[(size_t)c * cap + k] = (int)entry;
+  }
+  c++;
+}
+
+// sort key of a list entry: 2 id of a moving partner, 2 id + 1 of a fixed one
+__device__ __forceinline__ int entry_key(unsigned e, const int* __restrict__ pid, const int* __restrict__ fid) {
+  return (e & CL_FIXED) ? 2 * fid[e & CL_INDEX] + 1 : 2 * pid[e & CL_INDEX];
+}
+
+// The keys of a particle's nq <= lcap entries (loaded four at a time), then
+// an insertion sort by key (stable; equal keys only come from one partner
+// seen through two periodic images, whose entries are equal).
+__device__ __forceinline__ void vlist_sort(VlStage& L, int tid, int* __restrict__ vlist, int* __restrict__ vkey,
+                                           size_t cap, int k, int nq, const int* __restrict__ pid,
+                                           const int* __restrict__ fid) {
+  if (nq <= VL_LDS) {
+    for (int q = 0; q < nq; q += 4) {
+      int kk[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (q + u < nq) kk[u] = entry_key((unsigned)L.ent[(q + u) * VL_T + tid], pid, fid);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (q + u < nq) L.key[(q + u) * VL_T + tid] = kk[u];
+    }
+    for (int q = 1; q < nq; q++) {
+      const int key = L.key[q * VL_T + tid], ent = L.ent[q * VL_T + tid];
+      int b = q - 1;
       while (b >= 0) {
         const int kb = L.key[b * VL_T + tid];
         if (kb <= key) break;
@@ -756,18 +784,187 @@ __device__ __forceinline__ void vlist_add(VlStage& L, int tid, int* __restrict__
         b--;
       }
       L.key[(b + 1) * VL_T + tid] = key;
-      L.ent[(b + 1) * VL_T + tid] = (int)entry;
+      L.ent[(b + 1) * VL_T + tid] = ent;
+    }
+    return;
+  }
+  for (int q = 0; q < VL_LDS; q++) vlist[(size_t)q * cap + k] = L.ent[q * VL_T + tid];
+  for (int q = 0; q < nq; q += 4) {
+    int kk[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (q + u < nq) kk[u] = entry_key((unsigned)vlist[(size_t)(q + u) * cap + k], pid, fid);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (q + u < nq) vkey[(size_t)(q + u) * cap + k] = kk[u];
+  }
+  for (int q = 1; q < nq; q++) {
+    const int key = vkey[(size_t)q * cap + k], ent = vlist[(size_t)q * cap + k];
+    int b = q - 1;
+    while (b >= 0) {
+      const int kb = vkey[(size_t)b * cap + k];
+      if (kb <= key) break;
+      vkey[(size_t)(b + 1) * cap + k] = kb;
+      vlist[(size_t)(b + 1) * cap + k] = vlist[(size_t)b * cap + k];
+      b--;
+    }
+    vkey[(size_t)(b + 1) * cap + k] = key;
+    vlist[(size_t)(b + 1) * cap + k] = ent;
+  }
+}
+
+// The cells a particle searches on a grid, [lo, hi] per axis: those
+// overlapping [x - reach, x + reach], clamped to the grid (periodic axes to
+// one period either side).
+__device__ __forceinline__ void search_range(const Params& P, const Grid& G, const double xv[3], double reach,
+                                             int lo[3], int hi[3]) {
+  for (int d = 0; d < 3; d++) {
+    const double pad = 1.0e-6 * G.w[d];
+    lo[d] = cell_raw(G, d, xv[d] - reach - pad);
+    hi[d] = cell_raw(G, d, xv[d] + reach + pad);
+    if (!P.periodic[d]) {
+      lo[d] = lo[d] < 0 ? 0 : lo[d];
+      hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
     } else {
-      if (c == VL_LDS)
-#pragma unroll 1
-        for (int q = 0; q < VL_LDS; q++) {
-          vkey[(size_t)q * cap + k] = L.key[q * VL_T + tid];
-          vlist[(size_t)q * cap + k] = L.ent[q * VL_T + tid];
-        }
-      vlist_insert(vlist, vkey, cap, k, c, entry, key);
+      lo[d] = lo[d] < -G.n[d] ? -G.n[d] : lo[d];
+      hi[d] = hi[d] > 2 * G.n[d] - 1 ? 2 * G.n[d] - 1 : hi[d];
     }
   }
-  c++;
+}
+
+// One candidate: the reference's predicate with the skin, on the image
+// coordinates; the particle itself (no image) is left out.
+__device__ __forceinline__ void vl_test(const Params& P, const double4& pi, int k, const double4& pj, double shx,
+                                        double shy, double shz, unsigned entry, bool self_ok, VlStage& L, int tid,
+                                        int* __restrict__ vlist, size_t cap, int lcap, int& c, double& gmin) {
+  const double ri = pi.w;
+  const double ddx = pi.x - (pj.x + shx);
+  const double ddy = pi.y - (pj.y + shy);
+  const double ddz = pi.z - (pj.z + shz);
+  const double dr = ri + pj.w + P.skin;
+  const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+  if (d2 > dr * dr) return;
+  const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
+  if (!self_ok && entry == (unsigned)k && !shifted) return;
+  gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
+  vlist_append(L, tid, vlist, cap, k, lcap, c, entry);
+}
+
+// A particle's walk over one level's cells.  The walk is a chain of
+// dependent loads (a run's bounds, then its particles), so it is laid out
+// to keep several in flight: the x extent of the range splits into the
+// same runs of consecutive keys on every row (a row's cells within one
+// brick and one period), the bounds of a row's first three runs are loaded
+// together, one row ahead, and the row's particles four at a time.  Runs
+// beyond the third (ranges wider than three bricks) are walked one by one.
+__device__ __forceinline__ void vl_level(const Params& P, const Grid& G, const double4& pi, int k,
+                                         const int* __restrict__ cstart, int base, const double4* __restrict__ pk,
+                                         VlStage& L, int tid, int* __restrict__ vlist, size_t cap, int lcap, int& c,
+                                         double& gmin) {
+  const double xv[3] = {pi.x, pi.y, pi.z};
+  int lo[3], hi[3];
+  search_range(P, G, xv, pi.w + G.rmax + P.skin, lo, hi);
+  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;
+  const int n0 = G.n[0];
+  // x runs: key offsets of their first and last cells, the image shift
+  int a0 = 0, b0 = -1, a1 = 0, b1 = -1, a2 = 0, b2 = -1;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  int x = lo[0];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (x <= hi[0]) {
+      int wx = x;
+      double s = 0.0;
+      if (wx < 0) { wx += n0; s = -P.simlen[0]; }
+      else if (wx >= n0) { wx -= n0; s = P.simlen[0]; }
+      const int wend = min(wx | (BRICK - 1), n0 - 1);
+      const int x1 = min(hi[0], x + (wend - wx));
+      const int wx1 = wx + (x1 - x);
+      const int ka = ((wx >> BRICK_BITS) << (3 * BRICK_BITS)) | (wx & (BRICK - 1));
+      const int kb = ((wx1 >> BRICK_BITS) << (3 * BRICK_BITS)) | (wx1 & (BRICK - 1));
+      if (i == 0) { a0 = ka; b0 = kb; s0 = s; }
+      else if (i == 1) { a1 = ka; b1 = kb; s1 = s; }
+      else { a2 = ka; b2 = kb; s2 = s; }
+      x = x1 + 1;
+    }
+  }
+  const int xrest = x;  // first cell of the runs walked one by one
+  const int nrow = (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+  // bounds of row (cy, cz): first index and length of each of the three
+  // runs; the rows are walked y fastest
+  int ja0, ja1, ja2, n0r, n1r, n2r, wy, wz;
+  double shy, shz;
+  auto row_at = [&](int cy, int cz, int& j0, int& j1, int& j2, int& m0, int& m1, int& m2, int& ry, int& rz,
+                    double& sy, double& sz) {
+    sy = 0.0;
+    sz = 0.0;
+    if (cy < 0) { cy += G.n[1]; sy = -P.simlen[1]; }
+    else if (cy >= G.n[1]) { cy -= G.n[1]; sy = P.simlen[1]; }
+    if (cz < 0) { cz += G.n[2]; sz = -P.simlen[2]; }
+    else if (cz >= G.n[2]) { cz -= G.n[2]; sz = P.simlen[2]; }
+    ry = cy;
+    rz = cz;
+    const int64_t row = G.key_base +
+                        ((((int64_t)(cz >> BRICK_BITS) * G.nb[1] + (cy >> BRICK_BITS)) * G.nb[0]) << (3 * BRICK_BITS)) +
+                        ((((cz & (BRICK - 1)) << BRICK_BITS) | (cy & (BRICK - 1))) << BRICK_BITS);
+    const int* r = cstart + row;
+    j0 = r[a0] - base;
+    m0 = r[b0 + 1] - base - j0;
+    j1 = 0; m1 = 0; j2 = 0; m2 = 0;
+    if (b1 >= 0) {
+      j1 = r[a1] - base;
+      m1 = r[b1 + 1] - base - j1;
+    }
+    if (b2 >= 0) {
+      j2 = r[a2] - base;
+      m2 = r[b2 + 1] - base - j2;
+    }
+  };
+  int cy = lo[1], cz = lo[2];
+  row_at(cy, cz, ja0, ja1, ja2, n0r, n1r, n2r, wy, wz, shy, shz);
+  for (int rho = 0; rho < nrow; rho++) {
+    if (++cy > hi[1]) { cy = lo[1]; cz++; }
+    int na0 = 0, na1 = 0, na2 = 0, nm0 = 0, nm1 = 0, nm2 = 0, nwy = 0, nwz = 0;
+    double nsy = 0.0, nsz = 0.0;
+    if (rho + 1 < nrow) row_at(cy, cz, na0, na1, na2, nm0, nm1, nm2, nwy, nwz, nsy, nsz);
+    const int n01 = n0r + n1r, ntot = n01 + n2r;
+    for (int t = 0; t < ntot; t += 4) {
+      double4 q[4];
+      int kq[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int tt = t + u;
+        kq[u] = tt < n0r ? ja0 + tt : (tt < n01 ? ja1 + (tt - n0r) : ja2 + (tt - n01));
+        if (tt < ntot) q[u] = pk[kq[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int tt = t + u;
+        if (tt < ntot) {
+          const double sx = tt < n0r ? s0 : (tt < n01 ? s1 : s2);
+          vl_test(P, pi, k, q[u], sx, shy, shz, (unsigned)kq[u], false, L, tid, vlist, cap, lcap, c, gmin);
+        }
+      }
+    }
+    // runs past the third, one by one
+    for (int x0 = xrest; x0 <= hi[0];) {
+      int wx = x0;
+      double sx = 0.0;
+      if (wx < 0) { wx += n0; sx = -P.simlen[0]; }
+      else if (wx >= n0) { wx -= n0; sx = P.simlen[0]; }
+      const int wend = min(wx | (BRICK - 1), n0 - 1);
+      const int x1 = min(hi[0], x0 + (wend - wx));
+      const int ja = cstart[brick_key(G, wx, wy, wz)] - base;
+      const int jb = cstart[brick_key(G, wx + (x1 - x0), wy, wz) + 1] - base;
+      for (int kj = ja; kj < jb; kj++)
+        vl_test(P, pi, k, pk[kj], sx, shy, shz, (unsigned)kj, false, L, tid, vlist, cap, lcap, c, gmin);
+      x0 = x1 + 1;
+    }
+    ja0 = na0; ja1 = na1; ja2 = na2;
+    n0r = nm0; n1r = nm1; n2r = nm2;
+    wy = nwy; wz = nwz;
+    shy = nsy; shz = nsz;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, const int* __restrict__ blk_box,
@@ -784,19 +981,19 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const Params& P = PB[box];
   __shared__ VlStage stage;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = k - P.off < BS[box].n_alive;
   const size_t cap = P.cap;
   const int lcap = P.lcap;
   int c = 0;
   double gmin = 1.0e300;  // smallest gap to a candidate, |xi - xj| - (ri + rj)
-  double ri_live = 0.0;
+  double ri = 0.0;
   if (live) {
   const int base = box_start_base(P, cstart);
   const double4 pi = pk[k];
   const double xv[3] = {pi.x, pi.y, pi.z};
-  const double ri = pi.w;
-  ri_live = ri;
+  ri = pi.w;
   // The cell walk wraps a periodic dimension by one period at most: a
   // coordinate that is not finite, or that lies more than a period outside
   // the domain (a particle that crossed it in one step), is refused here
@@ -806,83 +1003,14 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
     if (!isfinite(xv[d]) || (P.periodic[d] && !(xv[d] >= P.dmin[d] - P.simlen[d] &&
                                                 xv[d] <= P.dmax[d] + P.simlen[d])))
       bad = true;
-  const int nlvl = bad ? 0 : P.nlvl;
   if (bad) atomicMax(&S->overflow, 3);
-  for (int L = 0; L < nlvl; L++) {
-    const Grid& G = P.lvl[L];
-    const double reach = ri + G.rmax + P.skin;
-    int lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-      const double pad = 1.0e-6 * G.w[d];
-      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-      if (!P.periodic[d]) {
-        lo[d] = lo[d] < 0 ? 0 : lo[d];
-        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-      } else {
-        lo[d] = lo[d] < -G.n[d] ? -G.n[d] : lo[d];
-        hi[d] = hi[d] > 2 * G.n[d] - 1 ? 2 * G.n[d] - 1 : hi[d];
-      }
-    }
-    for (int cz = lo[2]; cz <= hi[2]; cz++) {
-      int wz = cz;
-      double shz = 0.0;
-      if (wz < 0) { wz += G.n[2]; shz = -P.simlen[2]; }
-      else if (wz >= G.n[2]) { wz -= G.n[2]; shz = P.simlen[2]; }
-      for (int cy = lo[1]; cy <= hi[1]; cy++) {
-        int wy = cy;
-        double shy = 0.0;
-        if (wy < 0) { wy += G.n[1]; shy = -P.simlen[1]; }
-        else if (wy >= G.n[1]) { wy -= G.n[1]; shy = P.simlen[1]; }
-        for (int seg = 0; seg < 3; seg++) {
-          int xa, xb;
-          double shx;
-          if (seg == 0) { xa = lo[0]; xb = hi[0] < -1 ? hi[0] : -1; shx = -P.simlen[0]; }
-          else if (seg == 1) { xa = lo[0] > 0 ? lo[0] : 0; xb = hi[0] < G.n[0] - 1 ? hi[0] : G.n[0] - 1; shx = 0.0; }
-          else { xa = lo[0] > G.n[0] ? lo[0] : G.n[0]; xb = hi[0]; shx = P.simlen[0]; }
-          if (xa > xb) continue;
-          if (seg == 0) { xa += G.n[0]; xb += G.n[0]; }
-          if (seg == 2) { xa -= G.n[0]; xb -= G.n[0]; }
-          const bool shifted = (shx != 0.0) || (shy != 0.0) || (shz != 0.0);
-          // the row's cells xa..xb, one run of consecutive keys per brick
-          for (int x0 = xa; x0 <= xb;) {
-            const int x1 = min(xb, x0 | (BRICK - 1));
-            const int ja = cstart[brick_key(G, x0, wy, wz)] - base;
-            const int jb = cstart[brick_key(G, x1, wy, wz) + 1] - base;
-            for (int kj = ja; kj < jb; kj++) {
-              const double4 pj = pk[kj];
-              const double ddx = pi.x - (pj.x + shx);
-              const double ddy = pi.y - (pj.y + shy);
-              const double ddz = pi.z - (pj.z + shz);
-              const double dr = ri + pj.w + P.skin;
-              const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-              if (d2 > dr * dr) continue;
-              if (kj == k && !shifted) continue;
-              gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
-              vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj, 2 * pid[kj]);
-            }
-            x0 = x1 + 1;
-          }
-        }
-      }
-    }
-  }
+  const int nlvl = bad ? 0 : P.nlvl;
+  for (int lv = 0; lv < nlvl; lv++)
+    vl_level(P, P.lvl[lv], pi, k, cstart, base, pk, stage, tid, vlist, cap, lcap, c, gmin);
   if (!bad && P.search_fixed && P.nfix > 0) {
     const Grid& G = P.fgrid;
-    const double reach = ri + G.rmax + P.skin;
     int lo[3], hi[3];
-    for (int d = 0; d < 3; d++) {
-      const double pad = 1.0e-6 * G.w[d];
-      lo[d] = cell_raw(G, d, xv[d] - reach - pad);
-      hi[d] = cell_raw(G, d, xv[d] + reach + pad);
-      if (!P.periodic[d]) {
-        lo[d] = lo[d] < 0 ? 0 : lo[d];
-        hi[d] = hi[d] >= G.n[d] ? G.n[d] - 1 : hi[d];
-      } else {
-        lo[d] = lo[d] < -G.n[d] ? -G.n[d] : lo[d];
-        hi[d] = hi[d] > 2 * G.n[d] - 1 ? 2 * G.n[d] - 1 : hi[d];
-      }
-    }
+    search_range(P, G, xv, ri + G.rmax + P.skin, lo, hi);
     for (int cz = lo[2]; cz <= hi[2]; cz++) {
       int wz = cz;
       double shz = 0.0;
@@ -899,17 +1027,9 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
           if (wx < 0) { wx += G.n[0]; shx = -P.simlen[0]; }
           else if (wx >= G.n[0]) { wx -= G.n[0]; shx = P.simlen[0]; }
           const int64_t key = ((int64_t)wz * G.n[1] + wy) * G.n[0] + wx;
-          for (int kj = fstart[key]; kj < fstart[key + 1]; kj++) {
-            const double4 pj = fpk[kj];
-            const double ddx = pi.x - (pj.x + shx);
-            const double ddy = pi.y - (pj.y + shy);
-            const double ddz = pi.z - (pj.z + shz);
-            const double dr = ri + pj.w + P.skin;
-            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (d2 > dr * dr) continue;
-            gmin = fmin(gmin, sqrt(d2) - (ri + pj.w));
-            vlist_add(stage, tid, vlist, vkey, cap, k, lcap, c, (unsigned)kj | CL_FIXED, 2 * fid[kj] + 1);
-          }
+          for (int kj = fstart[key]; kj < fstart[key + 1]; kj++)
+            vl_test(P, pi, k, fpk[kj], shx, shy, shz, (unsigned)kj | CL_FIXED, true, stage, tid, vlist, cap,
+                    lcap, c, gmin);
         }
       }
     }
@@ -918,7 +1038,9 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
     atomicMax(&S->overflow, 2);
     atomicMax(&S->list_need, c);
   }
+  vlist_sort(stage, tid, vlist, vkey, cap, k, c < lcap ? c : lcap, pid, fid);
   }  // live
+  const double ri_live = ri;
   // The particles with candidates, packed per box (one atomic per wave; the
   // order does not enter any result): the contact test runs over these
   // only, with (particle, count, smallest gap) and the entries indexed by
@@ -935,7 +1057,6 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const unsigned long long bn = __ballot(near), bfar = __ballot(far);
   if (live && !has) ccnt[k] = 0;
   if (!(bn | bfar)) return;
-  const int lane = threadIdx.x & 63;
   const int ln = bn ? __ffsll((long long)bn) - 1 : 0, lf = bfar ? __ffsll((long long)bfar) - 1 : 0;
   int tn = 0, tf = 0;
   if (bn && lane == ln) tn = atomicAdd(&BS[box].n_act, __popcll(bn));
