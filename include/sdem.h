@@ -246,6 +246,13 @@ int sdem_get_counts(sdem_ctx* c, int32_t num_cntcts[4]);
  * (Prtcl_ContactSearchPW.f90:29,229). */
 int sdem_get_iter(sdem_ctx* c, int32_t* iter_number, int32_t* next_wall_update);
 
+/* Re-sorts and candidate-list builds so far, summed over the boxes (a
+ * statistic of this implementation, which keeps candidate lists between
+ * rebuilds; the reference searches the cells every step and has no
+ * counterpart).  A box rebuilds when the sum over steps of its largest
+ * particle move since its last rebuild may exceed half the skin. */
+int sdem_get_rebuilds(sdem_ctx* c, int64_t* rebuilds);
+
 /* Per-phase kernel timing with HIP events on the library's stream.
  * enable=1 starts collecting (resets totals); totals are in milliseconds. */
 int sdem_set_profiling(sdem_ctx* c, int32_t enable);

@@ -1744,6 +1744,17 @@ int sdem_get_counts(sdem_ctx* c, int32_t num[4]) {
   return SDEM_OK;
 }
 
+int sdem_get_rebuilds(sdem_ctx* c, int64_t* rebuilds) {
+  if (!c || !rebuilds) return SDEM_E_ARG;
+  *rebuilds = 0;
+  if (!c->stepped) return SDEM_OK;
+  BoxState B;
+  Scalars s;
+  if (int rc = read_box_state(c, &B, &s)) return rc;
+  *rebuilds = s.rebuilds;
+  return SDEM_OK;
+}
+
 int sdem_get_iter(sdem_ctx* c, int32_t* iter, int32_t* next_update) {
   if (!c) return SDEM_E_ARG;
   if (!c->stepped) {

@@ -119,6 +119,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "get_timers": ([V, _DP, _LP], C.c_int),
         "get_profile": ([V, _DP, _LP], C.c_int),
         "get_device_bytes": ([V, _LP], C.c_int),
+        "get_rebuilds": ([V, _LP], C.c_int),
         "set_tuning": ([V, _DP, C.c_double, C.c_int32], C.c_int),
         "set_capacity": ([V, C.c_int32, C.c_int32], C.c_int),
         "create_ensemble": ([C.c_int32, C.POINTER(Options), C.POINTER(V)], C.c_int),
@@ -356,6 +357,12 @@ class Engine:
         it, nx = C.c_int32(0), C.c_int32(0)
         self._check(self._fn("get_iter")(self.h, C.byref(it), C.byref(nx)), "get_iter")
         return it.value, nx.value
+
+    def get_rebuilds(self) -> int:
+        """Re-sorts and candidate-list builds so far, summed over the boxes."""
+        n = C.c_int64(0)
+        self._check(self._fn("get_rebuilds")(self.h, C.byref(n)), "get_rebuilds")
+        return n.value
 
     # ---------------------------------------------------------- profiling
     def set_profiling(self, enable: bool) -> None:

@@ -156,6 +156,32 @@ def aging_lists_scene(seed=31, speed=3.0):
 
 
 
+def drift_past_fixed_scene(seed=9, drift=(2.0, 0.6, 0.0)):
+    """A sparse cloud (no contact at the start) moving as one body at
+    ~2 m/s through 40 fixed spheres, periodic in x, y and z, dt 1e-5 s: the
+    spheres do not move relative to each other until they hit a fixed one,
+    so only their moves relative to the fixed spheres age the candidate
+    lists (tests/test_gpu_parity.py)."""
+    from sdem_amd import scenarios
+
+    sc = scenarios.random_cloud(n=300, seed=seed, spacing=2.0, vel=0.0, omega=0.0, dt=1.0e-5,
+                                periodic=(1, 1, 1), walls="", nfixed=40)
+    st, fx = sc["state"], sc["fixed"]
+    L = np.array(sc["options"]["domain_max"]) - np.array(sc["options"]["domain_min"])
+    p, f = st["posr"], fx["posr"]
+    d = p[:, None, :3] - f[None, :, :3]
+    d -= L * np.round(d / L)
+    gap = np.sqrt((d * d).sum(2)) - p[:, None, 3] - f[None, :, 3]
+    keep = gap.min(1) > 0.2 * f[0, 3]  # no sphere touches a fixed one at the start
+    n = len(st["id"])
+    for k, v in st.items():
+        if isinstance(v, np.ndarray) and v.shape[:1] == (n,):
+            st[k] = np.ascontiguousarray(v[keep])
+    st["lin_vel"][:, :, :] += np.asarray(drift)[None, None, :]
+    st["lin_acc"][:, :, :] = 0.0
+    return sc
+
+
 # finite plane quads for the wall-geometry tests
 def _rect(origin, eu, ev):
     o, eu, ev = (np.asarray(a, float) for a in (origin, eu, ev))

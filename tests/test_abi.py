@@ -56,4 +56,6 @@ def test_create_rejects_bad_options(hip_lib):
 
 def test_null_context_is_rejected(hip_lib):
     assert hip_lib.sdem_step(None, 1) == -1
+    n = ctypes.c_int64(7)
+    assert hip_lib.sdem_get_rebuilds(None, ctypes.byref(n)) == -1
     assert hip_lib.sdem_last_error(None) == b"null context"

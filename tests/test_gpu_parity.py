@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from parity import aging_lists_scene, compare, compare_per_record, quads
+from parity import aging_lists_scene, compare, compare_per_record, drift_past_fixed_scene, quads
 from sdem_amd import native, scenarios
 
 pytestmark = pytest.mark.gpu
@@ -96,6 +96,34 @@ def test_contacts_found_while_lists_age(hip_lib, oracle_lib):
             first = step
     assert first is not None and first > 0, first  # no contact at the start; contacts form later
     assert gpu.get_counts().sum() > 5
+
+
+def test_drift_past_fixed_spheres(hip_lib, oracle_lib):
+    """A sparse cloud moving as one body at ~2 m/s through 40 fixed spheres
+    (parity.drift_past_fixed_scene), skin 0.05 d_max: the lists age with the
+    spheres' moves, which are moves relative to the fixed spheres, and are
+    rebuilt every few steps; every contact with a fixed sphere is found (the
+    first after 14 steps, some 0.3 mm of drift: three skins).  The contact
+    set is the oracle's after every step; the hits rebound at up to ~4 m/s,
+    which amplifies the rounding differences of the two implementations'
+    force sums (other orders of summation), so the state is compared within
+    1e-8."""
+    print("drift past fixed spheres: 262 spheres + 40 fixed, skin 0.05, 80 single steps", flush=True)
+    sc = drift_past_fixed_scene()
+    gpu = native.build_engine(hip_lib, sc, tuning=dict(skin=0.05))
+    ref = native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX)
+    fixed_records = 0
+    for step in range(80):
+        gpu.iterate(1)
+        ref.iterate(1)
+        gc, rc = gpu.get_contacts(), ref.get_contacts()
+        for key in ("kind", "owner", "partner"):
+            assert np.array_equal(gc[key], rc[key]), (step, key)
+        errs = compare(gpu, ref, exact_positions=False, rtol=1e-8)
+        assert not errs, (step, errs)
+        fixed_records += int(gpu.get_counts()[2])
+    assert fixed_records > 50, fixed_records
+    assert gpu.get_rebuilds() > 5, gpu.get_rebuilds()
 
 
 def test_capacity_overflow_is_replayed(hip_lib, oracle_lib):
