@@ -523,13 +523,17 @@ def main():
     eng = native.build_engine(lib, sc, tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN, keep_forces=0))
     eng.iterate(args.warmup)
     log(f"# warm-up {args.warmup} steps done")
+    n_start = eng.num_particles()
     # Timed region: steps queued as HIP graphs, bracketed by device
     # synchronisation on both sides (timed_region).
     wall, P, _ = timed_region(eng, args.steps, log)
     ms_step = wall / args.steps * 1e3
-    value = n * args.steps / wall
+    n_end = eng.num_particles()
+    # spheres that leave through an open side are removed: the particle-steps
+    # done, by the mean of the live counts before and after the timed steps
+    value = 0.5 * (n_start + n_end) * args.steps / wall
     roof, breakdown = kernel_roofline(eng, n, P, min(args.steps, 1000), ms_step)
-    n_now = eng.num_particles()
+    n_now = n_end
     eng.close()
     metric, workload = WORKLOADS[args.workload]
     if args.workload == "settling" and n != 1_000_000:
@@ -557,7 +561,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded initial packing)",
         "config": {"workload": workload, "n": n, "seed": args.seed, "contact_records_per_step": P,
-                   "particles_at_end": n_now},
+                   "particles_timed": {"first_step": n_start, "last_step": n_now}, "particles_at_end": n_now},
         "roofline": roof,
         "step_breakdown_ms": breakdown,
         "whole_step_hbm_frac": roof["whole_step"]["frac"],
