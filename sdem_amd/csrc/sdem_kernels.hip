@@ -717,20 +717,8 @@ constexpr unsigned CL_INDEX = 0x3fffffffu, CL_FIXED = 0x40000000u;
 // +1 for a fixed partner), so that a particle's contact forces are summed in
 // an order that depends neither on the grid, the skin nor the storage order.
 // Entry q of particle k sits at vlist[q*cap + k] (lanes of a wave read
-// consecutive words); vkey holds the sort keys while the list is built.
-__device__ __forceinline__ void vlist_insert(int* __restrict__ vlist, int* __restrict__ vkey, size_t cap,
-                                             int k, int c, unsigned entry, int key) {
-  int b = c - 1;
-  while (b >= 0) {
-    const int kb = vkey[(size_t)b * cap + k];
-    if (kb <= key) break;
-    vkey[(size_t)(b + 1) * cap + k] = kb;
-    vlist[(size_t)(b + 1) * cap + k] = vlist[(size_t)b * cap + k];
-    b--;
-  }
-  vkey[(size_t)(b + 1) * cap + k] = key;
-  vlist[(size_t)(b + 1) * cap + k] = (int)entry;
-}
+// consecutive words); vkey holds the sort keys while a list longer than
+// the LDS part is sorted.
 
 // A particle's list is appended to while the cells are walked (entries
 // only) and sorted once at the end, after its keys are loaded.  Its first
