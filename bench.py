@@ -46,11 +46,11 @@ SKIN = 0.05
 # HBM bytes per k_force launch on the default 1M bed and on that bed
 # settled into contact, measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE and scaled by what the same counters read for a plain 512 MiB
-# copy (x2.00 and x1.00): 84,397 KiB x 2 read + 176,276 KiB written
-# (profiles/k_force_settling_1M.md); 319,031 KiB x 2 + 380,960 KiB
+# copy (x2.00 and x1.00): 84,369 KiB x 2 read + 176,270 KiB written
+# (profiles/k_force_settling_1M.md); 320,778 KiB x 2 + 380,903 KiB
 # (profiles/k_force_contact_bed_1M.md)
-KFORCE_TRAFFIC_1M = (84_397 * 2 + 176_276) * 1024
-KFORCE_TRAFFIC_CONTACT_1M = (319_031 * 2 + 380_960) * 1024
+KFORCE_TRAFFIC_1M = (84_369 * 2 + 176_270) * 1024
+KFORCE_TRAFFIC_CONTACT_1M = (320_778 * 2 + 380_903) * 1024
 # steps per library call in a timed region (a few seconds of work at most,
 # so that a long run still prints progress)
 CHUNK = 20_000
