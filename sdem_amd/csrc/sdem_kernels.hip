@@ -330,12 +330,13 @@ __device__ __forceinline__ int block_box(const int* __restrict__ blk_box) {
 
 // The workgroups of a launch go to the chip's NXCD compute dies in turn
 // (block b to die b % NXCD), and each die has an L2 of its own.  The force
-// kernel numbers its tiles so that each die takes runs of XCD_RUN
-// consecutive tiles: the partners a tile gathers (close in cell order) are
-// then more often in the L2 of the die that gathers them, while all dies
-// still work in one region of memory at a time (one contiguous range per
-// die was slower: 8 streams far apart).  A bijection of [0, gridDim.x); the
-// last incomplete round of runs keeps the plain numbering.
+// kernel, the contact test and the list kernel number their tiles so that
+// each die takes runs of XCD_RUN consecutive tiles: the partners a tile
+// gathers (close in cell order) are then more often in the L2 of the die
+// that gathers them, while all dies still work in one region of memory at
+// a time (one contiguous range per die was slower: 8 streams far apart).
+// A bijection of [0, gridDim.x); the last incomplete round of runs keeps
+// the plain numbering.
 constexpr int NXCD = 8, XCD_RUN = 16;
 __device__ __forceinline__ int xcd_tile() {
   const int nb = gridDim.x, b = blockIdx.x;
@@ -964,13 +965,14 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
                                                const double4* __restrict__ fpk,
                                                const int* __restrict__ fid, const int* __restrict__ fstart) {
   if (!S->rebuild || S->halt) return;
-  const int box = block_box(blk_box);
+  const int tile = xcd_tile();
+  const int box = blk_box[(tile * blockDim.x) / BOX_ALIGN];
   if (!BS[box].rebuild) return;
   const Params& P = PB[box];
   __shared__ VlStage stage;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * blockDim.x + threadIdx.x;
   const bool live = k - P.off < BS[box].n_alive;
   const size_t cap = P.cap;
   const int lcap = P.lcap;
@@ -1222,8 +1224,8 @@ __global__ __launch_bounds__(256) void k_contact(const Params* __restrict__ PB,
                                                  const double4* __restrict__ fpk,
                                                  const int* __restrict__ fid,
                                                  const BoxSlot* rd, BoxSlot* wr) {
-  const int tile = blockIdx.x;
-  const int box = block_box(blk_box);
+  const int tile = xcd_tile();
+  const int box = blk_box[(tile * blockDim.x) / BOX_ALIGN];
   // FAST: summed largest moves since the last rebuild, before the control
   // adds the last step's
   double disp = FAST ? BS[box].disp_acc : 0.0;
