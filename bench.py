@@ -43,6 +43,11 @@ BYTES_PER_CONTACT = 80
 # integrated, but only sdem_get_forces reads them)
 CELL_FACTOR = (0.5, 0.5, 0.5)
 SKIN = 0.05
+# The three-size pack's hierarchical search (cells halve from level to
+# level): at 0.5 its finest level has 0.625 mm cells, ~456M keys in all,
+# and settled on the incline it steps in 2.70 ms; at 1.0 (1.25 mm, ~57M
+# keys) in 2.16 ms (0.75: 2.28 ms)
+POLY_CELL_FACTOR = (1.0, 1.0, 1.0)
 # HBM bytes per k_force launch on the default 1M bed and on that bed
 # settled into contact, measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE and scaled by what the same counters read for a plain 512 MiB
@@ -86,6 +91,11 @@ def parse():
     ap.add_argument("--surrogate-window", choices=("contact", "free"), default="contact",
                     help="surrogate workload: train and judge on the bed collapsed into contact by the DEM "
                          "first (contact) or on the loose pack in free fall (free)")
+    ap.add_argument("--poly-settle", action="store_true",
+                    help="polydisperse workload: settle the pack onto the incline first (POLY_SETTLE), then "
+                         "time it in contact with the configuration's own options")
+    ap.add_argument("--poly-state", default=None,
+                    help=".npz of the settled polydisperse pack: read if it exists, else written after the settle")
     ap.add_argument("--boxes", type=int, default=256)
     ap.add_argument("--ensemble-settle", type=int, default=0,
                     help="ensemble workload: settle the boxes this many steps first (dt 1e-6 s)")
@@ -173,6 +183,34 @@ def cpu_baseline(args):
 # the settle: 60,000 steps at twice gravity (the fall from the loose pack),
 # then 12,000 at gravity, all with the configuration's own physics and dt
 CONTACT_SETTLE = ((60_000, 2.0), (12_000, 1.0))
+# The 4M three-size pack settled onto its incline: steps and gravity scale
+# at dt 5e-6 s (the 1M bed's time step; the configuration's own 2e-6 s
+# would take 2.5 times the steps to let the 0.56 m column fall)
+POLY_SETTLE = ((60_000, 2.0), (12_000, 1.0))
+POLY_SETTLE_DT = 5.0e-6
+
+
+def settled_poly(sc, log, state_path=None):
+    """The polydisperse pack settled onto the incline by the GPU path
+    (`scenarios.settle`, POLY_SETTLE at POLY_SETTLE_DT); with `state_path`
+    the settled state and contact records are read from / written to that
+    .npz file."""
+    from sdem_amd import scenarios
+
+    if state_path and os.path.exists(state_path):
+        z = np.load(state_path)
+        sc["state"] = {k[3:]: z[k] for k in z.files if k.startswith("st_")}
+        sc["contacts"] = {k[3:]: z[k] for k in z.files if k.startswith("ct_")}
+        return sc, 0.0
+    t0 = time.perf_counter()
+    for steps, g in POLY_SETTLE:
+        sc = scenarios.settle(sc, steps, dt=POLY_SETTLE_DT, gravity_scale=g, chunk=4000, log=log,
+                              tuning=dict(cell_factor=POLY_CELL_FACTOR, skin=SKIN))
+    t_settle = time.perf_counter() - t0
+    if state_path:
+        np.savez(state_path, **{"st_" + k: v for k, v in sc["state"].items()},
+                 **{"ct_" + k: v for k, v in sc["contacts"].items()})
+    return sc, t_settle
 
 
 def settled_bed(n, seed, log, state_path=None):
@@ -515,12 +553,15 @@ def main():
     elif args.workload == "polydisperse":
         n = 4_000_000
         sc = scenarios.polydisperse(n=n, seed=args.seed)
+        if args.poly_settle:
+            sc, poly_settle_s = settled_poly(sc, log, args.poly_state)
     else:
         n = args.n
         sc = scenarios.settling(n=n, seed=args.seed)
     log(f"# generated {n} spheres in {time.perf_counter() - t0:.1f}s")
     lib = native.load_hip_library()
-    eng = native.build_engine(lib, sc, tuning=dict(cell_factor=CELL_FACTOR, skin=SKIN, keep_forces=0))
+    cell_factor = POLY_CELL_FACTOR if args.workload == "polydisperse" else CELL_FACTOR
+    eng = native.build_engine(lib, sc, tuning=dict(cell_factor=cell_factor, skin=SKIN, keep_forces=0))
     eng.iterate(args.warmup)
     log(f"# warm-up {args.warmup} steps done")
     n_start = eng.num_particles()
@@ -536,6 +577,10 @@ def main():
     n_now = n_end
     eng.close()
     metric, workload = WORKLOADS[args.workload]
+    if args.workload == "polydisperse" and args.poly_settle:
+        metric += ", settled onto the incline"
+        workload += (f", settled onto the incline first ({', '.join(f'{k} steps at {g:g} g' for k, g in POLY_SETTLE)}"
+                     f", dt {POLY_SETTLE_DT:g} s, {poly_settle_s:.0f} s), then timed with the configuration's options")
     if args.workload == "settling" and n != 1_000_000:
         side = 0.2 * (n / 1.0e6) ** 0.5  # scenarios.settling scales the xy area with n
         metric = f"particle-steps/s, settling bed of {n} spheres"
@@ -561,6 +606,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded initial packing)",
         "config": {"workload": workload, "n": n, "seed": args.seed, "contact_records_per_step": P,
+                   "tuning": {"cell_factor": list(cell_factor), "skin_d_max": SKIN},
                    "particles_timed": {"first_step": n_start, "last_step": n_now}, "particles_at_end": n_now},
         "roofline": roof,
         "step_breakdown_ms": breakdown,
