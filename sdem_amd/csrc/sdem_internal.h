@@ -170,6 +170,10 @@ struct BoxState {
   int wall_prev;           // the step before refreshed the near-wall lists
   int n_far;               // ... of them with all gaps over half the skin (packed
                            // from the end of the box's slots down)
+  int n_unlisted;          // dense layout: live particles without candidates
+  int dense;               // the candidate lists are indexed by the particle's
+                           // own slot (cell order; most particles have
+                           // candidates), not packed
   double disp_acc;         // sum over steps since the box's last rebuild of the
                            // largest move of one of its particles: bounds every
                            // displacement since then

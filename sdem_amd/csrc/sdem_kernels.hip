@@ -401,7 +401,13 @@ __device__ bool box_control(const Params& P, int iter, BoxState& B, const BoxSlo
   const bool rebuild = forced || ndel > 0 || !(disp <= P.rebuild_disp);
   B.prev_rebuild = B.rebuild;
   B.rebuild = rebuild ? 1 : 0;
-  if (rebuild) B.n_act = B.n_far = 0;  // the list kernel packs the box's particles with candidates anew
+  if (rebuild) {
+    // the list kernel lays the lists out anew: in cell order when nearly
+    // every particle had candidates at the last rebuild, else packed
+    const int listed = B.dense ? B.n_alive - B.n_unlisted : B.n_act + B.n_far;
+    B.dense = (B.n_alive > 0 && listed >= B.n_alive - B.n_alive / 10) ? 1 : 0;
+    B.n_act = B.n_far = B.n_unlisted = 0;
+  }
   B.disp_acc = rebuild ? 0.0 : disp;
   const int next = B.next_wall_update;
   B.n_stored = B.n_alive;
@@ -974,6 +980,7 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   const int lane = tid & 63;
   const int k = tile * blockDim.x + threadIdx.x;
   const bool live = k - P.off < BS[box].n_alive;
+  const bool dense = BS[box].dense != 0;  // read before this kernel's atomics on the box's state
   const size_t cap = P.cap;
   const int lcap = P.lcap;
   int c = 0;
@@ -1031,21 +1038,38 @@ __global__ __launch_bounds__(256) void k_vlist(const Params* __restrict__ PB, co
   vlist_sort(stage, tid, vlist, vkey, cap, k, c < lcap ? c : lcap, pid, fid);
   }  // live
   const double ri_live = ri;
-  // The particles with candidates, packed per box (one atomic per wave; the
-  // order does not enter any result): the contact test runs over these
-  // only, with (particle, count, smallest gap) and the entries indexed by
-  // the packed slot t.  Those whose smallest gap is under half the skin go
+  // The contact test runs over slots t with (particle, count, smallest gap)
+  // in act[t] and the entries in cand[q*cap + t]; the layout does not enter
+  // any result.  Packed (most boxes): the particles with candidates only,
+  // one atomic per wave, those whose smallest gap is under half the skin
   // from slot P.off up, the others from the box's last slot down, so that
   // the waves of the latter skip as a whole until the box has moved a
-  // quarter of the skin.  The particles without candidates have no contact
-  // until the next rebuild.
+  // quarter of the skin.  Dense (nearly every particle had candidates at
+  // the last rebuild): slot t = k for every live particle, so that the
+  // contact test walks the particles in cell order (a particle without
+  // candidates has count 0 and an infinite gap).  The particles without
+  // candidates have no contact until the next rebuild.
   const bool has = live && c > 0;
   // the gap less a margin for the roundings of this and the contact test
   // (both are within a few units in the last place of ri + rj)
   const double gstore = gmin - 1.0e-9 * ri_live;
+  if (live && !has) ccnt[k] = 0;
+  if (dense) {
+    // the live particles without candidates are counted (few in a dense box)
+    const unsigned long long bu = __ballot(live && !has);
+    if (bu && lane == __ffsll((long long)bu) - 1) atomicAdd(&BS[box].n_unlisted, __popcll(bu));
+    if (k == P.off) BS[box].n_act = BS[box].n_alive;
+    if (!live) return;
+    act[k] = make_int4(k, c, __double2loint(gstore), __double2hiint(gstore));
+    const int nq = c < lcap ? c : lcap;
+    if (c <= VL_LDS || lcap <= VL_LDS)  // the list never left LDS
+      for (int q = 0; q < nq; q++) cand[(size_t)q * cap + k] = stage.ent[q * VL_T + tid];
+    else
+      for (int q = 0; q < nq; q++) cand[(size_t)q * cap + k] = vlist[(size_t)q * cap + k];
+    return;
+  }
   const bool far = has && gstore > 0.5 * P.skin, near = has && !far;
   const unsigned long long bn = __ballot(near), bfar = __ballot(far);
-  if (live && !has) ccnt[k] = 0;
   if (!(bn | bfar)) return;
   const int ln = bn ? __ffsll((long long)bn) - 1 : 0, lf = bfar ? __ffsll((long long)bfar) - 1 : 0;
   int tn = 0, tf = 0;
