@@ -51,11 +51,11 @@ POLY_CELL_FACTOR = (1.0, 1.0, 1.0)
 # HBM bytes per k_force launch on the default 1M bed and on that bed
 # settled into contact, measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE and scaled by what the same counters read for a plain 512 MiB
-# copy (x2.00 and x1.00): 84,369 KiB x 2 read + 176,270 KiB written
-# (profiles/k_force_settling_1M.md); 320,778 KiB x 2 + 380,903 KiB
+# copy (x2.00 and x1.00): 84,370 KiB x 2 read + 176,270 KiB written
+# (profiles/k_force_settling_1M.md); 321,137 KiB x 2 + 380,930 KiB
 # (profiles/k_force_contact_bed_1M.md)
-KFORCE_TRAFFIC_1M = (84_369 * 2 + 176_270) * 1024
-KFORCE_TRAFFIC_CONTACT_1M = (320_778 * 2 + 380_903) * 1024
+KFORCE_TRAFFIC_1M = (84_370 * 2 + 176_270) * 1024
+KFORCE_TRAFFIC_CONTACT_1M = (321_137 * 2 + 380_930) * 1024
 # steps per library call in a timed region (a few seconds of work at most,
 # so that a long run still prints progress)
 CHUNK = 20_000
