@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16* __restrict__ A, cons
 // products run in the same k order, with the same roundings, as k_gemm's
 // three launches on pack_cols inputs, so both paths give the same bits.
 // ---------------------------------------------------------------------------
-constexpr int FR = 128, FNT = 512, FKC = 32, FXK = FKC + 8, FHK = FUSED_H + 8;
+constexpr int FR = 128, FNT = 512, FKC = 32, FXK = FKC + 8, FHK = FUSED_H + 8, FPD = 3;
 
 struct FusedStage {
   __bf16 X[FR][FXK];       // input rows, chunk of FKC columns
@@ -209,10 +209,12 @@ __global__ __launch_bounds__(FNT, 2) void k_mlp_fused(FusedMLP a) {
     for (int r = 0; r < 16; r++) acc[i][r] = 0.0f;
 
   // per thread and chunk: 2 float4 of input (rows u/8, columns (u%8)*4) and
-  // 8 bf16 of W1^T (row u/4, columns (u%4)*8)
-  float4 xr[2];
-  bf16x8 wr;
-  auto load_chunk = [&](int c) {
+  // 8 bf16 of W1^T (row u/4, columns (u%4)*8).  The input rows are gathers
+  // (an edge's sender and receiver rows): FPD chunks are in flight at once,
+  // in registers, so that a block waits for one round trip per FPD chunks.
+  float4 xr[FPD][2];
+  bf16x8 wr[FPD];
+  auto load_chunk = [&](int c, float4 (&x)[2], bf16x8& wv) {
     const int k0 = c * FKC;
     const FusedSeg sg = a.seg[k0 / FUSED_H];
     const int c0 = k0 % FUSED_H;
@@ -220,29 +222,36 @@ __global__ __launch_bounds__(FNT, 2) void k_mlp_fused(FusedMLP a) {
       const int u = q * FNT + t;
       const int r = u >> 3, cc = (u & 7) * 4;
       const int gr = r0 + r;
-      xr[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      x[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (gr < a.R) {
         const int src = sg.idx ? sg.idx[gr] : gr;
-        xr[q] = *(const float4*)(sg.x + (size_t)src * FUSED_H + c0 + cc);
+        x[q] = *(const float4*)(sg.x + (size_t)src * FUSED_H + c0 + cc);
       }
     }
-    wr = *(const bf16x8*)(a.w1t + (size_t)(t >> 2) * K1 + k0 + (t & 3) * 8);
+    wv = *(const bf16x8*)(a.w1t + (size_t)(t >> 2) * K1 + k0 + (t & 3) * 8);
   };
-  load_chunk(0);
-  for (int c = 0; c < nch; c++) {
-    FusedStage& S = U.st[c & 1];
-    for (int q = 0; q < 2; q++) {
-      const int u = q * FNT + t;
-      const int r = u >> 3, cc = (u & 7) * 4;
-      S.X[r][cc] = (__bf16)xr[q].x;
-      S.X[r][cc + 1] = (__bf16)xr[q].y;
-      S.X[r][cc + 2] = (__bf16)xr[q].z;
-      S.X[r][cc + 3] = (__bf16)xr[q].w;
+#pragma unroll
+  for (int p = 0; p < FPD; p++)
+    if (p < nch) load_chunk(p, xr[p], wr[p]);
+  for (int cb = 0; cb < nch; cb += FPD) {
+#pragma unroll
+    for (int p = 0; p < FPD; p++) {
+      const int c = cb + p;
+      if (c >= nch) break;
+      FusedStage& S = U.st[c & 1];
+      for (int q = 0; q < 2; q++) {
+        const int u = q * FNT + t;
+        const int r = u >> 3, cc = (u & 7) * 4;
+        S.X[r][cc] = (__bf16)xr[p][q].x;
+        S.X[r][cc + 1] = (__bf16)xr[p][q].y;
+        S.X[r][cc + 2] = (__bf16)xr[p][q].z;
+        S.X[r][cc + 3] = (__bf16)xr[p][q].w;
+      }
+      *(bf16x8*)&S.W[t >> 2][(t & 3) * 8] = wr[p];
+      __syncthreads();
+      if (c + FPD < nch) load_chunk(c + FPD, xr[p], wr[p]);
+      fused_mma(&S.X[0][0], FXK, &S.W[0][0], FXK, FKC, wm, wn, lane, acc);
     }
-    *(bf16x8*)&S.W[t >> 2][(t & 3) * 8] = wr;
-    __syncthreads();
-    if (c + 1 < nch) load_chunk(c + 1);
-    fused_mma(&S.X[0][0], FXK, &S.W[0][0], FXK, FKC, wm, wn, lane, acc);
   }
   fused_act(acc, a.b1, Hs, wm, wn, lane);
   __syncthreads();  // layer 1 done everywhere: the staging area is free, a1 complete
