@@ -711,20 +711,50 @@ int oracle_set_state(oracle_ctx *c, int n, const int *id, const int *ptype, cons
 #define RV(c, i, l) ((c)->rvel[(size_t)((i)-1) * (c)->rsize + ((l)-1)])
 #define RA(c, i, l) ((c)->racc[(size_t)((i)-1) * (c)->rsize + ((l)-1)])
 
-static int local_index_of(const oracle_ctx *c, int id) {
-  for (int i = 1; i <= c->nlocal; i++)
-    if (c->id[i] == id) return i;
+/* (id, local index) pairs sorted by id, so that a record's owner is found
+   by bisection: the ids are unique, and a scan per record is quadratic in
+   the size of a settled pack (millions of records) */
+typedef struct {
+  int id, idx;
+} id_index;
+
+static int cmp_id_index(const void *a, const void *b) {
+  const int x = ((const id_index *)a)->id, y = ((const id_index *)b)->id;
+  return (x > y) - (x < y);
+}
+
+static int local_index_of(const id_index *t, int n, int id) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (t[mid].id == id) return t[mid].idx;
+    if (t[mid].id < id) lo = mid + 1;
+    else hi = mid - 1;
+  }
   return 0;
 }
 
 /* Add_RestartCntctlink-style install of history records (status 2) */
 int oracle_set_contacts(oracle_ctx *c, int64_t m, const int *kind, const int *owner,
                         const int *partner, const double *td, const double *vr) {
+  id_index *t = (id_index *)malloc(sizeof(id_index) * (size_t)(c->nlocal > 0 ? c->nlocal : 1));
+  if (!t) return fail(c, SDEM_E_CAPACITY, "out of host memory");
+  for (int i = 1; i <= c->nlocal; i++) {
+    t[i - 1].id = c->id[i];
+    t[i - 1].idx = i;
+  }
+  qsort(t, (size_t)c->nlocal, sizeof(id_index), cmp_id_index);
   for (int64_t k = 0; k < m; k++) {
-    int pid = local_index_of(c, owner[k]);
-    if (pid == 0) return fail(c, SDEM_E_ARG, "owner id not present");
+    int pid = local_index_of(t, c->nlocal, owner[k]);
+    if (pid == 0) {
+      free(t);
+      return fail(c, SDEM_E_ARG, "owner id not present");
+    }
     int ni = c->next_insert;
-    if (ni == 0) return fail(c, SDEM_E_CAPACITY, "contact container is full");
+    if (ni == 0) {
+      free(t);
+      return fail(c, SDEM_E_CAPACITY, "contact container is full");
+    }
     c->next_insert = -c->next[ni];
     c->id_j[ni] = partner[k];
     c->next[ni] = c->bucket[pid];
@@ -734,6 +764,7 @@ int oracle_set_contacts(oracle_ctx *c, int64_t m, const int *kind, const int *ow
     c->tandel[ni] = R3(td[3 * k], td[3 * k + 1], td[3 * k + 2]);
     c->vrel[ni] = vr[k];
   }
+  free(t);
   return SDEM_OK;
 }
 

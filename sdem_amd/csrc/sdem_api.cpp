@@ -383,6 +383,9 @@ int make_grid(sdem_ctx* c, const Box& b, double cell_len0, double rmax, int64_t 
     double L = b.o.domain_max[d] - b.o.domain_min[d];
     double cell_len = cell_len0 * c->cell_factor[d];
     G->origin[d] = b.o.domain_min[d];
+    // the cell count is checked before it is converted to an int (a tiny
+    // cell factor would overflow it)
+    if (!(L / cell_len <= (double)(1 << 30))) return c->fail(SDEM_E_ARG, "search grid too large");
     if (b.o.is_periodic[d]) {
       int n = (int)std::floor(L / cell_len);
       // the search range of one particle must not wrap onto itself

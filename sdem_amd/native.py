@@ -159,6 +159,16 @@ def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def cell_factors(cell_factor) -> np.ndarray:
+    """The three per-axis cell factors sdem_set_tuning reads: one number for
+    every axis, or one per axis (any other length is an error, not a read
+    past the array)."""
+    a = _f64(cell_factor)
+    if a.ndim > 1 or a.size not in (1, 3):
+        raise ValueError(f"cell_factor: one number or three, not {a.size}")
+    return np.ascontiguousarray(np.broadcast_to(a.reshape(-1), (3,)), dtype=np.float64)
+
+
 def _i32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.int32)
 
@@ -369,7 +379,7 @@ class Engine:
         self._check(self._fn("set_profiling")(self.h, int(bool(enable))), "set_profiling")
 
     def set_tuning(self, cell_factor=None, skin=-1.0, keep_forces=-1) -> None:
-        cf = None if cell_factor is None else _f64(cell_factor)
+        cf = None if cell_factor is None else cell_factors(cell_factor)
         self._check(self._fn("set_tuning")(self.h, _dptr(cf), float(skin), int(keep_forces)),
                     "set_tuning")
 

@@ -59,3 +59,14 @@ def test_null_context_is_rejected(hip_lib):
     n = ctypes.c_int64(7)
     assert hip_lib.sdem_get_rebuilds(None, ctypes.byref(n)) == -1
     assert hip_lib.sdem_last_error(None) == b"null context"
+
+
+def test_cell_factors_are_three_per_call():
+    """sdem_set_tuning reads three cell factors: the binding broadcasts one
+    number to every axis and refuses any other length rather than hand the
+    library a shorter array."""
+    assert native.cell_factors(0.5).tolist() == [0.5, 0.5, 0.5]
+    assert native.cell_factors((0.5, 0.75, 1.0)).tolist() == [0.5, 0.75, 1.0]
+    for bad in ((1.0, 2.0), (1.0, 2.0, 3.0, 4.0), [[1.0, 2.0, 3.0]]):
+        with pytest.raises(ValueError):
+            native.cell_factors(bad)

@@ -142,3 +142,16 @@ def test_runaway_particle_is_an_error(hip_lib):
     assert err.value.code == native.SDEM_E_STATE
     assert "blown up" in str(err.value)
     g.close()
+
+
+def test_too_fine_a_grid_is_an_error(hip_lib):
+    """A cell factor so small that the cell count of an axis would overflow
+    is refused when the grid is made (before any kernel runs), with
+    SDEM_E_ARG, instead of a grid built from a wrapped count."""
+    print("grid of 1e-12 cell factor: expecting SDEM_E_ARG", flush=True)
+    sc = scenarios.random_cloud(n=60, seed=12)
+    with pytest.raises(native.SdemError) as err:
+        g = native.build_engine(hip_lib, sc, tuning=dict(cell_factor=(1.0e-12, 1.0, 1.0)))
+        g.iterate(1)
+    assert err.value.code == native.SDEM_E_ARG
+    assert "search grid too large" in str(err.value)
