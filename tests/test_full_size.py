@@ -15,7 +15,10 @@ times, for the few steps the oracle can take in seconds at that size:
   forces at the full grid), then stepped by both sides; and on the HIP path
   alone, the same pack under two search tunings, which must give the same
   bits (the contact set is decided exactly every step);
-* configuration 3 from its own start, over the three-level grid.
+* configuration 3 from its own start, over the three-level grid;
+* configuration 4 (256 independent sand boxes, 3.76M spheres, linear and
+  Hertz-Mindlin boxes with their own materials) settled into contact on
+  the HIP path first, every box against an oracle context of its own.
 
 The contact set is compared exactly, the state and history within the
 stated tolerance (the first step with exact positions).
@@ -103,3 +106,29 @@ def test_polydisperse_4m_from_its_start(hip_lib, oracle_lib):
     sc = scenarios.polydisperse()
     _log(f"configuration 3 from its start: {len(sc['state']['id'])} spheres, 1 + 1 steps")
     _step_and_compare(sc, hip_lib, oracle_lib, ((1, True, 1e-12), (1, False, 1e-11)))
+
+
+def test_ensemble_256_boxes_in_contact(hip_lib, oracle_lib):
+    _log("configuration 4, 256 boxes: building the scenarios")
+    scs = scenarios.ensemble()
+    _log("configuration 4: settling on the HIP path (60k steps at dt 1e-6 s and 2 g)")
+    scs = scenarios.settle(scs, 60_000, dt=1.0e-6, gravity_scale=2.0, chunk=10_000, log=_log)
+    for sc in scs:
+        sc["options"]["dt"] = 1.0e-6  # at 1e-5 s the linear boxes' contacts last one to six steps
+    n = sum(len(sc["state"]["id"]) for sc in scs)
+    _log(f"configuration 4 in contact: {len(scs)} boxes, {n} spheres; 1 + 2 steps, each box against the oracle")
+    ens = native.build_ensemble(hip_lib, scs)
+    refs = [native.build_engine(oracle_lib, sc, prefix=oracle.PREFIX) for sc in scs]
+    for k, exact, rtol in ((1, True, 1e-12), (2, False, 1e-11)):
+        ens.iterate(k)
+        tot = np.zeros(4, np.int64)
+        for b, ref in enumerate(refs):
+            ref.iterate(k)
+            ens.select(b)
+            errs = compare(ens, ref, exact_positions=exact, rtol=rtol)
+            assert not errs, (b, k, errs)
+            c = ref.get_counts()
+            assert ens.get_counts().tolist() == c.tolist(), b
+            tot += c
+        _log(f"  after {k} more: counts over the boxes {tot.tolist()}")
+    assert tot[0] > n // 2 and tot[3] > 0, tot
