@@ -256,7 +256,8 @@ struct sdem_ctx {
   bool use_graph = true;
   // graphs of one and of two steps, without / with the speculative fast
   // path (single box: no sort and list kernels), by starting buffer
-  hipGraphExec_t gexec[2][4][2] = {};  // [fast][1, 2, 8 or 32 steps][buffer]
+  hipGraphExec_t gexec[2][2][4][2] = {};  // [force occupancy 2/3][fast][1, 2, 8 or 32 steps][buffer]
+  int occ3 = 0;                            // the force kernel at three waves per SIMD (Dev::occ3)
   bool spec_ready = false;  // the next step may try the fast path
   int spec_chunk = 8;       // steps queued per fast batch (adapted)
   int spec_full = 1;        // steps queued per full batch between fast ones: grows
@@ -289,13 +290,14 @@ int hip_fail(sdem_ctx* c, hipError_t e, const char* where) {
 }
 
 void invalidate_graphs(sdem_ctx* c) {
-  for (auto& a : c->gexec)
-    for (auto& b : a)
-      for (auto& g : b)
-        if (g) {
-          (void)hipGraphExecDestroy(g);
-          g = nullptr;
-        }
+  for (auto& o : c->gexec)
+    for (auto& a : o)
+      for (auto& b : a)
+        for (auto& g : b)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
   c->spec_ready = false;
 }
 
@@ -824,6 +826,7 @@ static int upload_params(sdem_ctx* c) {
   D.fpk = d_fpk; D.fid = d_fid; D.ftype = d_ftype; D.fstart = d_fstart;
   D.bs_contact = c->bs_contact; D.bs_force = c->bs_force;
   D.pim = c->boxes[0].o.pi_method; D.prim = c->boxes[0].o.pri_method;
+  D.occ3 = c->occ3;
   c->params_ready = true;
   return SDEM_OK;
 }
@@ -1236,7 +1239,7 @@ static hipError_t enqueue_step(sdem_ctx* c, hipEvent_t* ev, bool fast = false) {
 // graphs: each graph launch costs several microseconds more than a kernel
 // inside a graph.
 static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out) {
-  hipGraphExec_t& slot = c->gexec[fast ? 1 : 0][n == 1 ? 0 : (n == 2 ? 1 : (n == 8 ? 2 : 3))][c->cur];
+  hipGraphExec_t& slot = c->gexec[c->occ3][fast ? 1 : 0][n == 1 ? 0 : (n == 2 ? 1 : (n == 8 ? 2 : 3))][c->cur];
   if (slot) {
     *out = slot;
     return hipSuccess;
@@ -1254,6 +1257,34 @@ static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out)
   if (g) (void)hipGraphDestroy(g);
   if (e == hipSuccess) *out = slot;
   return e;
+}
+
+// The force kernel's build for the batch about to run: three waves per
+// SIMD where the spheres touch (0.25 contact records per live sphere or
+// more at the last step, or among the records installed with the state),
+// two otherwise.  Both give the same bits; in contact the third wave's
+// gathers in flight outweigh its spills (k_force 458 -> 409 us on a 1M bed
+// at 2.3 records per sphere, 979 -> 909 us on 256 settled boxes at 1.0),
+// without contacts they do not (63.2 -> 63.9 us on the falling 1M bed).
+static int choose_force_build(sdem_ctx* c) {
+  int64_t rec = 0, alive = 0;
+  if (!c->stepped) {
+    for (const Box& b : c->boxes) {
+      rec += (int64_t)b.ckind.size();
+      alive += b.n;
+    }
+  } else {
+    std::vector<BoxState> bs(c->boxes.size());
+    hipError_t e = hipMemcpy(bs.data(), c->BS, bs.size() * sizeof(BoxState), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(box states)");
+    for (const BoxState& B : bs) {
+      for (int k = 0; k < 4; k++) rec += B.num_cntcts[k];
+      alive += B.n_alive;
+    }
+  }
+  c->occ3 = (alive > 0 && 4 * rec >= alive) ? 1 : 0;
+  c->D.occ3 = c->occ3;
+  return SDEM_OK;
 }
 
 // Steps in batches.  With graphs and a single box, steps are queued on the
@@ -1276,6 +1307,7 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
   int remaining = nsteps, retries = 0;
   while (remaining > 0) {
     if (int rc = prepare(c)) return rc;
+    if (int rc = choose_force_build(c)) return rc;
     const bool fast = spec_mode && c->spec_ready;
     const int batch = !spec_mode ? remaining : std::min(remaining, fast ? c->spec_chunk : c->spec_full);
     std::vector<hipEvent_t> evs;

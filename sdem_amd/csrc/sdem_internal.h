@@ -227,6 +227,7 @@ struct Dev {
   const int *fid, *ftype, *fstart;
   int bs_contact, bs_force;  // block sizes (divisors of BOX_ALIGN)
   int pim, prim;             // integration methods (the same in every box)
+  int occ3;                  // the force kernel built for three waves per SIMD (spheres in contact)
 };
 
 }  // namespace sdem

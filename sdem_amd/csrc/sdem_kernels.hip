@@ -1804,9 +1804,13 @@ __device__ __forceinline__ Acc acc_zero() {
 // batch after the contact loop: loaded before it, they stay live across
 // the loop and spill at the cap of two waves per SIMD (256 VGPRs), and the
 // spill traffic of every particle costs a streaming step more than the
-// latency they hide.
-template <int PIM, int PRIM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_force(const Params* __restrict__ PB,
+// latency they hide.  OCC, the waves per SIMD the kernel is built for: 2
+// (no spills) for beds with few contacts; 3 (168 VGPRs, ~30 spilled) where
+// spheres touch, whose contact loop is bound by the partner gathers in
+// flight, which a third wave adds more of than its spills cost (the same
+// arithmetic: the same bits).
+template <int PIM, int PRIM, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void k_force(const Params* __restrict__ PB,
                                                const int* __restrict__ blk_box, Buf A, Buf B, Scalars* S,
                                                BoxState* BS,
                                                const int* __restrict__ perm,
@@ -2103,13 +2107,17 @@ hipError_t launch_force(const Dev& D, const Buf& A, const Buf& B, hipStream_t st
                        slot_set(D, A.idx), slot_set(D, B.idx));
   if (mid) (void)hipEventRecord(mid, st);
   const unsigned nf = nblk(D.nslots, D.bs_force);
+#define SDEM_FORCE(P, R, O)                                                                                  \
+  hipLaunchKernelGGL((k_force<P, R, O>), dim3(nf), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS, \
+                     D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx))
   if (D.pim == 2 && D.prim == 2) {
-    hipLaunchKernelGGL((k_force<2, 2>), dim3(nf), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS,
-                       D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
+    if (D.occ3) SDEM_FORCE(2, 2, 3);
+    else SDEM_FORCE(2, 2, 2);
   } else {
-    hipLaunchKernelGGL((k_force<0, 0>), dim3(nf), dim3(D.bs_force), 0, st, D.PB, D.blk_box, A, B, D.S, D.BS,
-                       D.perm, D.clist, D.ccnt, D.fpk, D.fid, D.ftype, slot_set(D, B.idx));
+    if (D.occ3) SDEM_FORCE(0, 0, 3);
+    else SDEM_FORCE(0, 0, 2);
   }
+#undef SDEM_FORCE
   return hipGetLastError();
 }
 
