@@ -52,10 +52,11 @@ POLY_CELL_FACTOR = (1.0, 1.0, 1.0)
 # settled into contact, measured with rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE and scaled by what the same counters read for a plain 512 MiB
 # copy (x2.00 and x1.00): 84,370 KiB x 2 read + 176,270 KiB written
-# (profiles/k_force_settling_1M.md); 321,137 KiB x 2 + 380,930 KiB
-# (profiles/k_force_contact_bed_1M.md)
+# (profiles/k_force_settling_1M.md); 434,902 KiB x 2 + 408,823 KiB for the
+# bed in contact, where the force kernel runs at three waves per SIMD and
+# its spills add to the traffic (profiles/k_force_contact_bed_1M.md)
 KFORCE_TRAFFIC_1M = (84_370 * 2 + 176_270) * 1024
-KFORCE_TRAFFIC_CONTACT_1M = (321_137 * 2 + 380_930) * 1024
+KFORCE_TRAFFIC_CONTACT_1M = (434_902 * 2 + 408_823) * 1024
 # steps per library call in a timed region (a few seconds of work at most,
 # so that a long run still prints progress)
 CHUNK = 20_000
