@@ -1264,11 +1264,18 @@ static hipError_t step_graph(sdem_ctx* c, int n, bool fast, hipGraphExec_t* out)
 // more at the last step, or among the records installed with the state),
 // two otherwise.  Both give the same bits; in contact the third wave's
 // gathers in flight outweigh its spills (k_force 458 -> 409 us on a 1M bed
-// at 2.3 records per sphere, 979 -> 909 us on 256 settled boxes at 1.0),
-// without contacts they do not (63.2 -> 63.9 us on the falling 1M bed).
-static int choose_force_build(sdem_ctx* c) {
+// at 2.3 records per sphere, 979 -> 909 us on 256 settled boxes at 1.0; the
+// step 2 % faster at 0.42), without contacts they do not (63.2 -> 63.9 us on
+// the falling 1M bed).  A single box's totals come with the scalars read
+// after each batch; otherwise the box states are read (once per call: only
+// a single box runs its steps in several batches).
+static int choose_force_build(sdem_ctx* c, const Scalars* last) {
   int64_t rec = 0, alive = 0;
-  if (!c->stepped) {
+  if (last) {
+    // a single box between the batches of a call: the totals k_finish left
+    rec = last->fin_records;
+    alive = last->fin_alive;
+  } else if (!c->stepped) {
     for (const Box& b : c->boxes) {
       rec += (int64_t)b.ckind.size();
       alive += b.n;
@@ -1305,9 +1312,11 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
   const int NE = SDEM_NUM_PHASES + 1;
   const bool spec_mode = !c->prof && c->use_graph && nbox == 1;
   int remaining = nsteps, retries = 0;
+  Scalars last;
+  bool have_last = false;  // the scalars after a batch of this call that did work
   while (remaining > 0) {
     if (int rc = prepare(c)) return rc;
-    if (int rc = choose_force_build(c)) return rc;
+    if (int rc = choose_force_build(c, have_last && nbox == 1 ? &last : nullptr)) return rc;
     const bool fast = spec_mode && c->spec_ready;
     const int batch = !spec_mode ? remaining : std::min(remaining, fast ? c->spec_chunk : c->spec_full);
     std::vector<hipEvent_t> evs;
@@ -1358,6 +1367,10 @@ int sdem_step(sdem_ctx* c, int32_t nsteps) {
     }
     const int ran = batch - s.halted_steps;  // steps that did work, the last one failed if overflow
     const int ok_steps = s.overflow ? ran - 1 : ran;
+    if (ok_steps > 0 && !s.halt) {
+      last = s;
+      have_last = true;
+    }
     for (int k = 0; k < ok_steps && c->prof; k++) {
       for (int q = 0; q < SDEM_NUM_PHASES; q++) {
         float ms = 0.0f;

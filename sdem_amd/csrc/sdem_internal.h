@@ -201,6 +201,8 @@ struct Scalars {
                            // array; 0 once the force kernel wrote the buffer:
                            // rotVel(2) is then linVel(1) of the same buffer
                            // (Prtcl_Integration.f90:64 stores one as the other)
+  int fin_records;         // a single box: contact records and live particles at
+  int fin_alive;           // the last step of the last call's batch (k_finish)
 };
 
 // Device pointers and launch shapes the host hands to the launchers.

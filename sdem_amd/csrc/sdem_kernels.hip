@@ -522,8 +522,9 @@ __global__ __launch_bounds__(64) void k_box(const Params* __restrict__ PB, Scala
 }
 
 // counters of the last step for the host (once per call), one wave per box,
-// from the slot set of the buffer the last step wrote
-__global__ __launch_bounds__(64) void k_finish(const Scalars* S, BoxState* BS, const BoxSlot* bslots) {
+// from the slot set of the buffer the last step wrote; a single box also
+// leaves its totals beside the scalars the host reads after every batch
+__global__ __launch_bounds__(64) void k_finish(Scalars* S, BoxState* BS, const BoxSlot* bslots) {
   if (S->halt || !S->have_parts) return;
   const int box = blockIdx.x;
   const int lane = threadIdx.x;
@@ -537,6 +538,10 @@ __global__ __launch_bounds__(64) void k_finish(const Scalars* S, BoxState* BS, c
   if (lane == 0) {
     for (int q = 0; q < 4; q++) BS[box].num_cntcts[q] = c[q];
     BS[box].n_deleted = nd;
+    if (gridDim.x == 1) {
+      S->fin_records = c[0] + c[1] + c[2] + c[3];
+      S->fin_alive = BS[box].n_alive;
+    }
   }
 }
 
