@@ -90,7 +90,7 @@ def parse():
     ap.add_argument("--dem-every", type=int, default=500,
                     help="surrogate workload: calls between the plain DEM's 10-step takeovers")
     ap.add_argument("--surrogate-window", choices=("contact", "free"), default="contact",
-                    help="surrogate workload: train and judge on the bed collapsed into contact by the DEM "
+                    help="surrogate workload: train and evaluate on the bed collapsed into contact by the DEM "
                          "first (contact) or on the loose pack in free fall (free)")
     ap.add_argument("--poly-settle", action="store_true",
                     help="polydisperse workload: settle the pack onto the incline first (POLY_SETTLE), then "

@@ -130,9 +130,9 @@ def rollout(model: gnn.Model, n: int = 1_000_000, seed: int = 20261203, warm: in
             dem_every: int = 500, dem_steps: int = 10, log=print, lib=None, settle=()) -> Dict:
     """The same settling bed (advanced through the `settle` stages first,
     start_state) three ways from one state after `warm` DEM steps: plain
-    DEM for `steps` steps (the judge), the surrogate for steps / nsub calls
+    DEM for `steps` steps (the reference trajectory), the surrogate for steps / nsub calls
     (with `dem_steps` plain DEM steps after every `dem_every` calls, as
-    judge and fallback), and the ballistic step without the network.
+    check and fallback), and the ballistic step without the network.
     Returns timings and errors against the DEM."""
     lib = lib or native.load_hip_library()
     nsub = model.cfg.nsub
