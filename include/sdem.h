@@ -294,6 +294,12 @@ int sdem_set_capacity(sdem_ctx* c, int32_t hist_slots, int32_t list_slots);
 /* Block sizes of the contact-test and force kernels (64, 128 or 256; no
  * counterpart in the reference, results do not depend on them). */
 int sdem_set_block_sizes(sdem_ctx* c, int32_t contact_block, int32_t force_block);
+
+/* Which build of the force kernel the steps launch: 0 (default) chooses per
+ * batch of steps, three waves per SIMD where the last step had 0.25 contact
+ * records per live particle or more, two otherwise; 2 or 3 fixes it.  No
+ * counterpart in the reference; results do not depend on it. */
+int sdem_set_force_build(sdem_ctx* c, int32_t waves_per_simd);
 int sdem_get_tuning(sdem_ctx* c, double cell_factor[3], double* skin_factor, int32_t* contact_block,
                     int32_t* force_block);
 

@@ -258,6 +258,7 @@ struct sdem_ctx {
   // path (single box: no sort and list kernels), by starting buffer
   hipGraphExec_t gexec[2][2][4][2] = {};  // [force occupancy 2/3][fast][1, 2, 8 or 32 steps][buffer]
   int occ3 = 0;                            // the force kernel at three waves per SIMD (Dev::occ3)
+  int force_build = 0;                     // 0: chosen per batch; 2 or 3: fixed (sdem_set_force_build)
   bool spec_ready = false;  // the next step may try the fast path
   int spec_chunk = 8;       // steps queued per fast batch (adapted)
   int spec_full = 1;        // steps queued per full batch between fast ones: grows
@@ -1290,6 +1291,7 @@ static int choose_force_build(sdem_ctx* c, const Scalars* last) {
     }
   }
   c->occ3 = (alive > 0 && 4 * rec >= alive) ? 1 : 0;
+  if (c->force_build) c->occ3 = c->force_build == 3 ? 1 : 0;
   c->D.occ3 = c->occ3;
   return SDEM_OK;
 }
@@ -1857,6 +1859,14 @@ int sdem_set_tuning(sdem_ctx* c, const double cell_factor[3], double skin_factor
   if (keep_forces >= 0) c->keep_forces = keep_forces ? 1 : 0;
   // new grids and parameters at the next step, which rebuilds the lists
   c->params_ready = false;
+  return SDEM_OK;
+}
+
+int sdem_set_force_build(sdem_ctx* c, int32_t waves_per_simd) {
+  if (!c) return SDEM_E_ARG;
+  if (waves_per_simd != 0 && waves_per_simd != 2 && waves_per_simd != 3)
+    return c->fail(SDEM_E_ARG, "force build: 0 (chosen per batch), 2 or 3 waves per SIMD");
+  c->force_build = waves_per_simd;
   return SDEM_OK;
 }
 

@@ -126,6 +126,7 @@ def _declare(lib: C.CDLL, prefix: str) -> None:
         "get_num_boxes": ([V, _IP], C.c_int),
         "select_box": ([V, C.c_int32], C.c_int),
         "set_block_sizes": ([V, C.c_int32, C.c_int32], C.c_int),
+        "set_force_build": ([V, C.c_int32], C.c_int),
         "get_tuning": ([V, _DP, _DP, _IP, _IP], C.c_int),
         "autotune": ([V, C.c_int32, _IP, _DP], C.c_int),
         "write_restart": ([V, C.c_char_p, C.c_int32], C.c_int),
@@ -386,6 +387,10 @@ class Engine:
     def select(self, box: int) -> None:
         """Address box `box` of an ensemble with the setters and readers."""
         self._check(self._fn("select_box")(self.h, int(box)), "select_box")
+
+    def set_force_build(self, waves_per_simd: int = 0) -> None:
+        """The force kernel's build: 0 chosen by contact density, 2 or 3 waves per SIMD."""
+        self._check(self._fn("set_force_build")(self.h, int(waves_per_simd)), "set_force_build")
 
     def set_block_sizes(self, contact_block=256, force_block=256) -> None:
         self._check(self._fn("set_block_sizes")(self.h, int(contact_block), int(force_block)),

@@ -246,6 +246,29 @@ def test_tuning_does_not_change_bits(hip_lib):
     assert out[0] == out[1] == out[2]
 
 
+def test_force_builds_give_the_same_bits(hip_lib):
+    """The force kernel's builds for two and three waves per SIMD (the host
+    picks one per batch by contact density) run the same arithmetic: on a
+    pack with ~3 contact records per sphere each gives every bit of the
+    other's result, and the automatic choice gives them too."""
+    print("force builds: 2 vs 3 waves per SIMD vs chosen, 40 steps in contact", flush=True)
+    sc = scenarios.random_cloud(n=2000, seed=8, periodic=(1, 0, 0), walls="yYzZ")
+    out = []
+    for build in (2, 3, 0):
+        g = native.build_engine(hip_lib, sc)
+        g.set_force_build(build)
+        g.iterate(40)
+        st, ct = g.get_state(), g.get_contacts()
+        out.append(([st[k].tobytes() for k in sorted(st)], [ct[k].tobytes() for k in sorted(ct)],
+                    g.get_counts().tolist()))
+        g.close()
+    assert out[0] == out[1] == out[2]
+    assert sum(out[0][2]) > 2000, out[0][2]
+    with pytest.raises(native.SdemError):
+        g = native.build_engine(hip_lib, sc)
+        g.set_force_build(4)
+
+
 # the benchmark configurations, shrunk so the oracle finishes in seconds
 CONFIGS = {
     "sandbox_linear": lambda: scenarios.sandbox(n=1500, box=(0.04, 0.04, 0.08)),
